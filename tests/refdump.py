@@ -1,0 +1,41 @@
+"""Readers for the raw arrays written by oracle/_ref/ref_harness and their
+conversion to the 0-based layout of the oracle and the HIP path."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+_SUFFIX = {"f32": np.float32, "f64": np.float64, "i32": np.int32, "u32": np.uint32}
+
+
+def load(dirpath: str, name: str, lev: int | None = None) -> np.ndarray:
+    fn = name if lev is None else f"{name}_lev{lev}"
+    dt = _SUFFIX[name.rsplit(".", 1)[1]]
+    return np.fromfile(os.path.join(dirpath, fn), dtype=dt)
+
+
+def full_rows_from_half(K1, K2, K3, MAP, nox, noy, noz):
+    """Expand the reference's symmetric half storage (Eqn_k1..3 + Node_map,
+    Construct_arrays.c:280-535) to full 27-neighbour rows [nno][3][27][3]."""
+    nno = nox * noy * noz
+    neq = 3 * nno
+    full = np.zeros((nno, 3, 27, 3), dtype=np.float32)
+    Kd = np.stack([K1[:nno * 42], K2[:nno * 42], K3[:nno * 42]], axis=0).reshape(3, nno, 42).transpose(1, 0, 2)
+    M = MAP.reshape(nno, 42)
+    n = np.arange(nno)
+    k, i, j = n % noz, (n // noz) % nox, n // (noz * nox)
+    for s in range(42):
+        eq = M[:, s]
+        ok = eq < neq
+        m = eq[ok] // 3
+        c = eq[ok] % 3
+        nn = n[ok]
+        mk, mi, mj = m % noz, (m // noz) % nox, m // (noz * nox)
+        slot = (mk - k[ok] + 1) + 3 * (mi - i[ok] + 1) + 9 * (mj - j[ok] + 1)
+        for r in range(3):
+            full[nn, r, slot, c] = Kd[ok, r, s]
+            # transpose into the row of the lower node m
+            tslot = 26 - slot
+            full[m, c, tslot, r] = Kd[ok, r, s]
+    return full.reshape(nno, 243)
