@@ -1,0 +1,170 @@
+"""Timestep benchmark of the CitcomCU hot path on one MI355X.
+
+One "step" is one pass of the reference's time loop (Citcom.c:111-161) with
+every stage on the GPU: PG_timestep (SUPG energy step), heat_flux, then
+general_stokes_solver (buoyancy forces, temperature-dependent viscosity,
+stiffness rebuild on all multigrid levels, Uzawa CG around the full
+multigrid velocity solve to accuracy 1e-6).
+
+Default workload: 128x128x64 elements in a 2x2x1 box, 6 multigrid levels,
+Ra = 1e7, eta = exp(-ln(1e4) T) clipped to [1e-4, 1], seeded initial
+temperature.  The step-0 Stokes solve and the warmup steps are untimed.
+
+Prints one JSON line (see the README of the measurement contract in
+DESIGN.md): s/timestep (lower is better), the roofline of the dominant
+kernel (the 8-colour Gauss-Seidel pass) from HIP events on the library's
+stream, and a CPU baseline from the C restatement on the host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from citcom_amd import deck  # noqa: E402
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "s/timestep at 128x128x64 elems: FP32+learned vs FP64 GPU, same 1e-6 residual"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_problem(name):
+    return deck.problem_from_deck({k: str(v) for k, v in deck.workload(name).items()})
+
+
+def cpu_baseline(name, mg_calls_per_step, seconds_budget=30.0):
+    """Time the C restatement (one host thread, the reference's algorithm with
+    its lexicographic smoother) on a bounded sample of the same workload: the
+    stiffness rebuild, one PG step and one multi_grid call; the per-step
+    estimate scales the multi_grid time by the multigrid calls per step that
+    the GPU run measured."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    if not os.path.exists(pyoracle.LIB_PATH):
+        pyoracle.build()
+    p = make_problem(name)
+    o = pyoracle.Oracle(p, smoother=0)
+    T = deck.seeded_temperature(p, seed=1)
+    nno = o.nno()
+    t0 = time.perf_counter()
+    o.set_viscosity(o.visc_from_T(T))
+    t_rebuild = time.perf_counter() - t0
+    F = o.assemble_forces(o.thermal_buoyancy(T))
+    t0 = time.perf_counter()
+    o.pg_timestep(T, np.zeros(nno, np.float32), np.zeros(3 * nno, np.float32))
+    t_pg = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    Fc = F.copy()
+    d = np.zeros_like(F)
+    import ctypes as C
+    pyoracle.lib().orc_multi_grid(o.h, d, Fc, None, 0.0, o.levmax)
+    t_mg = time.perf_counter() - t0
+    est = t_rebuild + t_pg + t_mg * mg_calls_per_step
+    return {"value": est, "unit": "s/timestep", "cores": 1, "kind": "port",
+            "sample": (f"C restatement on 1 host thread at {name} size: stiffness rebuild {t_rebuild:.2f} s + "
+                       f"one PG step {t_pg:.2f} s + one multi_grid call {t_mg:.2f} s x "
+                       f"{mg_calls_per_step:.1f} multigrid calls per step (as counted on the GPU)")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="cfg3")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mixed", action="store_true", help="FP32 multigrid inside FP64 refinement")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU: the path shards by subdomain "
+                                   "(see DESIGN.md), and no multi-GPU figure is produced"}))
+        sys.exit(2)
+
+    import torch  # device sync for the timed region
+
+    from citcom_amd.hip import Citcom
+
+    p = make_problem(args.config)
+    nox, noy, noz = p.dims()
+    log(f"config {args.config}: {nox - 1}x{noy - 1}x{noz - 1} elements, {p.levels} levels")
+    t0 = time.perf_counter()
+    c = Citcom(p, mixed=args.mixed)
+    T = deck.seeded_temperature(p, seed=args.seed) if p.tdepv else deck.perturbed_temperature(p)
+    c.initialize(T)
+    log(f"setup {time.perf_counter() - t0:.1f} s, device memory {c.device_bytes() / 2**30:.2f} GiB")
+    t0 = time.perf_counter()
+    st0 = c.general_stokes_solver()
+    log(f"step-0 Stokes solve {time.perf_counter() - t0:.2f} s, {st0['iterations']} Uzawa iterations")
+    for w in range(args.warmup):
+        r = c.timestep()
+        log(f"warmup {w}: {r['stokes']['iterations']} Uzawa its, {r['stokes']['mg_cycles']} MG calls")
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    recs = []
+    for k in range(args.steps):
+        recs.append(c.timestep())
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    s_per_step = elapsed / args.steps
+    for k, r in enumerate(recs):
+        log(f"step {k}: dt {r['pg']['timestep']:.3e} Nu_top {r['Nu_top']:.5f} Uzawa {r['stokes']['iterations']} "
+            f"MG {r['stokes']['mg_cycles']}")
+    mg_calls = float(np.mean([r["stokes"]["mg_cycles"] for r in recs]))
+
+    # dominant kernel: one more step with HIP events around every colour pass
+    c.profile("gs_colour")
+    c.timestep()
+    n_gs, ms_gs, bytes_gs = c.profile_read()
+    c.profile("matvec")
+    c.timestep()
+    n_mv, ms_mv, bytes_mv = c.profile_read()
+    c.profile(None)
+    avg_ms = ms_gs / max(n_gs, 1)
+    achieved = (bytes_gs / max(n_gs, 1)) / (avg_ms * 1e-3) / 1e9
+    log(f"gs_colour: {n_gs} launches, {ms_gs:.1f} ms total, {achieved:.0f} GB/s; "
+        f"matvec: {n_mv} launches, {ms_mv:.1f} ms, {(bytes_mv / max(ms_mv, 1e-9)) / 1e6:.0f} GB/s")
+
+    out = {
+        "metric": METRIC,
+        "value": s_per_step,
+        "unit": "s/timestep",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * s_per_step,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-mixed" if args.mixed else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.config}: {nox - 1}x{noy - 1}x{noz - 1} elements, {p.levels} MG levels, "
+                               f"Ra={p.rayleigh:g}, accuracy={p.accuracy:g}, plain FP64 vectors with the "
+                               f"reference's FP32 stiffness", "form": "mixed" if args.mixed else "plain",
+                   "uzawa_iterations_per_step": float(np.mean([r['stokes']['iterations'] for r in recs])),
+                   "mg_calls_per_step": mg_calls},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel": "k_gs_colour",
+                     "launches": n_gs, "avg_launch_ms": avg_ms},
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(args.config, mg_calls)
+    print(json.dumps(out), flush=True)
+    c.close()
+
+
+if __name__ == "__main__":
+    main()

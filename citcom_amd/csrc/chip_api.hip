@@ -1,0 +1,659 @@
+// C ABI and host orchestration of libcitcom_hip: the Uzawa loop
+// (solve_Ahat_p_fhat), the full multigrid (solve_del2_u / multi_grid), the
+// SUPG step (PG_timestep) and the per-step stiffness rebuild, all on the
+// device.  The host only keeps loop control: one 8-byte read per multigrid
+// call (the residual test) and one small read per Uzawa iteration (the
+// convergence monitors), as the reference's loops need them.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "chip_kernels.h"
+
+namespace chip {
+
+static thread_local std::string g_err;
+void set_error(const std::string &s) { g_err = s; }
+
+namespace {
+
+// slots of chip_ctx::dots
+enum : int {
+    D_TMP = 0,     // [0..3] scratch reductions
+    D_MG = 4,      // [4] AU.AU, [5] AU.res
+    D_R1Z1 = 6,    // [6], [7] alternate between Uzawa iterations
+    D_S2AH = 8,
+    D_VV = 9,      // [9] V.V, [10] u1.u1
+    D_PP = 11,     // [11] P.P, [12] Ah.Ah, [13] s2.s2
+    D_INIT = 14,   // [14] r1.r1, [15] V.V
+    D_HF = 24,     // [24..27] heat-flux sums
+};
+
+inline hipStream_t S(const chip_ctx *c) { return c->stream; }
+
+int sync_dots(chip_ctx *c) {
+    CHIP_CHECK(hipMemcpyAsync(c->h_dots, c->dots, 32 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CHIP_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+inline float unford(unsigned u) {
+    const unsigned b = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+}
+
+int sync_minmax(chip_ctx *c, float *mn, float *mx) {
+    CHIP_CHECK(hipMemcpyAsync(c->h_minmax, c->minmax, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    CHIP_CHECK(hipStreamSynchronize(c->stream));
+    if (mn) *mn = unford(c->h_minmax[0]);
+    if (mx) *mx = unford(c->h_minmax[1]);
+    return 0;
+}
+
+// ---- profiling of one kernel family -------------------------------------
+struct ProfGuard {
+    chip_ctx *c;
+    bool on;
+    ProfGuard(chip_ctx *cc, int kernel, double bytes) : c(cc), on(cc->prof.kernel == kernel) {
+        if (!on) return;
+        hipEvent_t a, b;
+        (void)hipEventCreate(&a);
+        (void)hipEventCreate(&b);
+        c->prof.ev.emplace_back(a, b);
+        c->prof.bytes.push_back(bytes);
+        (void)hipEventRecord(a, c->stream);
+    }
+    ~ProfGuard() {
+        if (on) (void)hipEventRecord(c->prof.ev.back().second, c->stream);
+    }
+};
+
+int prof_flush(chip_ctx *c) {
+    Prof &pr = c->prof;
+    if (pr.ev.empty()) return 0;
+    CHIP_CHECK(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < pr.ev.size(); i++) {
+        float ms = 0.0f;
+        CHIP_CHECK(hipEventElapsedTime(&ms, pr.ev[i].first, pr.ev[i].second));
+        pr.total_ms += ms;
+        pr.total_bytes += pr.bytes[i];
+        pr.launches++;
+        (void)hipEventDestroy(pr.ev[i].first);
+        (void)hipEventDestroy(pr.ev[i].second);
+    }
+    pr.ev.clear();
+    pr.bytes.clear();
+    return 0;
+}
+
+// algorithmic bytes: 243 FP32 coefficients per node plus the vectors
+// (matvec: read u, write Au; colour pass: read/write d, read F and BI)
+constexpr double MV_BYTES = 243.0 * 4 + 6 * 8;
+constexpr double GS_BYTES = 243.0 * 4 + 9 * 8 + 3 * 8;
+
+void do_matvec(chip_ctx *c, int lev, const double *u, double *Au, const double *rhs = nullptr, double *res = nullptr) {
+    const Level &L = c->L[lev];
+    ProfGuard pg(c, 2, MV_BYTES * L.g.nno);
+    matvec(S(c), L, u, Au, rhs, res);
+}
+
+// gauss_seidel (General_matrix_functions.c:1160-1361), 8-colour; returns Ad = K d
+void do_gs(chip_ctx *c, int lev, double *d, const double *F, double *Ad, int sweeps, int guess) {
+    const Level &L = c->L[lev];
+    if (gs_small_ok(L)) {
+        gs_small(S(c), L, d, F, Ad, sweeps, guess);
+        return;
+    }
+    if (!guess) fill(S(c), d, 0.0, 3L * L.g.nno);
+    for (int cyc = 0; cyc < sweeps; cyc++)
+        for (int col = 0; col < 8; col++) {
+            ProfGuard pg(c, 1, GS_BYTES * (L.g.coff[col + 1] - L.g.coff[col]));
+            gs_colour(S(c), L, col, d, F);
+        }
+    do_matvec(c, lev, d, Ad);
+}
+
+// multi_grid (General_matrix_functions.c:525-653).  F holds the right-hand
+// side on entry and the residual on exit; the correction is left in
+// L[levmax].vel.  Coarse right-hand sides share storage with fl[] (rhs[lev]
+// is a copy of fl[lev] in the reference, and fl[lev] is not read again).
+int multi_grid(chip_ctx *c, double *F, double *residual) {
+    const int levmin = c->levmin, levmax = c->levmax;
+    const chip_problem &p = c->p;
+    Level *L = c->L;
+    auto fl = [&](int lev) { return lev == levmax ? F : L[lev].fl; };
+    for (int lev = levmax; lev > levmin; lev--) project(S(c), L[lev], L[lev - 1], fl(lev), fl(lev - 1));
+    do_gs(c, levmin, L[levmin].vel, fl(levmin), L[levmin].AU, p.v_steps_low, 0);
+    for (int lev = levmin + 1; lev <= levmax; lev++) {
+        interp(S(c), L[lev - 1], L[lev], L[lev - 1].vel, L[lev].vel);
+        for (int Vn = 1; Vn <= p.mg_cycle; Vn++) {
+            for (int dlev = lev; dlev >= levmin + 1; dlev--) {
+                const int cycles = (dlev == levmax) ? p.v_steps_high : p.down_heavy;
+                double *rhs = (dlev == lev) ? fl(dlev) : L[dlev].fl;
+                do_gs(c, dlev, L[dlev].vel, rhs, L[dlev].AU, cycles, dlev == lev);
+                sub(S(c), L[dlev].res, rhs, L[dlev].AU, 3L * L[dlev].g.nno);
+                project(S(c), L[dlev], L[dlev - 1], L[dlev].res, L[dlev - 1].fl);
+            }
+            do_gs(c, levmin, L[levmin].vel, L[levmin].fl, L[levmin].AU, p.v_steps_low, 0);
+            for (int ulev = levmin + 1; ulev <= lev; ulev++) {
+                const int cycles = (ulev == levmax) ? p.v_steps_high : p.up_heavy;
+                const long n = 3L * L[ulev].g.nno;
+                interp(S(c), L[ulev - 1], L[ulev], L[ulev - 1].vel, L[ulev].del);
+                do_gs(c, ulev, L[ulev].del, L[ulev].res, L[ulev].AU, cycles, 1);
+                dots(S(c), c, (int)n, 2, L[ulev].AU, L[ulev].AU, L[ulev].AU, L[ulev].res, nullptr, nullptr, nullptr,
+                     nullptr, c->dots + D_MG);
+                mg_update(S(c), L[ulev].vel, L[ulev].del, ulev == levmax ? L[ulev].res : nullptr, L[ulev].AU,
+                          c->dots + D_MG, n);
+            }
+        }
+    }
+    const long neq = 3L * L[levmax].g.nno;
+    copy(S(c), F, L[levmax].res, neq);
+    dots(S(c), c, (int)neq, 1, F, F, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, c->dots + D_TMP);
+    if (sync_dots(c)) return -1;
+    *residual = std::sqrt(c->h_dots[D_TMP] / neq);
+    return 0;
+}
+
+// solve_del2_u (General_matrix_functions.c:368-520, multigrid branch):
+// d0 = K^-1 F; F is not modified
+int solve_del2_u(chip_ctx *c, double *d0, const double *F, double acc, int *valid, int *count) {
+    const Level &L = c->L[c->levmax];
+    const long neq = 3L * L.g.nno;
+    copy(S(c), c->sr, F, neq);
+    fill(S(c), d0, 0.0, neq);
+    dots(S(c), c, (int)neq, 1, c->sr, c->sr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, c->dots + D_TMP);
+    if (sync_dots(c)) return -1;
+    const double r0 = std::sqrt(c->h_dots[D_TMP] / neq);
+    double residual = r0;
+    acc = std::max(acc, r0 * c->p.accuracy);
+    *valid = (residual < acc) ? 0 : 1;
+    int cnt = 0;
+    while (residual > acc) {
+        if (multi_grid(c, c->sr, &residual)) return -1;
+        axpy_scalar(S(c), d0, c->L[c->levmax].vel, 1.0, neq);
+        cnt++;
+        if (cnt > 10000) {
+            set_error("solve_del2_u: multigrid does not converge");
+            return -1;
+        }
+    }
+    *count = cnt;
+    return 0;
+}
+
+// solve_Ahat_p_fhat (Stokes_flow_Incomp.c:295-497) + v_from_vector (:530-552)
+int uzawa(chip_ctx *c, chip_stokes_stats *st) {
+    const Level &L = c->L[c->levmax];
+    const long neq = 3L * L.g.nno, npno = L.g.nel;
+    const int gneq = (int)neq, gnpno = (int)npno;
+    const double imp = c->p.accuracy;
+    chip_stokes_stats s{};
+    hipStream_t q = S(c);
+
+    dots(q, c, gneq, 1, c->F, c->F, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, c->dots + D_TMP);
+    if (sync_dots(c)) return -1;
+    const double v_res = std::sqrt(c->h_dots[D_TMP] / gneq);
+
+    grad(q, L, c->P, c->Ah);
+    do_matvec(c, c->levmax, c->U, c->u1);
+    init_residual(q, c->Ah, c->F, c->Ah, c->u1, neq);
+    strip(q, L, c->Ah);
+    int valid = 0, cnt = 0;
+    if (solve_del2_u(c, c->u1, c->Ah, imp * v_res, &valid, &cnt)) return -1;
+    s.mg_cycles += cnt;
+    s.inner_solves++;
+    strip(q, L, c->u1);
+    axpy_scalar(q, c->U, c->u1, 1.0, neq);
+    div(q, L, c->U, c->r1);
+    dots(q, c, gnpno, 1, c->r1, c->r1, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, c->dots + D_INIT);
+    dots(q, c, gneq, 1, c->U, c->U, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, c->dots + D_INIT + 1);
+    if (sync_dots(c)) return -1;
+    s.pdotp = std::sqrt(c->h_dots[D_INIT] / gnpno);
+    s.vdotv = std::sqrt(c->h_dots[D_INIT + 1] / gneq);
+    s.incompressibility = s.pdotp / s.vdotv;
+
+    int count = 0;
+    float dpressure = 1.0f, dvelocity = 1.0f;
+    double *r0 = c->r0, *r1 = c->r1, *r2 = c->r2, *z0 = c->z0, *z1 = c->z1, *s1 = c->s1, *s2 = c->s2, *sh;
+    while ((count < c->p.p_iterations) && (dpressure >= imp || dvelocity >= imp)) {
+        double *r1z1 = c->dots + D_R1Z1 + (count & 1), *r0z0 = c->dots + D_R1Z1 + ((count + 1) & 1);
+        uz_z(q, z1, L.BPI, r1, npno);
+        dots(q, c, gnpno, 1, r1, z1, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, r1z1);
+        uz_search(q, s2, z1, s1, r1z1, r0z0, count == 0, npno);
+        grad(q, L, s2, c->Ah);
+        if (solve_del2_u(c, c->u1, c->Ah, imp * v_res, &valid, &cnt)) return -1;
+        s.mg_cycles += cnt;
+        s.inner_solves++;
+        strip(q, L, c->u1);
+        div(q, L, c->u1, c->Ah);
+        dots(q, c, gnpno, 1, s2, c->Ah, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, c->dots + D_S2AH);
+        uz_pupdate(q, r2, r1, c->Ah, c->P, s2, r1z1, c->dots + D_S2AH, valid, npno);
+        uz_vupdate(q, c->U, c->u1, r1z1, c->dots + D_S2AH, valid, neq);
+        div(q, L, c->U, c->Ah);
+        dots(q, c, gneq, 2, c->U, c->U, c->u1, c->u1, nullptr, nullptr, nullptr, nullptr, c->dots + D_VV);
+        dots(q, c, gnpno, 3, c->P, c->P, c->Ah, c->Ah, s2, s2, nullptr, nullptr, c->dots + D_PP);
+        if (sync_dots(c)) return -1;
+        const double *h = c->h_dots;
+        const double alpha = valid ? h[D_R1Z1 + (count & 1)] / h[D_S2AH] : 0.0;
+        double vdotv = h[D_VV], pdotp = h[D_PP];
+        s.incompressibility = std::sqrt((gneq / gnpno) * (1.0e-32 + h[D_PP + 1] / (1.0e-32 + vdotv)));
+        dpressure = (float)(alpha * std::sqrt(h[D_PP + 2] / (1.0e-32 + pdotp)));
+        dvelocity = (float)(alpha * std::sqrt(h[D_VV + 1] / (1.0e-32 + vdotv)));
+        s.vdotv = std::sqrt(vdotv / gneq);
+        s.pdotp = std::sqrt(pdotp / gnpno);
+        count++;
+        sh = s1; s1 = s2; s2 = sh;
+        sh = r0; r0 = r1; r1 = r2; r2 = sh;
+        sh = z0; z0 = z1; z1 = sh;
+    }
+    // keep the rotated buffers for the next call, as the reference's statics
+    c->r0 = r0; c->r1 = r1; c->r2 = r2; c->z0 = z0; c->z1 = z1; c->s1 = s1; c->s2 = s2;
+    v_from_vector(q, L, c->U, c->V);
+    s.iterations = count;
+    s.dpressure = dpressure;
+    s.dvelocity = dvelocity;
+    if (st) *st = s;
+    return prof_flush(c);
+}
+
+int rebuild(chip_ctx *c) {
+    hipStream_t q = S(c);
+    for (int lev = c->levmax; lev > c->levmin; lev--) project_viscosity(q, c->L[lev], c->L[lev - 1]);
+    for (int lev = c->levmin; lev <= c->levmax; lev++) elt_g(q, c->L[lev]);
+    for (int lev = c->levmax; lev >= c->levmin; lev--) node_ks(q, c->L[lev], c->p);
+    for (int lev = c->levmax; lev >= c->levmin; lev--) build_bpi(q, c->L[lev], c->p.precond);
+    CHIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+// PG_timestep (Advection_diffusion.c:251-345)
+int pg_timestep(chip_ctx *c, chip_pg_stats *st) {
+    hipStream_t q = S(c);
+    const Level &L = c->L[c->levmax];
+    const long nno = L.g.nno;
+    float adv, Tint1, Tint = 0.0f;
+    timestep_tmax(q, c, c->V, c->T);
+    if (sync_minmax(c, &adv, &Tint1)) return -1;
+    float timestep;
+    if (c->p.fixed_timestep != 0.0f) timestep = c->p.fixed_timestep;
+    else {
+        const float a = c->p.finetunedt * adv;
+        timestep = (float)(1.0e-32 + (a < c->diff_timestep ? a : c->diff_timestep));
+    }
+    fcopy(q, c->T1, c->T, nno);
+    fcopy(q, c->Tdot1, c->Tdot, nno);
+    float dt_reduced = 1.0f;
+    int last_sub = 1, iredo;
+    do {
+        timestep *= dt_reduced;
+        iredo = 0;
+        predictor(q, L, c->T, c->Tdot, (float)((1.0 - c->p.gamma) * timestep));
+        for (int pass = 0; pass < c->p.temp_iterations; pass++) {
+            pg_residual(q, c, c->T, c->Tdot, c->V, c->Eres);
+            pg_gather_correct(q, c, c->Eres, c->T, c->Tdot, c->p.gamma * timestep);
+        }
+        tmax(q, c, c->T);
+        if (sync_minmax(c, nullptr, &Tint)) return -1;
+        if (Tint / Tint1 > 1.01f) {
+            fcopy(q, c->T, c->T1, nno);
+            fcopy(q, c->Tdot, c->Tdot1, nno);
+            iredo = 1;
+            dt_reduced *= 0.5f;
+            last_sub++;
+        }
+    } while (iredo == 1 && last_sub <= 5);
+    c->elapsed_time += timestep;
+    conform_temperature(q, c, c->T);
+    thermal_buoyancy(q, c, c->T, c->buoy);
+    CHIP_CHECK(hipGetLastError());
+    if (st) {
+        st->timestep = timestep;
+        st->T_interior = Tint;
+        st->sub_iterations = last_sub;
+        st->elapsed_time = c->elapsed_time;
+    }
+    return 0;
+}
+
+int heat_flux_nu(chip_ctx *c, float *nu) {
+    heat_flux(S(c), c, c->T, c->V);
+    if (sync_dots(c)) return -1;
+    const double *h = c->h_dots + D_HF;
+    const float inp[4] = {(float)h[0], (float)h[1], (float)h[2], (float)h[3]};
+    nu[1] = inp[0] / (inp[1] * 4);
+    nu[0] = inp[2] / (inp[3] * 4);
+    return 0;
+}
+
+int stokes_step(chip_ctx *c, chip_stokes_stats *st) {
+    hipStream_t q = S(c);
+    const Level &L = c->L[c->levmax];
+    conform_velocity(q, L, c->U);
+    assemble_forces(q, c, c->buoy, c->F);
+    // get_system_viscosity + construct_stiffness_B_matrix (Drive_solvers.c:123-126)
+    if (!c->built || c->p.visc_update) {
+        c->built = true;
+        visc_from_T(q, c, c->T, c->L[c->levmax].EVI);
+        if (rebuild(c)) return -1;
+    }
+    return uzawa(c, st);
+}
+
+}  // namespace
+}  // namespace chip
+
+using namespace chip;
+
+extern "C" {
+
+const char *chip_last_error(void) { return chip::g_err.c_str(); }
+
+int chip_create(const chip_problem *p, chip_ctx **out) {
+    *out = nullptr;
+    if (!p || !p->xc || !p->yc || !p->zc) {
+        set_error("chip_create: missing problem or coordinates");
+        return -1;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        set_error("chip_create: no HIP device");
+        return -2;
+    }
+    chip_ctx *c = new chip_ctx();
+    c->p = *p;
+    const int f = 1 << (p->levels - 1);
+    c->xc.assign(p->xc, p->xc + p->mgunitx * f + 1);
+    c->yc.assign(p->yc, p->yc + p->mgunity * f + 1);
+    c->zc.assign(p->zc, p->zc + p->mgunitz * f + 1);
+    c->p.xc = c->xc.data();
+    c->p.yc = c->yc.data();
+    c->p.zc = c->zc.data();
+    if (upload_constants()) {
+        set_error("chip_create: constant upload failed");
+        delete c;
+        return -1;
+    }
+    if (hipStreamCreate(&c->stream) != hipSuccess) {
+        set_error("chip_create: stream");
+        delete c;
+        return -1;
+    }
+    c->own_stream = true;
+    if (build_mesh(c)) {
+        chip_destroy(c);
+        return -1;
+    }
+    *out = c;
+    return 0;
+}
+
+void chip_destroy(chip_ctx *c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void *ptr : c->allocs) (void)hipFree(ptr);
+    if (c->h_dots) (void)hipHostFree(c->h_dots);
+    if (c->h_minmax) (void)hipHostFree(c->h_minmax);
+    for (auto &e : c->prof.ev) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int chip_set_stream(chip_ctx *c, void *hip_stream) {
+    if (!hip_stream) return 0;
+    if (c->own_stream) {
+        CHIP_CHECK(hipStreamSynchronize(c->stream));
+        CHIP_CHECK(hipStreamDestroy(c->stream));
+        c->own_stream = false;
+    }
+    c->stream = (hipStream_t)hip_stream;
+    return 0;
+}
+
+void *chip_get_stream(chip_ctx *c) { return (void *)c->stream; }
+
+static int resolve(chip_ctx *c, const char *which, int lev, void **ptr, size_t *elem, long *count) {
+    if (lev < 0) lev = c->levmax;
+    if (lev < c->levmin || lev > c->levmax) {
+        set_error("level out of range");
+        return -1;
+    }
+    const Level &L = c->L[lev];
+    const Geo &gf = c->L[c->levmax].g;
+    const std::string w(which);
+    if (w == "T") { *ptr = c->T; *elem = 4; *count = gf.nno; }
+    else if (w == "Tdot") { *ptr = c->Tdot; *elem = 4; *count = gf.nno; }
+    else if (w == "U") { *ptr = c->U; *elem = 8; *count = 3L * gf.nno; }
+    else if (w == "P") { *ptr = c->P; *elem = 8; *count = gf.nel; }
+    else if (w == "F") { *ptr = c->F; *elem = 8; *count = 3L * gf.nno; }
+    else if (w == "V") { *ptr = c->V; *elem = 4; *count = 3L * gf.nno; }
+    else if (w == "buoyancy") { *ptr = c->buoy; *elem = 4; *count = gf.nno; }
+    else if (w == "EVI") { *ptr = L.EVI; *elem = 4; *count = 8L * L.g.nel; }
+    else if (w == "BI") { *ptr = L.BI; *elem = 8; *count = 3L * L.g.nno; }
+    else if (w == "BPI") { *ptr = L.BPI; *elem = 8; *count = L.g.nel; }
+    else if (w == "EG") { *ptr = L.EG; *elem = 4; *count = 24L * L.g.nel; }
+    else if (w == "MASS") { *ptr = L.MASS; *elem = 4; *count = L.g.nno; }
+    else if (w == "NODE") { *ptr = L.NODE; *elem = 4; *count = L.g.nno; }
+    else if (w == "GNX") { *ptr = L.GNX; *elem = 4; *count = (long)GNX_SZ * L.g.nel; }
+    else if (w == "GDA") { *ptr = L.GDA; *elem = 4; *count = (long)GDA_SZ * L.g.nel; }
+    else if (w == "ECO") { *ptr = L.ECO; *elem = 4; *count = (long)ECO_SZ * L.g.nel; }
+    else if (w == "TWW") { *ptr = L.TWW; *elem = 4; *count = (long)TWW_SZ * L.g.nel; }
+    else {
+        set_error("unknown array " + w);
+        return -1;
+    }
+    return 0;
+}
+
+int chip_upload(chip_ctx *c, const char *which, int lev, const void *host, long count) {
+    void *ptr;
+    size_t elem;
+    long n;
+    if (resolve(c, which, lev, &ptr, &elem, &n)) return -1;
+    if (count != n) {
+        set_error(std::string("chip_upload ") + which + ": size mismatch");
+        return -1;
+    }
+    CHIP_CHECK(hipMemcpyAsync(ptr, host, n * elem, hipMemcpyHostToDevice, c->stream));
+    CHIP_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int chip_download(chip_ctx *c, const char *which, int lev, void *host, long count) {
+    const std::string w(which);
+    if (w == "K") {
+        // device [slot*9 + r*3 + c][pitch] colour order -> host [n][r*81 + slot*3 + c]
+        const Level &L = c->L[lev < 0 ? c->levmax : lev];
+        const Geo &g = L.g;
+        if (count != (long)g.nno * KROW) {
+            set_error("chip_download K: size mismatch");
+            return -1;
+        }
+        std::vector<float> dev((size_t)KROW * g.pitch);
+        CHIP_CHECK(hipMemcpyAsync(dev.data(), L.K, dev.size() * 4, hipMemcpyDeviceToHost, c->stream));
+        CHIP_CHECK(hipStreamSynchronize(c->stream));
+        float *out = (float *)host;
+        for (int col = 0; col < 8; col++)
+            for (int q = g.coff[col]; q < g.coff[col + 1]; q++) {
+                const int r0 = q - g.coff[col], kk = r0 % g.cz[col], t = r0 / g.cz[col];
+                const int k = 2 * kk + (col & 1), i = 2 * (t % g.cx[col]) + ((col >> 1) & 1),
+                          j = 2 * (t / g.cx[col]) + ((col >> 2) & 1);
+                const int n = k + g.noz * (i + g.nox * j);
+                for (int s = 0; s < 27; s++)
+                    for (int r = 0; r < 3; r++)
+                        for (int cc = 0; cc < 3; cc++)
+                            out[(size_t)n * KROW + r * 81 + s * 3 + cc] = dev[(size_t)(s * 9 + r * 3 + cc) * g.pitch + q];
+            }
+        return 0;
+    }
+    void *ptr;
+    size_t elem;
+    long n;
+    if (resolve(c, which, lev, &ptr, &elem, &n)) return -1;
+    if (count != n) {
+        set_error(std::string("chip_download ") + which + ": size mismatch");
+        return -1;
+    }
+    CHIP_CHECK(hipMemcpyAsync(host, ptr, n * elem, hipMemcpyDeviceToHost, c->stream));
+    CHIP_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int chip_get_system_viscosity(chip_ctx *c) {
+    visc_from_T(c->stream, c, c->T, c->L[c->levmax].EVI);
+    CHIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int chip_construct_stiffness_B_matrix(chip_ctx *c) {
+    if (rebuild(c)) return -1;
+    CHIP_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int chip_apply(chip_ctx *c, const char *op, int lev, const double *in, const double *in2, double *out, double *out2,
+               int arg) {
+    if (lev < c->levmin || lev > c->levmax) {
+        set_error("chip_apply: level out of range");
+        return -1;
+    }
+    const std::string o(op);
+    const Level &L = c->L[lev];
+    const long nv = 3L * L.g.nno, np = L.g.nel;
+    hipStream_t q = c->stream;
+    // scratch on the level's MG vectors (and the coarser level's for transfers)
+    double *a = L.del, *b = L.res, *d = L.AU;
+    auto up = [&](double *dst, const double *src, long n) {
+        return hipMemcpyAsync(dst, src, n * 8, hipMemcpyHostToDevice, q);
+    };
+    auto down = [&](double *dst, const double *src, long n) {
+        return hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToHost, q);
+    };
+    if (o == "matvec") {
+        CHIP_CHECK(up(a, in, nv));
+        do_matvec(c, lev, a, b);
+        CHIP_CHECK(down(out, b, nv));
+    } else if (o == "gs0" || o == "gs1") {
+        const int guess = (o == "gs1");
+        if (guess) CHIP_CHECK(up(a, in, nv));
+        CHIP_CHECK(up(b, in2, nv));
+        do_gs(c, lev, a, b, d, arg, guess);
+        CHIP_CHECK(down(out, a, nv));
+        CHIP_CHECK(down(out2, d, nv));
+    } else if (o == "project") {
+        if (lev == c->levmin) {
+            set_error("project: no coarser level");
+            return -1;
+        }
+        const Level &C = c->L[lev - 1];
+        CHIP_CHECK(up(a, in, nv));
+        project(q, L, C, a, C.AU);
+        CHIP_CHECK(down(out, C.AU, 3L * C.g.nno));
+    } else if (o == "interp") {
+        if (lev == c->levmin) {
+            set_error("interp: no coarser level");
+            return -1;
+        }
+        const Level &C = c->L[lev - 1];
+        CHIP_CHECK(up(C.AU, in, 3L * C.g.nno));
+        interp(q, C, L, C.AU, a);
+        CHIP_CHECK(down(out, a, nv));
+    } else if (o == "div") {
+        CHIP_CHECK(up(a, in, nv));
+        div(q, L, a, b);
+        CHIP_CHECK(down(out, b, np));
+    } else if (o == "grad") {
+        CHIP_CHECK(up(a, in, np));
+        grad(q, L, a, b);
+        CHIP_CHECK(down(out, b, nv));
+    } else if (o == "strip") {
+        CHIP_CHECK(up(a, in, nv));
+        strip(q, L, a);
+        CHIP_CHECK(down(out, a, nv));
+    } else {
+        set_error("chip_apply: unknown op " + o);
+        return -1;
+    }
+    CHIP_CHECK(hipGetLastError());
+    CHIP_CHECK(hipStreamSynchronize(q));
+    return prof_flush(c);
+}
+
+int chip_solve_del2_u(chip_ctx *c, const double *F, double *d0, double acc, int *valid, int *cycles) {
+    const long neq = 3L * c->L[c->levmax].g.nno;
+    CHIP_CHECK(hipMemcpyAsync(c->Ah, F, neq * 8, hipMemcpyHostToDevice, c->stream));
+    int v = 0, n = 0;
+    if (solve_del2_u(c, c->u1, c->Ah, acc, &v, &n)) return -1;
+    CHIP_CHECK(hipMemcpyAsync(d0, c->u1, neq * 8, hipMemcpyDeviceToHost, c->stream));
+    CHIP_CHECK(hipStreamSynchronize(c->stream));
+    if (valid) *valid = v;
+    if (cycles) *cycles = n;
+    return prof_flush(c);
+}
+
+int chip_solve_constrained_flow_iterative(chip_ctx *c, chip_stokes_stats *st) { return uzawa(c, st); }
+
+int chip_PG_timestep(chip_ctx *c, chip_pg_stats *st) { return pg_timestep(c, st); }
+
+int chip_thermal_buoyancy(chip_ctx *c) {
+    thermal_buoyancy(c->stream, c, c->T, c->buoy);
+    CHIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int chip_assemble_forces(chip_ctx *c) {
+    assemble_forces(c->stream, c, c->buoy, c->F);
+    CHIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int chip_heat_flux(chip_ctx *c, float *nu) { return heat_flux_nu(c, nu); }
+
+int chip_general_stokes_solver(chip_ctx *c, chip_stokes_stats *st) {
+    const int r = stokes_step(c, st);
+    return r;
+}
+
+int chip_timestep(chip_ctx *c, chip_pg_stats *pg, chip_stokes_stats *st, float *nu) {
+    c->solution_cycles++;
+    if (pg_timestep(c, pg)) return -1;
+    if (nu && heat_flux_nu(c, nu)) return -1;
+    return stokes_step(c, st);
+}
+
+int chip_profile_enable(chip_ctx *c, const char *kernel, int on) {
+    const std::string k(kernel ? kernel : "");
+    if (prof_flush(c)) return -1;
+    if (!on) {
+        c->prof.kernel = 0;
+        return 0;
+    }
+    c->prof.kernel = (k == "gs_colour") ? 1 : (k == "matvec") ? 2 : 0;
+    c->prof.launches = 0;
+    c->prof.total_ms = c->prof.total_bytes = 0.0;
+    if (!c->prof.kernel) {
+        set_error("chip_profile_enable: unknown kernel " + k);
+        return -1;
+    }
+    return 0;
+}
+
+int chip_profile_read(chip_ctx *c, long *launches, double *total_ms, double *bytes) {
+    if (prof_flush(c)) return -1;
+    if (launches) *launches = c->prof.launches;
+    if (total_ms) *total_ms = c->prof.total_ms;
+    if (bytes) *bytes = c->prof.total_bytes;
+    return 0;
+}
+
+long chip_device_bytes(chip_ctx *c) { return c->dev_bytes; }
+
+}  // extern "C"

@@ -1,0 +1,129 @@
+// Internal data model of libcitcom_hip: one struct per multigrid level, all
+// arrays resident in HBM.  Node/element numbering follows the reference
+// (z fastest); the stiffness is stored as full 27-neighbour rows of 3x3
+// FP32 blocks, coefficient-major in 8-colour order so that both the matvec
+// and every colour pass of the Gauss-Seidel smoother stream contiguous,
+// 256-byte aligned runs of each coefficient.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/citcom_hip.h"
+
+namespace chip {
+
+constexpr int MAXL = 12;
+constexpr int KROW = 243;   // 3 rows x 27 neighbour nodes x 3 columns
+constexpr int GNX_SZ = 216; // Shape_function_dx: vpt[3*8*8] + ppt[3*8]
+constexpr int GDA_SZ = 11;  // Shape_function_dA: vpt[9] + ppt[2]
+constexpr int ECO_SZ = 13;  // COORD: centre[4] size[4] recip_size[4] area
+constexpr int TWW_SZ = 9;   // FNODE: node[9]
+constexpr int EG_SZ = 24;   // EG: g[24][1]
+
+// node flag bits (global_defs.h:65-89)
+constexpr unsigned VBX = 0x2, VBZ = 0x4, VBY = 0x8;
+constexpr unsigned TBX = 0x10, TBZ = 0x20, TBY = 0x40;
+constexpr unsigned FBX = 0x80000, FBZ = 0x100000, FBY = 0x200000;
+
+// Index bookkeeping of one level, passed to kernels by value.
+struct Geo {
+    int nox, noy, noz;
+    int elx, ely, elz;
+    int nno, nel;
+    int pitch;     // stride of one stiffness coefficient run (nno rounded up)
+    int coff[9];   // first colour-ordered index of each colour, coff[8] = nno
+    int cz[8], cx[8]; // nodes of colour c along z and x
+};
+
+struct Level {
+    Geo g;
+    // operator
+    float *K = nullptr;      // [243][pitch], colour order
+    double *BI = nullptr;    // [3 nno]
+    double *BPI = nullptr;   // [nel]
+    float *EG = nullptr;     // [nel][24]
+    float *EVI = nullptr;    // [nel][8]
+    // geometry
+    float *GNX = nullptr;    // [nel][216]
+    float *GDA = nullptr;    // [nel][11]
+    float *ECO = nullptr;    // [nel][13]
+    float *TWW = nullptr;    // [nel][9]
+    float *MASS = nullptr;   // [nno]
+    unsigned *NODE = nullptr;// [nno]
+    float *iw = nullptr;     // interpolation weights n1,n2 per fine index: x[2*nox], z[2*noz], y[2*noy]
+    float *VN = nullptr;     // [nno] nodal viscosity scratch
+    // multigrid work vectors [3 nno]
+    double *vel = nullptr, *res = nullptr, *fl = nullptr, *del = nullptr, *AU = nullptr;
+    // FP32 copies for the mixed-precision cycle
+    float *velf = nullptr, *resf = nullptr, *flf = nullptr, *delf = nullptr, *AUf = nullptr;
+};
+
+struct Prof {
+    int kernel = 0;          // 0 off, 1 gs_colour, 2 matvec
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    std::vector<double> bytes;
+    long launches = 0;
+    double total_ms = 0.0, total_bytes = 0.0;
+};
+
+}  // namespace chip
+
+struct chip_ctx {
+    chip_problem p{};
+    std::vector<float> xc, yc, zc;
+    int levmin = 0, levmax = 0;
+    chip::Level L[chip::MAXL];
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    // finest-level state
+    double *U = nullptr, *P = nullptr, *F = nullptr;
+    float *T = nullptr, *Tdot = nullptr, *V = nullptr, *buoy = nullptr;
+    float *T1 = nullptr, *Tdot1 = nullptr;
+    double *Eres = nullptr;  // [nel][8] element residual / element force scratch
+    float *TB = nullptr;     // [nno] boundary temperature (TB[3])
+    float *expansivity = nullptr, *diffusivity = nullptr; // [noz]
+    float *xc_d = nullptr, *yc_d = nullptr;               // finest coordinates per axis
+    int *mat = nullptr;      // [nel] material group (1-based ids)
+    float *H = nullptr;      // [noz] horizontal average
+    float *hf = nullptr;     // [nno] nodal heat flux
+    // Uzawa vectors
+    double *r0 = nullptr, *r1 = nullptr, *r2 = nullptr, *z0 = nullptr, *z1 = nullptr;
+    double *s1 = nullptr, *s2 = nullptr, *Ah = nullptr, *u1 = nullptr;
+    // inner solve vectors
+    double *sr = nullptr, *sd = nullptr;
+    // reductions
+    double *partial = nullptr;   // [4 * 1024]
+    unsigned *counter = nullptr; // [1]
+    double *dots = nullptr;      // [32] results, read by later kernels
+    double *h_dots = nullptr;    // pinned host mirror
+    unsigned *minmax = nullptr;  // [4] float bits for min/max reductions
+    unsigned *h_minmax = nullptr;
+    float diff_timestep = 0.0f;
+    float elapsed_time = 0.0f;
+    int solution_cycles = 0;
+    bool built = false;      // operators constructed at least once
+    chip::Prof prof;
+    long dev_bytes = 0;
+    std::vector<void *> allocs;
+};
+
+namespace chip {
+// setup (chip_setup.cpp)
+int build_mesh(chip_ctx *c);
+// errors
+void set_error(const std::string &s);
+}  // namespace chip
+
+#define CHIP_CHECK(expr)                                                          \
+    do {                                                                          \
+        hipError_t _e = (expr);                                                   \
+        if (_e != hipSuccess) {                                                   \
+            chip::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));   \
+            return -1;                                                            \
+        }                                                                         \
+    } while (0)

@@ -1,0 +1,68 @@
+// Launch wrappers of the gfx950 kernels (chip_kernels.hip).  All take the
+// work stream and device pointers; none allocates.
+#pragma once
+
+#include "chip_common.h"
+
+namespace chip {
+
+// master-element constants into __constant__ memory
+int upload_constants();
+
+// ---- stiffness rebuild --------------------------------------------------
+void visc_from_T(hipStream_t s, const chip_ctx *c, const float *T, float *evi);
+void project_viscosity(hipStream_t s, const Level &fine, const Level &coarse);
+void elt_g(hipStream_t s, const Level &L);
+void node_ks(hipStream_t s, const Level &L, const chip_problem &p);
+void build_bpi(hipStream_t s, const Level &L, int precond);
+
+// ---- vector operators ---------------------------------------------------
+// Au = K u (strip BC rows); optionally res = rhs - Au as well
+void matvec(hipStream_t s, const Level &L, const double *u, double *Au, const double *rhs, double *res);
+// one colour pass of the node-block Gauss-Seidel
+void gs_colour(hipStream_t s, const Level &L, int colour, double *d, const double *F);
+// every sweep of a small level in one workgroup, then Ad = K d
+void gs_small(hipStream_t s, const Level &L, double *d, const double *F, double *Ad, int sweeps, int guess);
+bool gs_small_ok(const Level &L);
+void project(hipStream_t s, const Level &fine, const Level &coarse, const double *AU, double *AD);
+void interp(hipStream_t s, const Level &coarse, const Level &fine, const double *AD, double *AU);
+void div(hipStream_t s, const Level &L, const double *U, double *divU);
+void grad(hipStream_t s, const Level &L, const double *P, double *gradP);
+void strip(hipStream_t s, const Level &L, double *v);
+
+// ---- reductions: out[m] = sum a_m . b_m over n entries (m < nd <= 4) -----
+void dots(hipStream_t s, chip_ctx *c, int n, int nd, const double *a0, const double *b0, const double *a1,
+          const double *b1, const double *a2, const double *b2, const double *a3, const double *b3, double *out);
+
+// ---- elementwise ------------------------------------------------------------
+void fill(hipStream_t s, double *v, double val, long n);
+void copy(hipStream_t s, double *dst, const double *src, long n);
+void axpy_scalar(hipStream_t s, double *y, const double *x, double a, long n);   // y += a x
+void sub(hipStream_t s, double *r, const double *a, const double *b, long n);    // r = a - b
+// vel += alpha del, alpha = sc[1]/sc[0]; if res: res -= alpha AU
+void mg_update(hipStream_t s, double *vel, const double *del, double *res, const double *AU, const double *sc, long n);
+// Uzawa pieces (Stokes_flow_Incomp.c:393-464)
+void uz_z(hipStream_t s, double *z1, const double *BPI, const double *r1, long n);
+void uz_search(hipStream_t s, double *s2, const double *z1, const double *s1, const double *num, const double *den,
+               int first, long n);
+void uz_pupdate(hipStream_t s, double *r2, const double *r1, const double *Ah, double *P, const double *s2,
+                const double *num, const double *den, int valid, long n);
+void uz_vupdate(hipStream_t s, double *V, const double *u1, const double *num, const double *den, int valid, long n);
+void init_residual(hipStream_t s, double *Ah, const double *F, const double *gradP, const double *KV, long n);
+void v_from_vector(hipStream_t s, const Level &L, const double *U, float *V);
+void conform_velocity(hipStream_t s, const Level &L, double *U);
+
+// ---- energy --------------------------------------------------------------
+// per-element min of the advective step and Tmax of T into c->minmax
+void timestep_tmax(hipStream_t s, chip_ctx *c, const float *V, const float *T);
+void tmax(hipStream_t s, chip_ctx *c, const float *T);
+void predictor(hipStream_t s, const Level &L, float *T, float *Tdot, float mult);
+void pg_residual(hipStream_t s, const chip_ctx *c, const float *T, const float *Tdot, const float *V, double *Eres);
+void pg_gather_correct(hipStream_t s, const chip_ctx *c, const double *Eres, float *T, float *Tdot, float mult);
+void conform_temperature(hipStream_t s, const chip_ctx *c, float *T);
+void thermal_buoyancy(hipStream_t s, const chip_ctx *c, const float *T, float *buoy);
+void assemble_forces(hipStream_t s, const chip_ctx *c, const float *buoy, double *F);
+void heat_flux(hipStream_t s, chip_ctx *c, const float *T, const float *V);
+void fcopy(hipStream_t s, float *dst, const float *src, long n);
+
+}  // namespace chip

@@ -1,0 +1,1383 @@
+// gfx950 kernels of the CitcomCU hot path.
+//
+// Every scatter-add of the reference (element loops adding into nodes) is
+// written here as a per-node gather over the node's elements in increasing
+// element number, which reproduces the reference's summation order and
+// rounding points, so these kernels agree bit for bit with the reference
+// wherever the reference itself is deterministic.  Built with
+// -ffp-contract=off for that reason.  The stiffness operator is bandwidth
+// bound (2 flops per 4-byte coefficient): coefficient streams are read once,
+// coalesced, 256-byte aligned; vectors are gathered through L2.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+
+#include "chip_kernels.h"
+
+namespace chip {
+
+__constant__ double cNv[64];   // N.vpt   (Shape_functions.c:104-110)
+__constant__ double cNp[8];    // N.ppt at the element centre
+__constant__ double cMv[16];   // M.vpt   (2-D surface shape functions)
+__constant__ double cMxv[32];  // Mx.vpt
+__constant__ int cOffz[9] = {0, 0, 0, 0, 0, 1, 1, 1, 1};
+__constant__ int cOffx[9] = {0, 0, 1, 1, 0, 0, 1, 1, 0};
+__constant__ int cOffy[9] = {0, 0, 0, 1, 1, 0, 0, 1, 1};
+// local node of offsets (dz, dx, dy): cLoc[dz + 2 dx + 4 dy]
+__constant__ int cLoc[8] = {1, 5, 2, 6, 4, 8, 3, 7};
+
+namespace {
+constexpr int TPB = 256;
+inline int nblk(long n, int tpb = TPB) { return (int)std::max<long>(1, (n + tpb - 1) / tpb); }
+}  // namespace
+
+int upload_constants() {
+    const double B = 0.57735026918962576451;
+    const float b = (float)B;
+    const float gp[9][3] = {{0, 0, 0}, {-b, -b, -b}, {b, -b, -b}, {b, b, -b}, {-b, b, -b},
+                            {-b, -b, b}, {b, -b, b}, {b, b, b}, {-b, b, b}};
+    const float sp[5][2] = {{0, 0}, {-b, -b}, {b, -b}, {b, b}, {-b, b}};
+    const int bbv[3][9] = {{0, 1, 2, 2, 1, 1, 2, 2, 1}, {0, 1, 1, 2, 2, 1, 1, 2, 2}, {0, 1, 1, 1, 1, 2, 2, 2, 2}};
+    auto lp = [](int p, double y) { return p == 1 ? 0.5 * (1 - y) : (p == 2 ? 0.5 * (1 + y) : 0.0); };
+    auto ld = [](int p) { return p == 1 ? -0.5 : (p == 2 ? 0.5 : 0.0); };
+    double Nv[64], Np[8], Mv[16], Mxv[32];
+    for (int i = 1; i <= 8; i++) {
+        for (int j = 1; j <= 8; j++) {
+            double v = 1.0;
+            for (int d = 1; d <= 3; d++) v *= lp(bbv[d - 1][i], gp[j][d - 1]);
+            Nv[8 * (i - 1) + j - 1] = v;
+        }
+        double v = 1.0;
+        for (int d = 1; d <= 3; d++) v *= lp(bbv[d - 1][i], 0.0);
+        Np[i - 1] = v;
+    }
+    // construct_shape_functions, surface part (Shape_functions.c:130-158)
+    for (int j = 1; j <= 4; j++)
+        for (int k = 1; k <= 4; k++) {
+            Mv[4 * (j - 1) + k - 1] = lp(bbv[0][j], sp[k][0]) * lp(bbv[1][j], sp[k][1]);
+            Mxv[4 * (j - 1) + k - 1] = ld(bbv[0][j]) * lp(bbv[1][j], sp[k][1]);
+            Mxv[16 + 4 * (j - 1) + k - 1] = ld(bbv[1][j]) * lp(bbv[0][j], sp[k][0]);
+        }
+    if (hipMemcpyToSymbol(HIP_SYMBOL(cNv), Nv, sizeof Nv) != hipSuccess) return -1;
+    if (hipMemcpyToSymbol(HIP_SYMBOL(cNp), Np, sizeof Np) != hipSuccess) return -1;
+    if (hipMemcpyToSymbol(HIP_SYMBOL(cMv), Mv, sizeof Mv) != hipSuccess) return -1;
+    if (hipMemcpyToSymbol(HIP_SYMBOL(cMxv), Mxv, sizeof Mxv) != hipSuccess) return -1;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// index helpers
+
+__device__ __forceinline__ int nid(const Geo &g, int k, int i, int j) { return k + g.noz * (i + g.nox * j); }
+
+__device__ __forceinline__ void node_kij(const Geo &g, int n, int &k, int &i, int &j) {
+    k = n % g.noz;
+    const int t = n / g.noz;
+    i = t % g.nox;
+    j = t / g.nox;
+}
+
+__device__ __forceinline__ void colour_kij(const Geo &g, int c, int q, int &k, int &i, int &j) {
+    const int r = q - g.coff[c];
+    const int nz = g.cz[c], nx = g.cx[c];
+    const int kk = r % nz, t = r / nz;
+    k = 2 * kk + (c & 1);
+    i = 2 * (t % nx) + ((c >> 1) & 1);
+    j = 2 * (t / nx) + ((c >> 2) & 1);
+}
+
+__device__ __forceinline__ int colour_of(const Geo &g, int q) {
+    int c = 0;
+#pragma unroll
+    for (int m = 1; m < 8; m++) c += (q >= g.coff[m]);
+    return c;
+}
+
+// colour-ordered index of node (k,i,j)
+__device__ __forceinline__ int colour_index(const Geo &g, int k, int i, int j) {
+    const int c = (k & 1) | ((i & 1) << 1) | ((j & 1) << 2);
+    return g.coff[c] + (k >> 1) + g.cz[c] * ((i >> 1) + g.cx[c] * (j >> 1));
+}
+
+// full-row product of the node at colour index q, (k,i,j), with u.  The
+// neighbour order (dy, dx, dz ascending) and the per-block expression are
+// those of the CPU restatement.  Missing neighbours carry zero coefficients
+// and read the node's own entries instead.
+__device__ __forceinline__ void row_product(const Geo &g, const float *__restrict__ K, int q, int k, int i, int j,
+                                            const double *__restrict__ u, double out[3]) {
+    const size_t P = (size_t)g.pitch;
+    const float *kp = K + q;
+    const int self = nid(g, k, i, j);
+    out[0] = out[1] = out[2] = 0.0;
+#pragma unroll
+    for (int dy = -1; dy <= 1; dy++) {
+        const bool oy = (unsigned)(j + dy) < (unsigned)g.noy;
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) {
+            const bool ox = oy && (unsigned)(i + dx) < (unsigned)g.nox;
+#pragma unroll
+            for (int dz = -1; dz <= 1; dz++) {
+                const bool ok = ox && (unsigned)(k + dz) < (unsigned)g.noz;
+                const int m = ok ? self + dz + g.noz * (dx + g.nox * dy) : self;
+                const double u0 = u[3 * m], u1 = u[3 * m + 1], u2 = u[3 * m + 2];
+#pragma unroll
+                for (int r = 0; r < 3; r++) {
+                    const float a0 = kp[0], a1 = kp[P], a2 = kp[2 * P];
+                    kp += 3 * P;
+                    out[r] += a0 * u0 + a1 * u1 + a2 * u2;
+                }
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void strip3(unsigned f, double v[3]) {
+    if (f & VBX) v[0] = 0.0;
+    if (f & VBY) v[1] = 0.0;
+    if (f & VBZ) v[2] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// matvec: n_assemble_del2_u (Element_calculations.c:552-621) with strip
+
+__global__ void __launch_bounds__(TPB) k_matvec(Geo g, const float *__restrict__ K, const unsigned *__restrict__ NODE,
+                                                const double *__restrict__ u, double *__restrict__ Au,
+                                                const double *__restrict__ rhs, double *__restrict__ res) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nno) return;
+    int k, i, j;
+    colour_kij(g, colour_of(g, q), q, k, i, j);
+    double out[3];
+    row_product(g, K, q, k, i, j, u, out);
+    const int n = nid(g, k, i, j);
+    strip3(NODE[n], out);
+    Au[3 * n] = out[0];
+    Au[3 * n + 1] = out[1];
+    Au[3 * n + 2] = out[2];
+    if (res) {
+        res[3 * n] = rhs[3 * n] - out[0];
+        res[3 * n + 1] = rhs[3 * n + 1] - out[1];
+        res[3 * n + 2] = rhs[3 * n + 2] - out[2];
+    }
+}
+
+void matvec(hipStream_t s, const Level &L, const double *u, double *Au, const double *rhs, double *res) {
+    hipLaunchKernelGGL(k_matvec, dim3(nblk(L.g.nno)), dim3(TPB), 0, s, L.g, L.K, L.NODE, u, Au, rhs, res);
+}
+
+// ---------------------------------------------------------------------------
+// Gauss-Seidel: gauss_seidel (General_matrix_functions.c:1160-1361) as eight
+// colour passes; the node-block update t = (F - (K d)_node) * BI, rounded to
+// float as the reference's temp[] is, then d += t.
+
+__device__ __forceinline__ void node_update(const Geo &g, const float *__restrict__ K, const double *__restrict__ BI,
+                                            int q, int k, int i, int j, double *d, const double *__restrict__ F) {
+    double out[3];
+    row_product(g, K, q, k, i, j, d, out);
+    const int n = nid(g, k, i, j);
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const float t = (float)((F[3 * n + r] - out[r]) * BI[3 * n + r]);
+        d[3 * n + r] += t;
+    }
+}
+
+__global__ void __launch_bounds__(TPB) k_gs_colour(Geo g, int c, const float *__restrict__ K,
+                                                   const double *__restrict__ BI, double *d,
+                                                   const double *__restrict__ F) {
+    const int q = g.coff[c] + blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.coff[c + 1]) return;
+    int k, i, j;
+    colour_kij(g, c, q, k, i, j);
+    node_update(g, K, BI, q, k, i, j, d, F);
+}
+
+void gs_colour(hipStream_t s, const Level &L, int colour, double *d, const double *F) {
+    const int n = L.g.coff[colour + 1] - L.g.coff[colour];
+    hipLaunchKernelGGL(k_gs_colour, dim3(nblk(n)), dim3(TPB), 0, s, L.g, colour, L.K, L.BI, d, F);
+}
+
+constexpr int SMALL_TPB = 512;
+constexpr int SMALL_NNO = 6144;
+
+bool gs_small_ok(const Level &L) { return L.g.nno <= SMALL_NNO; }
+
+__global__ void __launch_bounds__(SMALL_TPB) k_gs_small(Geo g, const float *__restrict__ K,
+                                                        const double *__restrict__ BI,
+                                                        const unsigned *__restrict__ NODE, double *d,
+                                                        const double *__restrict__ F, double *__restrict__ Ad,
+                                                        int sweeps, int guess) {
+    const int tid = threadIdx.x;
+    if (!guess) {
+        for (int n = tid; n < 3 * g.nno; n += blockDim.x) d[n] = 0.0;
+        __syncthreads();
+    }
+    for (int cyc = 0; cyc < sweeps; cyc++)
+        for (int c = 0; c < 8; c++) {
+            for (int q = g.coff[c] + tid; q < g.coff[c + 1]; q += blockDim.x) {
+                int k, i, j;
+                colour_kij(g, c, q, k, i, j);
+                node_update(g, K, BI, q, k, i, j, d, F);
+            }
+            __syncthreads();
+        }
+    for (int q = tid; q < g.nno; q += blockDim.x) {
+        int k, i, j;
+        colour_kij(g, colour_of(g, q), q, k, i, j);
+        double out[3];
+        row_product(g, K, q, k, i, j, d, out);
+        const int n = nid(g, k, i, j);
+        strip3(NODE[n], out);
+        Ad[3 * n] = out[0];
+        Ad[3 * n + 1] = out[1];
+        Ad[3 * n + 2] = out[2];
+    }
+}
+
+void gs_small(hipStream_t s, const Level &L, double *d, const double *F, double *Ad, int sweeps, int guess) {
+    hipLaunchKernelGGL(k_gs_small, dim3(1), dim3(SMALL_TPB), 0, s, L.g, L.K, L.BI, L.NODE, d, F, Ad, sweeps, guess);
+}
+
+// ---------------------------------------------------------------------------
+// grid transfers
+
+// project_vector (Solver_multigrid.c:72-159) + strip of the coarse result
+__global__ void __launch_bounds__(TPB) k_project(Geo gf, Geo gc, const float *__restrict__ TWW,
+                                                 const float *__restrict__ MASS, const unsigned *__restrict__ NODE,
+                                                 const double *__restrict__ AU, double *__restrict__ AD) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= gc.nno) return;
+    int k, i, j;
+    node_kij(gc, n, k, i, j);
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int ey = j - 1; ey <= j; ey++) {
+        if ((unsigned)ey >= (unsigned)gc.ely) continue;
+        for (int ex = i - 1; ex <= i; ex++) {
+            if ((unsigned)ex >= (unsigned)gc.elx) continue;
+            for (int ez = k - 1; ez <= k; ez++) {
+                if ((unsigned)ez >= (unsigned)gc.elz) continue;
+                const int a = cLoc[(k - ez) + 2 * (i - ex) + 4 * (j - ey)];
+                const int el = ez + gc.elz * (ex + gc.elx * ey);
+                const int fz = 2 * ez + cOffz[a], fx = 2 * ex + cOffx[a], fy = 2 * ey + cOffy[a];
+                double av0 = 0.0, av1 = 0.0, av2 = 0.0;
+#pragma unroll
+                for (int b = 1; b <= 8; b++) {
+                    const int fn = nid(gf, fz + cOffz[b], fx + cOffx[b], fy + cOffy[b]);
+                    av0 += AU[3 * fn];
+                    av1 += AU[3 * fn + 1];
+                    av2 += AU[3 * fn + 2];
+                }
+                const float w = TWW[(size_t)el * TWW_SZ + a];
+                acc[0] += w * av0;
+                acc[1] += w * av1;
+                acc[2] += w * av2;
+            }
+        }
+    }
+    const float m = MASS[n];
+    acc[0] = acc[0] * m;
+    acc[1] = acc[1] * m;
+    acc[2] = acc[2] * m;
+    strip3(NODE[n], acc);
+    AD[3 * n] = acc[0];
+    AD[3 * n + 1] = acc[1];
+    AD[3 * n + 2] = acc[2];
+}
+
+void project(hipStream_t s, const Level &fine, const Level &coarse, const double *AU, double *AD) {
+    hipLaunchKernelGGL(k_project, dim3(nblk(coarse.g.nno)), dim3(TPB), 0, s, fine.g, coarse.g, coarse.TWW,
+                       coarse.MASS, coarse.NODE, AU, AD);
+}
+
+// interp_vector + un_inject_vector (Solver_multigrid.c:173-298, 581-634):
+// injection, then linear fills along x, z, y in the reference's order, + strip
+__device__ __forceinline__ void line_x(const Geo &gc, const float *wx, const double *AD, int k, int i, int j,
+                                       double v[3]) {
+    // k, j even fine indices; i fine x index
+    if (!(i & 1)) {
+        const int cn = nid(gc, k >> 1, i >> 1, j >> 1);
+        v[0] = AD[3 * cn];
+        v[1] = AD[3 * cn + 1];
+        v[2] = AD[3 * cn + 2];
+    } else {
+        const float n1 = wx[2 * i], n2 = wx[2 * i + 1];
+        const int c1 = nid(gc, k >> 1, (i - 1) >> 1, j >> 1), c2 = nid(gc, k >> 1, (i + 1) >> 1, j >> 1);
+#pragma unroll
+        for (int d = 0; d < 3; d++) v[d] = n1 * AD[3 * c1 + d] + n2 * AD[3 * c2 + d];
+    }
+}
+
+__device__ __forceinline__ void plane_xz(const Geo &gc, const float *wx, const float *wz, const double *AD, int k, int i,
+                                         int j, double v[3]) {
+    if (!(k & 1)) {
+        line_x(gc, wx, AD, k, i, j, v);
+    } else {
+        double a[3], b[3];
+        line_x(gc, wx, AD, k - 1, i, j, a);
+        line_x(gc, wx, AD, k + 1, i, j, b);
+        const float n1 = wz[2 * k], n2 = wz[2 * k + 1];
+#pragma unroll
+        for (int d = 0; d < 3; d++) v[d] = n1 * a[d] + n2 * b[d];
+    }
+}
+
+__global__ void __launch_bounds__(TPB) k_interp(Geo gc, Geo gf, const float *__restrict__ iw,
+                                                const unsigned *__restrict__ NODE, const double *__restrict__ AD,
+                                                double *__restrict__ AU) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= gf.nno) return;
+    int k, i, j;
+    node_kij(gf, n, k, i, j);
+    const float *wx = iw, *wz = iw + 2 * gf.nox, *wy = iw + 2 * (gf.nox + gf.noz);
+    double v[3];
+    if (!(j & 1)) {
+        plane_xz(gc, wx, wz, AD, k, i, j, v);
+    } else {
+        double a[3], b[3];
+        plane_xz(gc, wx, wz, AD, k, i, j - 1, a);
+        plane_xz(gc, wx, wz, AD, k, i, j + 1, b);
+        const float n1 = wy[2 * j], n2 = wy[2 * j + 1];
+#pragma unroll
+        for (int d = 0; d < 3; d++) v[d] = n1 * a[d] + n2 * b[d];
+    }
+    strip3(NODE[n], v);
+    AU[3 * n] = v[0];
+    AU[3 * n + 1] = v[1];
+    AU[3 * n + 2] = v[2];
+}
+
+void interp(hipStream_t s, const Level &coarse, const Level &fine, const double *AD, double *AU) {
+    hipLaunchKernelGGL(k_interp, dim3(nblk(fine.g.nno)), dim3(TPB), 0, s, coarse.g, fine.g, fine.iw, fine.NODE, AD,
+                       AU);
+}
+
+// ---------------------------------------------------------------------------
+// divergence and gradient (Element_calculations.c:691-769)
+
+__global__ void __launch_bounds__(TPB) k_div(Geo g, const float *__restrict__ EG, const double *__restrict__ U,
+                                             double *__restrict__ divU) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.nel) return;
+    const int ez = e % g.elz, ex = (e / g.elz) % g.elx, ey = e / (g.elz * g.elx);
+    const float *eg = EG + (size_t)e * EG_SZ;
+    double acc = 0.0;
+#pragma unroll
+    for (int a = 1; a <= 8; a++) {
+        const int n = nid(g, ez + cOffz[a], ex + cOffx[a], ey + cOffy[a]);
+        const int p = 3 * (a - 1);
+        acc += eg[p] * U[3 * n] + eg[p + 1] * U[3 * n + 1] + eg[p + 2] * U[3 * n + 2];
+    }
+    divU[e] = acc;
+}
+
+void div(hipStream_t s, const Level &L, const double *U, double *divU) {
+    hipLaunchKernelGGL(k_div, dim3(nblk(L.g.nel)), dim3(TPB), 0, s, L.g, L.EG, U, divU);
+}
+
+__global__ void __launch_bounds__(TPB) k_grad(Geo g, const float *__restrict__ EG, const unsigned *__restrict__ NODE,
+                                              const double *__restrict__ P, double *__restrict__ gradP) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nno) return;
+    int k, i, j;
+    node_kij(g, n, k, i, j);
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int ey = j - 1; ey <= j; ey++) {
+        if ((unsigned)ey >= (unsigned)g.ely) continue;
+        for (int ex = i - 1; ex <= i; ex++) {
+            if ((unsigned)ex >= (unsigned)g.elx) continue;
+            for (int ez = k - 1; ez <= k; ez++) {
+                if ((unsigned)ez >= (unsigned)g.elz) continue;
+                const int e = ez + g.elz * (ex + g.elx * ey);
+                const double pe = P[e];
+                if (0.0 == pe) continue;
+                const int a = cLoc[(k - ez) + 2 * (i - ex) + 4 * (j - ey)];
+                const float *eg = EG + (size_t)e * EG_SZ + 3 * (a - 1);
+                acc[0] += eg[0] * pe;
+                acc[1] += eg[1] * pe;
+                acc[2] += eg[2] * pe;
+            }
+        }
+    }
+    strip3(NODE[n], acc);
+    gradP[3 * n] = acc[0];
+    gradP[3 * n + 1] = acc[1];
+    gradP[3 * n + 2] = acc[2];
+}
+
+void grad(hipStream_t s, const Level &L, const double *P, double *gradP) {
+    hipLaunchKernelGGL(k_grad, dim3(nblk(L.g.nno)), dim3(TPB), 0, s, L.g, L.EG, L.NODE, P, gradP);
+}
+
+__global__ void k_strip(Geo g, const unsigned *__restrict__ NODE, double *v) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nno) return;
+    const unsigned f = NODE[n];
+    if (f & VBX) v[3 * n] = 0.0;
+    if (f & VBY) v[3 * n + 1] = 0.0;
+    if (f & VBZ) v[3 * n + 2] = 0.0;
+}
+
+void strip(hipStream_t s, const Level &L, double *v) {
+    hipLaunchKernelGGL(k_strip, dim3(nblk(L.g.nno)), dim3(TPB), 0, s, L.g, L.NODE, v);
+}
+
+// ---------------------------------------------------------------------------
+// reductions: one launch, per-block partial sums, the last block to finish
+// sums the partials in block order (deterministic for a given size)
+
+constexpr int DOT_TPB = 256;
+constexpr int DOT_MAXB = 1024;
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+template <int ND>
+__device__ void block_sum(double v[ND], double *sh /* [ND][DOT_TPB/64] */) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int m = 0; m < ND; m++) v[m] = wave_sum(v[m]);
+    if (lane == 0)
+#pragma unroll
+        for (int m = 0; m < ND; m++) sh[m * 16 + w] = v[m];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int m = 0; m < ND; m++) {
+            double t = 0.0;
+            for (int q = 0; q < nw; q++) t += sh[m * 16 + q];
+            v[m] = t;
+        }
+}
+
+template <int ND>
+__global__ void __launch_bounds__(DOT_TPB) k_dots(int n, const double *__restrict__ a0, const double *__restrict__ b0,
+                                                  const double *__restrict__ a1, const double *__restrict__ b1,
+                                                  const double *__restrict__ a2, const double *__restrict__ b2,
+                                                  const double *__restrict__ a3, const double *__restrict__ b3,
+                                                  double *partial, unsigned *counter, double *out) {
+    __shared__ double sh[4 * 16];
+    __shared__ bool last;
+    double v[ND];
+#pragma unroll
+    for (int m = 0; m < ND; m++) v[m] = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        v[0] += a0[i] * b0[i];
+        if (ND > 1) v[1] += a1[i] * b1[i];
+        if (ND > 2) v[2] += a2[i] * b2[i];
+        if (ND > 3) v[3] += a3[i] * b3[i];
+    }
+    block_sum<ND>(v, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int m = 0; m < ND; m++) partial[blockIdx.x * 4 + m] = v[m];
+        __threadfence();
+        const unsigned t = atomicAdd(counter, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+#pragma unroll
+    for (int m = 0; m < ND; m++) v[m] = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
+#pragma unroll
+        for (int m = 0; m < ND; m++) v[m] += __builtin_nontemporal_load(&partial[b * 4 + m]);
+    __syncthreads();
+    block_sum<ND>(v, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int m = 0; m < ND; m++) out[m] = v[m];
+        *counter = 0u;
+    }
+}
+
+void dots(hipStream_t s, chip_ctx *c, int n, int nd, const double *a0, const double *b0, const double *a1,
+          const double *b1, const double *a2, const double *b2, const double *a3, const double *b3, double *out) {
+    const int nb = std::min(DOT_MAXB, std::max(1, (n + DOT_TPB * 4 - 1) / (DOT_TPB * 4)));
+    switch (nd) {
+    case 1: hipLaunchKernelGGL(k_dots<1>, dim3(nb), dim3(DOT_TPB), 0, s, n, a0, b0, a1, b1, a2, b2, a3, b3, c->partial, c->counter, out); break;
+    case 2: hipLaunchKernelGGL(k_dots<2>, dim3(nb), dim3(DOT_TPB), 0, s, n, a0, b0, a1, b1, a2, b2, a3, b3, c->partial, c->counter, out); break;
+    case 3: hipLaunchKernelGGL(k_dots<3>, dim3(nb), dim3(DOT_TPB), 0, s, n, a0, b0, a1, b1, a2, b2, a3, b3, c->partial, c->counter, out); break;
+    default: hipLaunchKernelGGL(k_dots<4>, dim3(nb), dim3(DOT_TPB), 0, s, n, a0, b0, a1, b1, a2, b2, a3, b3, c->partial, c->counter, out); break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// elementwise
+
+__global__ void k_fill(double *v, double val, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = val;
+}
+void fill(hipStream_t s, double *v, double val, long n) { hipLaunchKernelGGL(k_fill, dim3(nblk(n)), dim3(TPB), 0, s, v, val, n); }
+
+__global__ void k_copy(double *d, const double *a, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = a[i];
+}
+void copy(hipStream_t s, double *dst, const double *src, long n) { hipLaunchKernelGGL(k_copy, dim3(nblk(n)), dim3(TPB), 0, s, dst, src, n); }
+
+__global__ void k_fcopy(float *d, const float *a, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = a[i];
+}
+void fcopy(hipStream_t s, float *dst, const float *src, long n) { hipLaunchKernelGGL(k_fcopy, dim3(nblk(n)), dim3(TPB), 0, s, dst, src, n); }
+
+__global__ void k_axpy(double *y, const double *x, double a, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += a * x[i];
+}
+void axpy_scalar(hipStream_t s, double *y, const double *x, double a, long n) { hipLaunchKernelGGL(k_axpy, dim3(nblk(n)), dim3(TPB), 0, s, y, x, a, n); }
+
+__global__ void k_sub(double *r, const double *a, const double *b, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) r[i] = a[i] - b[i];
+}
+void sub(hipStream_t s, double *r, const double *a, const double *b, long n) { hipLaunchKernelGGL(k_sub, dim3(nblk(n)), dim3(TPB), 0, s, r, a, b, n); }
+
+// multi_grid line search update (General_matrix_functions.c:626-638)
+__global__ void k_mg_update(double *vel, const double *del, double *res, const double *AU, const double *sc, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double alpha = sc[1] / sc[0];
+    vel[i] += alpha * del[i];
+    if (res) res[i] -= alpha * AU[i];
+}
+void mg_update(hipStream_t s, double *vel, const double *del, double *res, const double *AU, const double *sc, long n) {
+    hipLaunchKernelGGL(k_mg_update, dim3(nblk(n)), dim3(TPB), 0, s, vel, del, res, AU, sc, n);
+}
+
+__global__ void k_uz_z(double *z1, const double *BPI, const double *r1, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) z1[i] = BPI[i] * r1[i];
+}
+void uz_z(hipStream_t s, double *z1, const double *BPI, const double *r1, long n) { hipLaunchKernelGGL(k_uz_z, dim3(nblk(n)), dim3(TPB), 0, s, z1, BPI, r1, n); }
+
+// s2 = z1 + delta s1, delta = *num / *den (Stokes_flow_Incomp.c:401-411)
+__global__ void k_uz_search(double *s2, const double *z1, const double *s1, const double *num, const double *den,
+                            int first, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (first) s2[i] = z1[i];
+    else {
+        const double delta = *num / *den;
+        s2[i] = z1[i] + delta * s1[i];
+    }
+}
+void uz_search(hipStream_t s, double *s2, const double *z1, const double *s1, const double *num, const double *den,
+               int first, long n) {
+    hipLaunchKernelGGL(k_uz_search, dim3(nblk(n)), dim3(TPB), 0, s, s2, z1, s1, num, den, first, n);
+}
+
+// r2 = r1 - alpha Ah, P += alpha s2 with alpha = valid ? *num / *den : 0
+// (Stokes_flow_Incomp.c:420-431)
+__global__ void k_uz_pupdate(double *r2, const double *r1, const double *Ah, double *P, const double *s2,
+                             const double *num, const double *den, int valid, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double alpha = valid ? *num / *den : 0.0;
+    r2[i] = r1[i] - alpha * Ah[i];
+    P[i] += alpha * s2[i];
+}
+void uz_pupdate(hipStream_t s, double *r2, const double *r1, const double *Ah, double *P, const double *s2,
+                const double *num, const double *den, int valid, long n) {
+    hipLaunchKernelGGL(k_uz_pupdate, dim3(nblk(n)), dim3(TPB), 0, s, r2, r1, Ah, P, s2, num, den, valid, n);
+}
+
+// V -= alpha u1 (Stokes_flow_Incomp.c:433-434)
+__global__ void k_uz_vupdate(double *V, const double *u1, const double *num, const double *den, int valid, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double alpha = valid ? *num / *den : 0.0;
+    V[i] -= alpha * u1[i];
+}
+void uz_vupdate(hipStream_t s, double *V, const double *u1, const double *num, const double *den, int valid, long n) {
+    hipLaunchKernelGGL(k_uz_vupdate, dim3(nblk(n)), dim3(TPB), 0, s, V, u1, num, den, valid, n);
+}
+
+// Ah = F - gradP - K V (Stokes_flow_Incomp.c:357-358)
+__global__ void k_init_residual(double *Ah, const double *F, const double *gP, const double *KV, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) Ah[i] = F[i] - gP[i] - KV[i];
+}
+void init_residual(hipStream_t s, double *Ah, const double *F, const double *gradP, const double *KV, long n) {
+    hipLaunchKernelGGL(k_init_residual, dim3(nblk(n)), dim3(TPB), 0, s, Ah, F, gradP, KV, n);
+}
+
+// v_from_vector (Stokes_flow_Incomp.c:530-552); prescribed velocities are zero
+__global__ void k_v_from_vector(Geo g, const unsigned *__restrict__ NODE, const double *__restrict__ U, float *V) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nno) return;
+    const unsigned f = NODE[n];
+    V[n] = (f & VBX) ? 0.0f : (float)U[3 * n];
+    V[g.nno + n] = (f & VBY) ? 0.0f : (float)U[3 * n + 1];
+    V[2 * g.nno + n] = (f & VBZ) ? 0.0f : (float)U[3 * n + 2];
+}
+void v_from_vector(hipStream_t s, const Level &L, const double *U, float *V) {
+    hipLaunchKernelGGL(k_v_from_vector, dim3(nblk(L.g.nno)), dim3(TPB), 0, s, L.g, L.NODE, U, V);
+}
+
+// velocities_conform_bcs (Boundary_conditions.c:993-1023) with VB = 0
+void conform_velocity(hipStream_t s, const Level &L, double *U) { strip(s, L, U); }
+
+// ---------------------------------------------------------------------------
+// stiffness rebuild
+
+// visc_from_T (Viscosity_structures.c:491-737, rheol 0 and 3) or
+// visc_from_mat (:477-489), then the MAX / MIN clip of get_system_viscosity
+struct ViscLaw {
+    int rheol, tdepv, vmin_on, vmax_on;
+    float visc0[4], viscE[4], viscT[4];
+    float visc_min, visc_max;
+};
+
+__global__ void __launch_bounds__(TPB) k_visc_from_T(Geo g, ViscLaw law, const int *__restrict__ mat,
+                                                     const float *__restrict__ T, float *__restrict__ evi) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.nel) return;
+    const int ez = e % g.elz, ex = (e / g.elz) % g.elx, ey = e / (g.elz * g.elx);
+    const int l = mat[e] - 1;
+    const float tempa = law.visc0[l];
+    float TT[9];
+#pragma unroll
+    for (int a = 1; a <= 8; a++) TT[a] = T[nid(g, ez + cOffz[a], ex + cOffx[a], ey + cOffy[a])];
+    for (int jj = 1; jj <= 8; jj++) {
+        float v;
+        if (!law.tdepv) {
+            v = tempa;
+        } else {
+            float temp = 1.0e-32f;
+#pragma unroll
+            for (int kk = 1; kk <= 8; kk++) temp = (float)(temp + fmaxf(0.0f, TT[kk]) * cNv[8 * (kk - 1) + jj - 1]);
+            float arg;
+            if (law.rheol == 3) arg = law.viscE[l] * (law.viscT[l] - temp);
+            else arg = law.viscE[l] * (1.0f - temp);
+            v = (float)(tempa * exp((double)arg));
+        }
+        if (law.vmax_on && v > law.visc_max) v = law.visc_max;
+        if (law.vmin_on && v < law.visc_min) v = law.visc_min;
+        evi[e * 8 + jj - 1] = v;
+    }
+}
+
+void visc_from_T(hipStream_t s, const chip_ctx *c, const float *T, float *evi) {
+    ViscLaw law{};
+    law.rheol = c->p.rheol;
+    law.tdepv = c->p.tdepv;
+    law.vmin_on = c->p.vmin_on;
+    law.vmax_on = c->p.vmax_on;
+    law.visc_min = c->p.visc_min;
+    law.visc_max = c->p.visc_max;
+    for (int i = 0; i < 4; i++) {
+        law.visc0[i] = c->p.visc0[i];
+        law.viscE[i] = c->p.viscE[i];
+        law.viscT[i] = c->p.viscT[i];
+    }
+    const Geo &g = c->L[c->levmax].g;
+    hipLaunchKernelGGL(k_visc_from_T, dim3(nblk(g.nel)), dim3(TPB), 0, s, g, law, c->mat, T, evi);
+}
+
+// visc_from_gint_to_nodes (Nodal_mesh.c:583-615)
+__global__ void __launch_bounds__(TPB) k_visc_to_nodes(Geo g, const float *__restrict__ EVI,
+                                                       const float *__restrict__ TWW, const float *__restrict__ MASS,
+                                                       float *__restrict__ VN) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nno) return;
+    int k, i, j;
+    node_kij(g, n, k, i, j);
+    float acc = 0.0f;
+    for (int ey = j - 1; ey <= j; ey++) {
+        if ((unsigned)ey >= (unsigned)g.ely) continue;
+        for (int ex = i - 1; ex <= i; ex++) {
+            if ((unsigned)ex >= (unsigned)g.elx) continue;
+            for (int ez = k - 1; ez <= k; ez++) {
+                if ((unsigned)ez >= (unsigned)g.elz) continue;
+                const int e = ez + g.elz * (ex + g.elx * ey);
+                double tv = 0.0;
+#pragma unroll
+                for (int q = 0; q < 8; q++) tv += EVI[e * 8 + q];
+                tv = tv / 8;
+                const int a = cLoc[(k - ez) + 2 * (i - ex) + 4 * (j - ey)];
+                acc = (float)(acc + TWW[(size_t)e * TWW_SZ + a] * tv);
+            }
+        }
+    }
+    VN[n] = acc * MASS[n];
+}
+
+// project_scalar (Solver_multigrid.c:344-388), result into the coarse VN
+__global__ void __launch_bounds__(TPB) k_visc_project(Geo gf, Geo gc, const float *__restrict__ VNf,
+                                                      const float *__restrict__ TWW, const float *__restrict__ MASS,
+                                                      float *__restrict__ VNc) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= gc.nno) return;
+    int k, i, j;
+    node_kij(gc, n, k, i, j);
+    float acc = 0.0f;
+    for (int ey = j - 1; ey <= j; ey++) {
+        if ((unsigned)ey >= (unsigned)gc.ely) continue;
+        for (int ex = i - 1; ex <= i; ex++) {
+            if ((unsigned)ex >= (unsigned)gc.elx) continue;
+            for (int ez = k - 1; ez <= k; ez++) {
+                if ((unsigned)ez >= (unsigned)gc.elz) continue;
+                const int a = cLoc[(k - ez) + 2 * (i - ex) + 4 * (j - ey)];
+                const int el = ez + gc.elz * (ex + gc.elx * ey);
+                const int fz = 2 * ez + cOffz[a], fx = 2 * ex + cOffx[a], fy = 2 * ey + cOffy[a];
+                float average = 0.0f;
+#pragma unroll
+                for (int b = 1; b <= 8; b++) average += VNf[nid(gf, fz + cOffz[b], fx + cOffx[b], fy + cOffy[b])];
+                const float w = (float)((1.0 / 8) * average);
+                acc += w * TWW[(size_t)el * TWW_SZ + a];
+            }
+        }
+    }
+    VNc[n] = acc * MASS[n];
+}
+
+// visc_from_nodes_to_gint (Nodal_mesh.c:617-640)
+__global__ void __launch_bounds__(TPB) k_visc_to_gint(Geo g, const float *__restrict__ VN, float *__restrict__ EVI) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.nel) return;
+    const int ez = e % g.elz, ex = (e / g.elz) % g.elx, ey = e / (g.elz * g.elx);
+    float vn[9];
+#pragma unroll
+    for (int a = 1; a <= 8; a++) vn[a] = VN[nid(g, ez + cOffz[a], ex + cOffx[a], ey + cOffy[a])];
+#pragma unroll
+    for (int i = 1; i <= 8; i++) {
+        double tv = 0.0;
+#pragma unroll
+        for (int j = 1; j <= 8; j++) tv += cNv[8 * (j - 1) + i - 1] * vn[j];
+        EVI[e * 8 + i - 1] = (float)tv;
+    }
+}
+
+void project_viscosity(hipStream_t s, const Level &fine, const Level &coarse) {
+    hipLaunchKernelGGL(k_visc_to_nodes, dim3(nblk(fine.g.nno)), dim3(TPB), 0, s, fine.g, fine.EVI, fine.TWW, fine.MASS,
+                       fine.VN);
+    hipLaunchKernelGGL(k_visc_project, dim3(nblk(coarse.g.nno)), dim3(TPB), 0, s, fine.g, coarse.g, fine.VN,
+                       coarse.TWW, coarse.MASS, coarse.VN);
+    hipLaunchKernelGGL(k_visc_to_gint, dim3(nblk(coarse.g.nel)), dim3(TPB), 0, s, coarse.g, coarse.VN, coarse.EVI);
+}
+
+// get_elt_g (Element_calculations.c:831-934, CART3D)
+__global__ void k_elt_g(Geo g, const float *__restrict__ GNX, const float *__restrict__ GDA, float *__restrict__ EG) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.nel) return;
+    const float *gnx = GNX + (size_t)e * GNX_SZ + 192;
+    const float tf = 8.0f * GDA[(size_t)e * GDA_SZ + 10];
+    const double temp = tf;
+#pragma unroll
+    for (int a = 1; a <= 8; a++) {
+        const int p = 3 * (a - 1);
+        EG[(size_t)e * EG_SZ + p] = (float)(-gnx[a - 1] * temp);
+        EG[(size_t)e * EG_SZ + p + 1] = (float)(-gnx[8 + a - 1] * temp);
+        EG[(size_t)e * EG_SZ + p + 2] = (float)(-gnx[16 + a - 1] * temp);
+    }
+}
+
+void elt_g(hipStream_t s, const Level &L) {
+    hipLaunchKernelGGL(k_elt_g, dim3(nblk(L.g.nel)), dim3(TPB), 0, s, L.g, L.GNX, L.GDA, L.EG);
+}
+
+// 3x3 block of the element stiffness elt_k (get_elt_k, Element_calculations.c:
+// 133-293, cartesian isotropic) plus get_aug_k (:1127-1163): rows of local
+// node ar, columns of local node bc.  The reference evaluates the (a,b) block
+// for a <= b and stores its transpose below the diagonal; the aug term is
+// added per entry in the (row, column) orientation.
+__device__ void elt_block(const float *__restrict__ gnx, const float *__restrict__ gda, const float *__restrict__ evi,
+                          const float *__restrict__ eg, int aug_on, double aug, int ar, int bc, double blk[3][3]) {
+    const int a = ar < bc ? ar : bc, b = ar < bc ? bc : ar;
+    double m[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, temp = 0.0;
+#pragma unroll
+    for (int k = 1; k <= 8; k++) {
+        float wf = 1.0f * gda[k];
+        wf = wf * evi[k - 1];
+        const double W = wf;
+        const float ax = gnx[8 * (a - 1) + k - 1], ay = gnx[64 + 8 * (a - 1) + k - 1], az = gnx[128 + 8 * (a - 1) + k - 1];
+        const float bx = gnx[8 * (b - 1) + k - 1], by = gnx[64 + 8 * (b - 1) + k - 1], bz = gnx[128 + 8 * (b - 1) + k - 1];
+        m[0][0] += W * ax * bx;
+        m[0][1] += W * ay * bx;
+        m[0][2] += W * az * bx;
+        m[1][0] += W * ax * by;
+        m[1][1] += W * ay * by;
+        m[1][2] += W * az * by;
+        m[2][0] += W * ax * bz;
+        m[2][1] += W * ay * bz;
+        m[2][2] += W * az * bz;
+    }
+#pragma unroll
+    for (int k = 1; k <= 8; k++) {
+        float wf = 1.0f * gda[k];
+        wf = wf * evi[k - 1];
+        const double W = wf;
+        const float s = gnx[8 * (a - 1) + k - 1] * gnx[8 * (b - 1) + k - 1] +
+                        gnx[64 + 8 * (a - 1) + k - 1] * gnx[64 + 8 * (b - 1) + k - 1] +
+                        gnx[128 + 8 * (a - 1) + k - 1] * gnx[128 + 8 * (b - 1) + k - 1];
+        temp += W * s;
+    }
+    m[0][0] += temp;
+    m[1][1] += temp;
+    m[2][2] += temp;
+    const bool tr = ar > bc;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) blk[r][c] = tr ? m[c][r] : m[r][c];
+    if (aug_on) {
+        double visc = 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) visc += evi[q];
+        visc = visc / 8;
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) blk[r][c] += visc * aug * eg[3 * (ar - 1) + r] * eg[3 * (bc - 1) + c];
+    }
+}
+
+// construct_node_ks (Construct_arrays.c:359-535) restated per node: the block
+// coupling node n with neighbour m is the float accumulation, in element
+// order, of w_r ww_c elt_k[rows of max(n,m)][cols of min(n,m)] (BC weights
+// zero the prescribed rows and columns), transposed when m > n; also
+// build_diagonal_of_K (Element_calculations.c:624-652) into BI = 1/diag.
+__global__ void __launch_bounds__(128) k_node_ks(Geo g, const float *__restrict__ GNX, const float *__restrict__ GDA,
+                                                 const float *__restrict__ EVI, const float *__restrict__ EG,
+                                                 const unsigned *__restrict__ NODE, int aug_on, double aug,
+                                                 float *__restrict__ K, double *__restrict__ BI) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nno) return;
+    int k, i, j;
+    colour_kij(g, colour_of(g, q), q, k, i, j);
+    const int n = nid(g, k, i, j);
+    const size_t P = (size_t)g.pitch;
+    double diag[3] = {0.0, 0.0, 0.0};
+    for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++)
+            for (int dz = -1; dz <= 1; dz++) {
+                const int s = (dz + 1) + 3 * (dx + 1) + 9 * (dy + 1);
+                const int km = k + dz, im = i + dx, jm = j + dy;
+                float acc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                const bool ok = (unsigned)km < (unsigned)g.noz && (unsigned)im < (unsigned)g.nox &&
+                                (unsigned)jm < (unsigned)g.noy;
+                if (ok) {
+                    const int m = nid(g, km, im, jm);
+                    const bool nbig = (m <= n);
+                    const int big = nbig ? n : m, small = nbig ? m : n;
+                    const unsigned fb = NODE[big], fs = NODE[small];
+                    const double w[3] = {(fb & VBX) ? 0.0 : 1.0, (fb & VBY) ? 0.0 : 1.0, (fb & VBZ) ? 0.0 : 1.0};
+                    const double ww[3] = {(fs & VBX) ? 0.0 : 1.0, (fs & VBY) ? 0.0 : 1.0, (fs & VBZ) ? 0.0 : 1.0};
+                    const int kb = nbig ? k : km, ib = nbig ? i : im, jb = nbig ? j : jm;
+                    const int ks = nbig ? km : k, is = nbig ? im : i, js = nbig ? jm : j;
+                    // elements holding both nodes, ascending element number
+                    for (int ey = max(j, jm) - 1; ey <= min(j, jm); ey++) {
+                        if ((unsigned)ey >= (unsigned)g.ely) continue;
+                        for (int ex = max(i, im) - 1; ex <= min(i, im); ex++) {
+                            if ((unsigned)ex >= (unsigned)g.elx) continue;
+                            for (int ez = max(k, km) - 1; ez <= min(k, km); ez++) {
+                                if ((unsigned)ez >= (unsigned)g.elz) continue;
+                                const int e = ez + g.elz * (ex + g.elx * ey);
+                                const int ab = cLoc[(kb - ez) + 2 * (ib - ex) + 4 * (jb - ey)];
+                                const int as = cLoc[(ks - ez) + 2 * (is - ex) + 4 * (js - ey)];
+                                double blk[3][3];
+                                elt_block(GNX + (size_t)e * GNX_SZ, GDA + (size_t)e * GDA_SZ, EVI + (size_t)e * 8,
+                                          EG + (size_t)e * EG_SZ, aug_on, aug, ab, as, blk);
+                                if (s == 13) {
+                                    diag[0] += blk[0][0];
+                                    diag[1] += blk[1][1];
+                                    diag[2] += blk[2][2];
+                                }
+                                // reference order: column direction outer, row inner
+#pragma unroll
+                                for (int c = 0; c < 3; c++)
+#pragma unroll
+                                    for (int r = 0; r < 3; r++)
+                                        acc[r][c] = (float)(acc[r][c] + w[r] * ww[c] * blk[r][c]);
+                            }
+                        }
+                    }
+                    if (!nbig) {
+                        float t[3][3];
+#pragma unroll
+                        for (int r = 0; r < 3; r++)
+#pragma unroll
+                            for (int c = 0; c < 3; c++) t[r][c] = acc[c][r];
+#pragma unroll
+                        for (int r = 0; r < 3; r++)
+#pragma unroll
+                            for (int c = 0; c < 3; c++) acc[r][c] = t[r][c];
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < 3; r++)
+#pragma unroll
+                    for (int c = 0; c < 3; c++) K[(size_t)(s * 9 + r * 3 + c) * P + q] = acc[r][c];
+            }
+    BI[3 * n] = (float)1.0 / diag[0];
+    BI[3 * n + 1] = (float)1.0 / diag[1];
+    BI[3 * n + 2] = (float)1.0 / diag[2];
+}
+
+void node_ks(hipStream_t s, const Level &L, const chip_problem &p) {
+    hipLaunchKernelGGL(k_node_ks, dim3(nblk(L.g.nno, 128)), dim3(128), 0, s, L.g, L.GNX, L.GDA, L.EVI, L.EG, L.NODE,
+                       p.aug_lagr, p.aug_number, L.K, L.BI);
+}
+
+// assemble_dAhatp_entry / build_diagonal_of_Ahat (Element_calculations.c:654-685, 771-824)
+__global__ void k_bpi(Geo g, const float *__restrict__ EG, const double *__restrict__ BI, int precond,
+                      double *__restrict__ BPI) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.nel) return;
+    if (!precond) {
+        BPI[e] = 1.0;
+        return;
+    }
+    const int ez = e % g.elz, ex = (e / g.elz) % g.elx, ey = e / (g.elz * g.elx);
+    const float *eg = EG + (size_t)e * EG_SZ;
+    double divU = 0.0;
+#pragma unroll
+    for (int a = 1; a <= 8; a++) {
+        const int n = nid(g, ez + cOffz[a], ex + cOffx[a], ey + cOffy[a]);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double gp = BI[3 * n + d] * eg[3 * (a - 1) + d];
+            divU += eg[3 * (a - 1) + d] * gp;
+        }
+    }
+    BPI[e] = (divU != 0.0) ? 1.0 / divU : 1.0;
+}
+
+void build_bpi(hipStream_t s, const Level &L, int precond) {
+    hipLaunchKernelGGL(k_bpi, dim3(nblk(L.g.nel)), dim3(TPB), 0, s, L.g, L.EG, L.BI, precond, L.BPI);
+}
+
+// ---------------------------------------------------------------------------
+// energy equation
+
+// order-preserving float <-> unsigned maps for atomic min / max
+__device__ __forceinline__ unsigned ford(float f) {
+    const unsigned b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+// std_timestep (Advection_diffusion.c:737-810): per-element advective limit
+// min into minmax[0]; Tmax (Global_operations.c:394-406) into minmax[1]
+__global__ void __launch_bounds__(TPB) k_timestep_tmax(Geo g, const float *__restrict__ ECO, const float *__restrict__ V,
+                                                       const float *__restrict__ T, unsigned *minmax) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned mn = ford(1.0e8f), mx = ford(-10.0f);
+    if (V && t < g.nel) {
+        const int e = t;
+        const int ez = e % g.elz, ex = (e / g.elz) % g.elx, ey = e / (g.elz * g.elx);
+        float uc1 = 0.0f, uc2 = 0.0f, uc3 = 0.0f;
+#pragma unroll
+        for (int a = 1; a <= 8; a++) {
+            const int nd = nid(g, ez + cOffz[a], ex + cOffx[a], ey + cOffy[a]);
+            uc1 = (float)(uc1 + cNp[a - 1] * V[nd]);
+            uc2 = (float)(uc2 + cNp[a - 1] * V[g.nno + nd]);
+            uc3 = (float)(uc3 + cNp[a - 1] * V[2 * g.nno + nd]);
+        }
+        const float *eco = ECO + (size_t)e * ECO_SZ;
+        const float uc = (float)(fabs((double)uc1) / eco[5] + fabs((double)uc2) / eco[6] + fabs((double)uc3) / eco[7]);
+        const float step = (float)(0.5 / uc);
+        mn = ford(step);
+    }
+    if (t < g.nno) mx = ford(T[t]);
+    // wave reductions, then one atomic per wave
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mn = min(mn, (unsigned)__shfl_down((int)mn, o, 64));
+        mx = max(mx, (unsigned)__shfl_down((int)mx, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&minmax[0], mn);
+        atomicMax(&minmax[1], mx);
+    }
+}
+
+__global__ void k_minmax_init(unsigned *minmax) {
+    minmax[0] = ford(1.0e8f);
+    minmax[1] = ford(-10.0f);
+}
+
+void timestep_tmax(hipStream_t s, chip_ctx *c, const float *V, const float *T) {
+    const Level &L = c->L[c->levmax];
+    hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(1), 0, s, c->minmax);
+    hipLaunchKernelGGL(k_timestep_tmax, dim3(nblk(std::max(L.g.nel, L.g.nno))), dim3(TPB), 0, s, L.g, L.ECO, V, T,
+                       c->minmax);
+}
+
+void tmax(hipStream_t s, chip_ctx *c, const float *T) { timestep_tmax(s, c, nullptr, T); }
+
+// predictor (Advection_diffusion.c:352-366)
+__global__ void k_predictor(Geo g, const unsigned *__restrict__ NODE, float *T, float *Tdot, float mult) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nno) return;
+    if (!(NODE[n] & (TBX | TBZ | TBY))) T[n] += mult * Tdot[n];
+    Tdot[n] = 0.0f;
+}
+
+void predictor(hipStream_t s, const Level &L, float *T, float *Tdot, float mult) {
+    hipLaunchKernelGGL(k_predictor, dim3(nblk(L.g.nno)), dim3(TPB), 0, s, L.g, L.NODE, T, Tdot, mult);
+}
+
+// pg_shape_fn + element_residual (Advection_diffusion.c:448-727, CART3D,
+// diffusion on, no FBZ flux nodes): Eres[e][a] for the 8 nodes of element e
+__global__ void __launch_bounds__(TPB) k_pg_residual(Geo g, const float *__restrict__ GNX, const float *__restrict__ GDA,
+                                                     const float *__restrict__ ECO, const unsigned *__restrict__ NODE,
+                                                     const float *__restrict__ diffusivity, double Q,
+                                                     const float *__restrict__ T, const float *__restrict__ Tdot,
+                                                     const float *__restrict__ V, double *__restrict__ Eres) {
+    const int el = blockIdx.x * blockDim.x + threadIdx.x;
+    if (el >= g.nel) return;
+    const int ez = el % g.elz, ex = (el / g.elz) % g.elx, ey = el / (g.elz * g.elx);
+    const float diff = (float)((diffusivity[ez] + diffusivity[ez + 1]) * 0.5);
+    const float *gnx = GNX + (size_t)el * GNX_SZ;
+    const float *gda = GDA + (size_t)el * GDA_SZ;
+    const float *eco = ECO + (size_t)el * ECO_SZ;
+    int nodes[9];
+    float v1n[9], v2n[9], v3n[9];
+#pragma unroll
+    for (int a = 1; a <= 8; a++) {
+        nodes[a] = nid(g, ez + cOffz[a], ex + cOffx[a], ey + cOffy[a]);
+        v1n[a] = V[nodes[a]];
+        v2n[a] = V[g.nno + nodes[a]];
+        v3n[a] = V[2 * g.nno + nodes[a]];
+    }
+    // pg_shape_fn
+    double uc1 = 0.0, uc2 = 0.0, uc3 = 0.0;
+#pragma unroll
+    for (int a = 1; a <= 8; a++) {
+        uc1 += cNp[a - 1] * v1n[a];
+        uc2 += cNp[a - 1] * v2n[a];
+        uc3 += cNp[a - 1] * v3n[a];
+    }
+    const double twodiff = 2.0 * diff;
+    const double uxse = fabs(uc1 * (double)eco[5]), ueta = fabs(uc2 * (double)eco[6]), ufai = fabs(uc3 * (double)eco[7]);
+    const double xse = (uxse > twodiff) ? (1.0 - twodiff / uxse) : 0.0;
+    const double eta = (ueta > twodiff) ? (1.0 - twodiff / ueta) : 0.0;
+    const double fai = (ufai > twodiff) ? (1.0 - twodiff / ufai) : 0.0;
+    const double unorm = uc1 * uc1 + uc2 * uc2 + uc3 * uc3;
+    const double adiff = (unorm > 0.000001) ? ((uxse * xse + ueta * eta + ufai * fai) / (2.0 * unorm)) : 0.0;
+    // element_residual: gauss-point values
+    double dT[9], tx1[9], tx2[9], tx3[9], v1[9], v2[9], v3[9];
+#pragma unroll
+    for (int i = 1; i <= 8; i++) dT[i] = tx1[i] = tx2[i] = tx3[i] = v1[i] = v2[i] = v3[i] = 0.0;
+#pragma unroll
+    for (int j = 1; j <= 8; j++) {
+        const int node = nodes[j];
+        const double Tn = T[node];
+        const double DT = (NODE[node] & (TBX | TBY | TBZ)) ? 0.0 : (double)Tdot[node];
+#pragma unroll
+        for (int i = 1; i <= 8; i++) {
+            const double sfn = cNv[8 * (j - 1) + i - 1];
+            dT[i] += DT * sfn;
+            tx1[i] += gnx[8 * (j - 1) + i - 1] * Tn;
+            tx2[i] += gnx[64 + 8 * (j - 1) + i - 1] * Tn;
+            tx3[i] += gnx[128 + 8 * (j - 1) + i - 1] * Tn;
+            v1[i] += v1n[j] * sfn;
+            v2[i] += v2n[j] * sfn;
+            v3[i] += v3n[j] * sfn;
+        }
+    }
+    double u1g[9], u2g[9], u3g[9];
+#pragma unroll
+    for (int i = 1; i <= 8; i++) {
+        double a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll
+        for (int j = 1; j <= 8; j++) {
+            const double sfn = cNv[8 * (j - 1) + i - 1];
+            a1 += v1n[j] * sfn;
+            a2 += v2n[j] * sfn;
+            a3 += v3n[j] * sfn;
+        }
+        u1g[i] = a1;
+        u2g[i] = a2;
+        u3g[i] = a3;
+    }
+#pragma unroll
+    for (int j = 1; j <= 8; j++) {
+        double r = 0.0;
+#pragma unroll
+        for (int i = 1; i <= 8; i++) {
+            const float gx = gnx[8 * (j - 1) + i - 1], gy = gnx[64 + 8 * (j - 1) + i - 1], gz = gnx[128 + 8 * (j - 1) + i - 1];
+            const double prod1 = (u1g[i] * gx + u2g[i] * gy + u3g[i] * gz);
+            const double PG = cNv[8 * (j - 1) + i - 1] + adiff * prod1;
+            const float dl = diff * 1.0f * gda[i];
+            r -= PG * gda[i] * (dT[i] - Q + v1[i] * tx1[i] + v2[i] * tx2[i] + v3[i] * tx3[i]) +
+                 dl * (gx * tx1[i] + gy * tx2[i] + gz * tx3[i]);
+        }
+        Eres[(size_t)el * 8 + j - 1] = r;
+    }
+}
+
+void pg_residual(hipStream_t s, const chip_ctx *c, const float *T, const float *Tdot, const float *V, double *Eres) {
+    const Level &L = c->L[c->levmax];
+    // heating_adi = heating_visc = 0 and heating_latent = 1 (Instructions.c:376-381)
+    const double Q = (c->p.Q0 - 0.0f + 0.0f) * 1.0f;
+    hipLaunchKernelGGL(k_pg_residual, dim3(nblk(L.g.nel)), dim3(TPB), 0, s, L.g, L.GNX, L.GDA, L.ECO, L.NODE,
+                       c->diffusivity, Q, T, Tdot, V, Eres);
+}
+
+// pg_solver's scatter and lumped-mass scaling (Advection_diffusion.c:423-437)
+// fused with corrector (:369-387)
+__global__ void __launch_bounds__(TPB) k_pg_gather(Geo g, const unsigned *__restrict__ NODE,
+                                                   const float *__restrict__ MASS, const double *__restrict__ Eres,
+                                                   float *T, float *Tdot, float mult) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nno) return;
+    int k, i, j;
+    node_kij(g, n, k, i, j);
+    float dt = 0.0f;
+    for (int ey = j - 1; ey <= j; ey++) {
+        if ((unsigned)ey >= (unsigned)g.ely) continue;
+        for (int ex = i - 1; ex <= i; ex++) {
+            if ((unsigned)ex >= (unsigned)g.elx) continue;
+            for (int ez = k - 1; ez <= k; ez++) {
+                if ((unsigned)ez >= (unsigned)g.elz) continue;
+                const int e = ez + g.elz * (ex + g.elx * ey);
+                const int a = cLoc[(k - ez) + 2 * (i - ex) + 4 * (j - ey)];
+                dt = (float)(dt + Eres[(size_t)e * 8 + a - 1]);
+            }
+        }
+    }
+    dt *= MASS[n];
+    if (!(NODE[n] & (TBX | TBZ | TBY))) T[n] += mult * dt;
+    Tdot[n] += dt;
+}
+
+void pg_gather_correct(hipStream_t s, const chip_ctx *c, const double *Eres, float *T, float *Tdot, float mult) {
+    const Level &L = c->L[c->levmax];
+    hipLaunchKernelGGL(k_pg_gather, dim3(nblk(L.g.nno)), dim3(TPB), 0, s, L.g, L.NODE, L.MASS, Eres, T, Tdot, mult);
+}
+
+// temperatures_conform_bcs (Boundary_conditions.c:949-990); the side walls
+// carry zero-flux flags, so only TBZ nodes are set
+__global__ void k_conform_T(Geo g, const unsigned *__restrict__ NODE, const float *__restrict__ TB, float *T) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nno) return;
+    if (NODE[n] & TBZ) T[n] = TB[n];
+}
+
+void conform_temperature(hipStream_t s, const chip_ctx *c, float *T) {
+    const Level &L = c->L[c->levmax];
+    hipLaunchKernelGGL(k_conform_T, dim3(nblk(L.g.nno)), dim3(TPB), 0, s, L.g, L.NODE, c->TB, T);
+}
+
+// thermal_buoyancy (Pan_problem_misc_functions.c:83-153) with remove_horiz_ave
+// / return_horiz_ave (Global_operations.c:55-80, 133-251)
+__global__ void k_buoy_raw(Geo g, float Ra, const float *__restrict__ expansivity, const float *__restrict__ T,
+                           float *__restrict__ b) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nno) return;
+    b[n] = Ra * T[n] * expansivity[n % g.noz];
+}
+
+// horizontal average of a nodal field on node layer kz = blockIdx.x, with
+// the bilinear surface quadrature of get_global_1d_shape_fn
+__global__ void __launch_bounds__(TPB) k_horiz_ave(Geo g, const float *__restrict__ xc, const float *__restrict__ yc,
+                                                   const float *__restrict__ X, float *__restrict__ H) {
+    __shared__ double sh[4 * 16];
+    const int kz = blockIdx.x;
+    double v[2] = {0.0, 0.0};
+    for (int f = threadIdx.x; f < g.elx * g.ely; f += blockDim.x) {
+        const int ex = f / g.ely, ey = f % g.ely;
+        const int n1 = nid(g, kz, ex, ey), n2 = nid(g, kz, ex + 1, ey), n3 = nid(g, kz, ex + 1, ey + 1),
+                  n4 = nid(g, kz, ex, ey + 1);
+        const double xx1[5] = {0, xc[ex], xc[ex + 1], xc[ex + 1], xc[ex]};
+        const double xx2[5] = {0, yc[ey], yc[ey], yc[ey + 1], yc[ey + 1]};
+        double dG[5];
+#pragma unroll
+        for (int nint = 1; nint <= 4; nint++) {
+            double d11 = 0, d12 = 0, d21 = 0, d22 = 0;
+#pragma unroll
+            for (int ii = 1; ii <= 4; ii++) {
+                const double mx0 = cMxv[4 * (ii - 1) + nint - 1], mx1 = cMxv[16 + 4 * (ii - 1) + nint - 1];
+                d11 += xx1[ii] * mx0;
+                d12 += xx2[ii] * mx0;
+                d21 += xx1[ii] * mx1;
+                d22 += xx2[ii] * mx1;
+            }
+            dG[nint] = d11 * d22 - d12 * d21;
+        }
+        const float xv[5] = {0, X[n1], X[n2], X[n3], X[n4]};
+#pragma unroll
+        for (int d = 1; d <= 4; d++)
+#pragma unroll
+            for (int nint = 1; nint <= 4; nint++) {
+                v[0] += xv[d] * cMv[4 * (d - 1) + nint - 1] * dG[nint];
+                v[1] += cMv[4 * (d - 1) + nint - 1] * dG[nint];
+            }
+    }
+    block_sum<2>(v, sh);
+    if (threadIdx.x == 0 && v[1] != 0.0) H[kz] = (float)(v[0] / v[1]);
+}
+
+__global__ void k_sub_layer(Geo g, const float *__restrict__ H, float *b) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nno) return;
+    b[n] -= H[n % g.noz];
+}
+
+void thermal_buoyancy(hipStream_t s, const chip_ctx *c, const float *T, float *buoy) {
+    const Level &L = c->L[c->levmax];
+    hipLaunchKernelGGL(k_buoy_raw, dim3(nblk(L.g.nno)), dim3(TPB), 0, s, L.g, c->p.rayleigh, c->expansivity, T, buoy);
+    hipLaunchKernelGGL(k_horiz_ave, dim3(L.g.noz), dim3(TPB), 0, s, L.g, c->xc_d, c->yc_d, buoy, c->H);
+    hipLaunchKernelGGL(k_sub_layer, dim3(nblk(L.g.nno)), dim3(TPB), 0, s, L.g, c->H, buoy);
+}
+
+// assemble_forces + get_elt_f (Element_calculations.c:74-125, 989-1121,
+// CART3D, no imposed velocities): element z-forces, then a per-node gather
+__global__ void __launch_bounds__(TPB) k_elt_f(Geo g, const float *__restrict__ GDA, const float *__restrict__ buoy,
+                                               double *__restrict__ Ef) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.nel) return;
+    const int ez = e % g.elz, ex = (e / g.elz) % g.elx, ey = e / (g.elz * g.elx);
+    const float *gda = GDA + (size_t)e * GDA_SZ;
+    double force[9], fg[9];
+#pragma unroll
+    for (int a = 1; a <= 8; a++) force[a] = buoy[nid(g, ez + cOffz[a], ex + cOffx[a], ey + cOffy[a])];
+#pragma unroll
+    for (int j = 1; j <= 8; j++) {
+        double t = 0.0;
+#pragma unroll
+        for (int k = 1; k <= 8; k++) t += force[k] * cNv[8 * (k - 1) + j - 1];
+        fg[j] = t;
+    }
+#pragma unroll
+    for (int a = 1; a <= 8; a++) {
+        double f = 0.0;
+#pragma unroll
+        for (int j = 1; j <= 8; j++) f += fg[j] * cNv[8 * (a - 1) + j - 1] * gda[j] * 1.0f;
+        Ef[(size_t)e * 8 + a - 1] = f;
+    }
+}
+
+__global__ void __launch_bounds__(TPB) k_forces_gather(Geo g, const unsigned *__restrict__ NODE,
+                                                       const double *__restrict__ Ef, double *__restrict__ F) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nno) return;
+    int k, i, j;
+    node_kij(g, n, k, i, j);
+    double acc = 0.0;
+    for (int ey = j - 1; ey <= j; ey++) {
+        if ((unsigned)ey >= (unsigned)g.ely) continue;
+        for (int ex = i - 1; ex <= i; ex++) {
+            if ((unsigned)ex >= (unsigned)g.elx) continue;
+            for (int ez = k - 1; ez <= k; ez++) {
+                if ((unsigned)ez >= (unsigned)g.elz) continue;
+                const int e = ez + g.elz * (ex + g.elx * ey);
+                acc += Ef[(size_t)e * 8 + cLoc[(k - ez) + 2 * (i - ex) + 4 * (j - ey)] - 1];
+            }
+        }
+    }
+    const unsigned f = NODE[n];
+    F[3 * n] = 0.0;
+    F[3 * n + 1] = 0.0;
+    F[3 * n + 2] = (f & VBZ) ? 0.0 : acc;
+}
+
+void assemble_forces(hipStream_t s, const chip_ctx *c, const float *buoy, double *F) {
+    const Level &L = c->L[c->levmax];
+    hipLaunchKernelGGL(k_elt_f, dim3(nblk(L.g.nel)), dim3(TPB), 0, s, L.g, L.GDA, buoy, c->Eres);
+    hipLaunchKernelGGL(k_forces_gather, dim3(nblk(L.g.nno)), dim3(TPB), 0, s, L.g, L.NODE, c->Eres, F);
+}
+
+// heat_flux (Process_buoyancy.c:63-203)
+__global__ void __launch_bounds__(TPB) k_hf_elem(Geo g, const float *__restrict__ GNX, const float *__restrict__ GDA,
+                                                 const float *__restrict__ ECO, const float *__restrict__ diffusivity,
+                                                 const float *__restrict__ T, const float *__restrict__ V,
+                                                 double *__restrict__ uTe) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.nel) return;
+    const int ez = e % g.elz, ex = (e / g.elz) % g.elx, ey = e / (g.elz * g.elx);
+    const double diff = (diffusivity[ez] + diffusivity[ez + 1]) * 0.5;
+    const float *gnx = GNX + (size_t)e * GNX_SZ;
+    const float *gda = GDA + (size_t)e * GDA_SZ;
+    double VZ[9];
+    float Tn[9];
+#pragma unroll
+    for (int a = 1; a <= 8; a++) {
+        const int n = nid(g, ez + cOffz[a], ex + cOffx[a], ey + cOffy[a]);
+        VZ[a] = V[2 * g.nno + n];
+        Tn[a] = T[n];
+    }
+    double uT = 0.0;
+#pragma unroll
+    for (int i = 1; i <= 8; i++) {
+        double u = 0.0, Tg = 0.0, dTdz = 0.0;
+#pragma unroll
+        for (int j = 1; j <= 8; j++) {
+            u += VZ[j] * cNv[8 * (j - 1) + i - 1];
+            Tg += Tn[j] * cNv[8 * (j - 1) + i - 1];
+            dTdz = dTdz + Tn[j] * gnx[128 + 8 * (j - 1) + i - 1];
+        }
+        uT = uT + (u * Tg - diff * dTdz) * gda[i];
+    }
+    uTe[e] = uT / ECO[(size_t)e * ECO_SZ + 12];
+}
+
+__global__ void __launch_bounds__(TPB) k_hf_gather(Geo g, const float *__restrict__ TWW, const float *__restrict__ MASS,
+                                                   const double *__restrict__ uTe, float *__restrict__ hf) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nno) return;
+    int k, i, j;
+    node_kij(g, n, k, i, j);
+    float acc = 0.0f;
+    for (int ey = j - 1; ey <= j; ey++) {
+        if ((unsigned)ey >= (unsigned)g.ely) continue;
+        for (int ex = i - 1; ex <= i; ex++) {
+            if ((unsigned)ex >= (unsigned)g.elx) continue;
+            for (int ez = k - 1; ez <= k; ez++) {
+                if ((unsigned)ez >= (unsigned)g.elz) continue;
+                const int e = ez + g.elz * (ex + g.elx * ey);
+                const int a = cLoc[(k - ez) + 2 * (i - ex) + 4 * (j - ey)];
+                acc = (float)(acc + TWW[(size_t)e * TWW_SZ + a] * uTe[e]);
+            }
+        }
+    }
+    hf[n] = acc * MASS[n];
+}
+
+// surface extrapolation and surface-element sums (Process_buoyancy.c:158-193)
+__global__ void __launch_bounds__(TPB) k_hf_surface(Geo g, const float *__restrict__ ECO, const float *__restrict__ hf,
+                                                    double *__restrict__ out) {
+    __shared__ double sh[4 * 16];
+    double v[4] = {0, 0, 0, 0};
+    for (int f = threadIdx.x; f < g.elx * g.ely; f += blockDim.x) {
+        const int ex = f % g.elx, ey = f / g.elx;
+        double tb = 0.0, tt = 0.0;
+        const int cols[4] = {ex + g.nox * ey, ex + 1 + g.nox * ey, ex + 1 + g.nox * (ey + 1), ex + g.nox * (ey + 1)};
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int bot = cols[q] * g.noz, top = bot + g.noz - 1;
+            const float sh_ = 2 * hf[top] - hf[top - 1];
+            const float bh_ = 2 * hf[bot] - hf[bot + 1];
+            tb += bh_;
+            tt += sh_;
+        }
+        const int eb = g.elz * (ex + g.elx * ey), et = eb + g.elz - 1;
+        const float ab = ECO[(size_t)eb * ECO_SZ + 12], at = ECO[(size_t)et * ECO_SZ + 12];
+        v[0] += tb * ab;
+        v[1] += ab;
+        v[2] += tt * at;
+        v[3] += at;
+    }
+    block_sum<4>(v, sh);
+    if (threadIdx.x == 0)
+        for (int m = 0; m < 4; m++) out[m] = v[m];
+}
+
+void heat_flux(hipStream_t s, chip_ctx *c, const float *T, const float *V) {
+    const Level &L = c->L[c->levmax];
+    hipLaunchKernelGGL(k_hf_elem, dim3(nblk(L.g.nel)), dim3(TPB), 0, s, L.g, L.GNX, L.GDA, L.ECO, c->diffusivity, T, V,
+                       c->Eres);
+    hipLaunchKernelGGL(k_hf_gather, dim3(nblk(L.g.nno)), dim3(TPB), 0, s, L.g, L.TWW, L.MASS, c->Eres, c->hf);
+    hipLaunchKernelGGL(k_hf_surface, dim3(1), dim3(TPB), 0, s, L.g, L.ECO, c->hf, c->dots + 24);
+}
+
+}  // namespace chip

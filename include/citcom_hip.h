@@ -1,0 +1,164 @@
+/*
+ * citcom_hip.h -- C ABI of libcitcom_hip.so, the MI355X (gfx950) implementation
+ * of CitcomCU's timestep hot path: the Uzawa pressure CG around the full-
+ * multigrid velocity solve, and the Petrov-Galerkin (SUPG) energy step.
+ *
+ * Plain C types only.  Every entry point returns 0 on success and a negative
+ * code on failure; chip_last_error() then describes the failure.  Host arrays
+ * use the reference's numbering made 0-based:
+ *   node  n = k + noz*(i + nox*j)    (k: z index, i: x index, j: y index,
+ *                                     Construct_arrays.c:188, Nodal_mesh.c:182)
+ *   element e = ez + elz*(ex + elx*ey)        (Construct_arrays.c:70-72)
+ *   velocity dof 3n+d, d = 0:x 1:y 2:z        (ID doff, Construct_arrays.c:158-165)
+ * i.e. E->U[0..neq-1], E->P[1..npno], E->T[1..nno], E->V[d+1][1..nno] of the
+ * reference map to these arrays with the 1-based offset removed.
+ */
+#ifndef CITCOM_HIP_H
+#define CITCOM_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Problem description: the control data of struct All_variables that the hot
+ * path reads (global_defs.h:931-1102), set by read_instructions
+ * (Instructions.c:55-157) in the reference. */
+typedef struct chip_problem
+{
+	int levels;	/* levmax = levels-1, levmin = 0 (Instructions.c:593-598) */
+	int mgunitx, mgunity, mgunitz;
+	const float *xc, *yc, *zc;	/* finest-level node coordinates per axis */
+	int topvbc, botvbc;	/* 0: free slip, 1: prescribed (zero) velocity */
+	int toptbc, bottbc;	/* 1: fixed temperature */
+	float toptbcval, bottbcval;
+	double accuracy;	/* control.accuracy */
+	int p_iterations;	/* piterations */
+	int v_steps_low, v_steps_high, down_heavy, up_heavy, mg_cycle;
+	int aug_lagr;
+	double aug_number;
+	int precond;
+	int rheol, tdepv, num_mat;	/* viscosity law, Viscosity_structures.c:491-737 */
+	int visc_update;	/* VISC_UPDATE: rebuild viscosity and stiffness every step */
+	float visc0[4], viscE[4], viscT[4], z_layer[4];
+	int vmin_on, vmax_on;
+	float visc_min, visc_max;
+	float rayleigh, finetunedt;
+	int temp_iterations;	/* adv_sub_iterations */
+	float gamma;
+	double Q0;
+	float fixed_timestep;
+	int mixed;	/* 0: reference arithmetic (FP64 vectors, FP32 stored K);
+			   1: FP32 multigrid inside FP64 iterative refinement */
+} chip_problem;
+
+typedef struct chip_ctx chip_ctx;
+
+/* Uzawa monitors written by solve_Ahat_p_fhat (Stokes_flow_Incomp.c:370-446) */
+typedef struct chip_stokes_stats
+{
+	int iterations;	/* Uzawa iterations (count) */
+	int mg_cycles;	/* multi_grid calls over all inner solves */
+	int inner_solves;	/* solve_del2_u calls */
+	double vdotv, pdotp, incompressibility;
+	double dvelocity, dpressure;
+} chip_stokes_stats;
+
+typedef struct chip_pg_stats
+{
+	float timestep;	/* advection.timestep */
+	float T_interior;	/* monitor.T_interior (Tmax after the step) */
+	int sub_iterations;	/* advection.last_sub_iterations of the retry loop */
+	float elapsed_time;	/* monitor.elapsed_time */
+} chip_pg_stats;
+
+const char *chip_last_error(void);
+
+/* Builds the mesh of every multigrid level on the current HIP device:
+ * coordinates, boundary flags, Jacobians and shape-function derivatives,
+ * lumped mass and nodal weights.  Replaces the mesh part of
+ * read_instructions (Instructions.c:94-136): construct_ien/masks/id
+ * (Construct_arrays.c:52-211), node_locations (Nodal_mesh.c:53-283),
+ * velocity/temperature_boundary_conditions (Boundary_conditions.c:44-246),
+ * construct_shape_functions (Shape_functions.c:53-170), mass_matrix
+ * (Size_does_matter.c:618-757). */
+int chip_create(const chip_problem *p, chip_ctx **out);
+void chip_destroy(chip_ctx *c);
+
+/* Stream all work is queued on (a hipStream_t; NULL = the library's own). */
+int chip_set_stream(chip_ctx *c, void *hip_stream);
+void *chip_get_stream(chip_ctx *c);
+
+/* ---- state (host <-> device copies; sizes in elements) ------------------- */
+/* which: "T" float[nno], "Tdot" float[nno], "U" double[3nno], "P" double[nel],
+ * "F" double[3nno], "V" float[3*nno] (V[d*nno+n]), "buoyancy" float[nno],
+ * "EVI" float[8*nel(lev)] (lev = -1: finest), "K" float[nno(lev)*243] row
+ * layout [n][r*81 + slot*3 + c], slot = (dz+1)+3(dx+1)+9(dy+1),
+ * "BI" double[3nno(lev)], "BPI" double[nel(lev)] */
+int chip_upload(chip_ctx *c, const char *which, int lev, const void *host, long count);
+int chip_download(chip_ctx *c, const char *which, int lev, void *host, long count);
+
+/* ---- operator build (stiffness rebuild) ---------------------------------- */
+/* get_system_viscosity (Viscosity_structures.c:369-443) from the device T:
+ * visc_from_T / visc_from_mat and the MIN/MAX clip, into EVI[levmax]. */
+int chip_get_system_viscosity(chip_ctx *c);
+/* construct_stiffness_B_matrix (Construct_arrays.c:834-889): project_viscosity
+ * (Solver_multigrid.c:398-480), construct_elt_gs, construct_node_ks with
+ * get_elt_k/get_aug_k/build_diagonal_of_K (Construct_arrays.c:359-535,
+ * Element_calculations.c:133-293, 624-652, 1127-1163), build_diagonal_of_Ahat
+ * (Element_calculations.c:654-685), from EVI[levmax]. */
+int chip_construct_stiffness_B_matrix(chip_ctx *c);
+
+/* ---- single operators, for verification (host arrays in and out) --------- */
+/* op: "matvec" n_assemble_del2_u (Element_calculations.c:552-621, strip on),
+ *     "gs0"/"gs1" gauss_seidel guess 0/1 with arg sweeps, 8-colour
+ *         (General_matrix_functions.c:1160-1361); in = d0 (gs1), in2 = F,
+ *         out = d0 after, out2 = A d0
+ *     "project" project_vector lev -> lev-1 (Solver_multigrid.c:72-159)
+ *     "interp"  interp_vector lev-1 -> lev (Solver_multigrid.c:173-298)
+ *     "div" assemble_div_u (Element_calculations.c:691-720)
+ *     "grad" assemble_grad_p (Element_calculations.c:727-769)
+ *     "strip" strip_bcs_from_residual (Boundary_conditions.c:926-947) */
+int chip_apply(chip_ctx *c, const char *op, int lev, const double *in, const double *in2, double *out, double *out2, int arg);
+
+/* ---- drop-in #1: solve_constrained_flow_iterative ------------------------ */
+/* Stokes_flow_Incomp.c:50-90: Uzawa CG (solve_Ahat_p_fhat :295-497) on the
+ * device U, P with right-hand side F, then v_from_vector (:530-552) into V. */
+int chip_solve_constrained_flow_iterative(chip_ctx *c, chip_stokes_stats *st);
+/* solve_del2_u (General_matrix_functions.c:368-520) on host arrays, for
+ * verification: d0 = K^-1 F to max(acc, accuracy*|F|) */
+int chip_solve_del2_u(chip_ctx *c, const double *F, double *d0, double acc, int *valid, int *cycles);
+
+/* ---- drop-in #2: PG_timestep --------------------------------------------- */
+/* Advection_diffusion.c:251-345 with std_timestep, predictor, pg_solver,
+ * corrector, Tmax retry, temperatures_conform_bcs, thermal_buoyancy. */
+int chip_PG_timestep(chip_ctx *c, chip_pg_stats *st);
+
+/* ---- per-step inputs of the Stokes solve ---------------------------------- */
+/* thermal_buoyancy (Pan_problem_misc_functions.c:83-153) from the device T */
+int chip_thermal_buoyancy(chip_ctx *c);
+/* assemble_forces (Element_calculations.c:74-125) from the device buoyancy */
+int chip_assemble_forces(chip_ctx *c);
+/* heat_flux (Process_buoyancy.c:63-203): nu[0] = Nu_top, nu[1] = Nu_bot */
+int chip_heat_flux(chip_ctx *c, float *nu);
+
+/* One timestep of the reference's time loop (Citcom.c:111-161) with every
+ * stage on the device: PG_timestep, heat_flux, then general_stokes_solver
+ * (Drive_solvers.c:45-165: velocities_conform_bcs, assemble_forces, viscosity
+ * and stiffness update when the viscosity is temperature dependent, Uzawa). */
+int chip_timestep(chip_ctx *c, chip_pg_stats *pg, chip_stokes_stats *st, float *nu);
+/* general_stokes_solver alone (the step-0 solve of Citcom.c:97) */
+int chip_general_stokes_solver(chip_ctx *c, chip_stokes_stats *st);
+
+/* ---- measurement ---------------------------------------------------------- */
+/* kernel: "gs_colour" or "matvec".  While enabled, every launch of that kernel
+ * is bracketed by HIP events on the work stream; read returns the number of
+ * launches, their summed duration and the algorithmic bytes they moved. */
+int chip_profile_enable(chip_ctx *c, const char *kernel, int on);
+int chip_profile_read(chip_ctx *c, long *launches, double *total_ms, double *bytes);
+/* device memory held by the context, bytes */
+long chip_device_bytes(chip_ctx *c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

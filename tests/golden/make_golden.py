@@ -1,0 +1,121 @@
+"""Regenerate the golden fixtures in this directory from the reference.
+
+Runs oracle/_ref/ref_harness (the unmodified reference objects linked with a
+dump harness; build it with ``make -C oracle ref``) on the decks of
+citcom_amd.deck and stores what the tests compare against:
+
+* ``<deck>_setup.npz``  SHA-256 of every per-level array the reference
+  builds (geometry, flags, viscosity, stiffness expanded to full rows, BI,
+  BPI, elt_del) plus the finest-level fields; arrays are hashed because the
+  restatement and the HIP path reproduce them bit for bit.
+* ``<deck>_ops.npz``    outputs of n_assemble_del2_u, gauss_seidel,
+  project_vector, interp_vector, assemble_div_u, assemble_grad_p for the
+  harness's seeded inputs (regenerated in Python by ``xorshift``).
+* ``<deck>_run.npz``    the first Stokes solve, the first PG step and the
+  per-step (time, dt, Nu_top, Nu_bot) of a short run.
+* ``cfg1_run.npz``      the 16^3 isoviscous Ra=1e4 workload: 200 steps.
+
+Usage: python tests/golden/make_golden.py [deck ...]
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+
+from citcom_amd import deck  # noqa: E402
+import refdump  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+
+def sha(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def run_harness(deck_name: str, workdir: str, mode: str, arg: str | None = None) -> str:
+    keys = deck.workload(deck_name)
+    keys["datafile"] = os.path.join(workdir, "out", deck_name)
+    keys["oldfile"] = keys["datafile"]
+    os.makedirs(os.path.join(workdir, "out"), exist_ok=True)
+    path = os.path.join(workdir, f"{deck_name}.input")
+    deck.write_deck(path, keys)
+    out = os.path.join(workdir, f"{deck_name}_{mode}")
+    cmd = [HARNESS, path, out, mode] + ([arg] if arg else [])
+    subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, cwd=workdir)
+    return out
+
+
+def setup_fixture(name: str, d: str) -> dict:
+    p = deck.problem_from_deck({k: str(v) for k, v in deck.workload(name).items()})
+    out = {}
+    for lev in range(p.levels):
+        nox, noy, noz = p.dims(lev)
+        for key, fn in (("X", "X1.f32"), ("Y", "X2.f32"), ("Z", "X3.f32"), ("GNX", "GNX.f32"), ("GDA", "GDA.f32"),
+                        ("ECO", "ECO.f32"), ("MASS", "MASS.f32"), ("EVI", "EVI.f32"), ("EG", "EG.f32"),
+                        ("BI", "BI.f64"), ("BPI", "BPI.f64")):
+            out[f"sha_{key}_{lev}"] = sha(refdump.load(d, fn, lev))
+        out[f"sha_TWW_{lev}"] = sha(refdump.load(d, "TWW.f32", lev).reshape(-1, 9)[:, 1:])
+        out[f"sha_NODEV_{lev}"] = sha(refdump.load(d, "NODE.u32", lev) & np.uint32(0xE))
+        full = refdump.full_rows_from_half(*(refdump.load(d, f"K{i}.f32", lev) for i in (1, 2, 3)),
+                                           refdump.load(d, "MAP.i32", lev), nox, noy, noz)
+        out[f"sha_K_{lev}"] = sha(full)
+    for key, fn in (("T", "T.f32"), ("buoyancy", "buoyancy.f32"), ("F", "F.f64"), ("TB3", "TB3.f32")):
+        out[f"sha_{key}"] = sha(refdump.load(d, fn))
+    # velocity and temperature boundary bits (VBX|VBZ|VBY|TBX|TBZ|TBY)
+    out["sha_nodeflags"] = sha(refdump.load(d, "node.u32") & np.uint32(0x7E))
+    out["T"] = refdump.load(d, "T.f32")
+    out["F"] = refdump.load(d, "F.f64")
+    out["EVI"] = refdump.load(d, "EVI.f32", p.levmax)
+    return out
+
+
+def ops_fixture(name: str, d: str) -> dict:
+    p = deck.problem_from_deck({k: str(v) for k, v in deck.workload(name).items()})
+    out = {}
+    for lev in range(p.levels):
+        for fn in ("mv_out.f64", "gs0_d.f64", "gs0_Ad.f64", "gs1_d.f64", "gs1_Ad.f64", "div_out.f64",
+                   "grad_out.f64") + (("proj_out.f64", "interp_out.f64") if lev > 0 else ()):
+            out[f"{fn.split('.')[0]}_{lev}"] = refdump.load(d, fn, lev)
+    return out
+
+
+def run_fixture(d: str, fields=("U.f64", "P.f64", "T.f32", "Tdot.f32", "V1.f32", "V2.f32", "V3.f32")) -> dict:
+    out = {"steps": np.loadtxt(os.path.join(d, "steps.txt"), ndmin=2)}
+    for tag in ("step0", "pg1", "final"):
+        for fn in fields:
+            path = os.path.join(d, f"{tag}_{fn}")
+            if os.path.exists(path):
+                out[f"{tag}_{fn.split('.')[0]}"] = refdump.load(d, f"{tag}_{fn}")
+    return out
+
+
+def main(names):
+    if not os.path.exists(HARNESS):
+        raise SystemExit(f"{HARNESS} missing: make -C oracle ref")
+    with tempfile.TemporaryDirectory() as tmp:
+        for name in names:
+            if name == "cfg1":
+                d = run_harness(name, tmp, "run", "200")
+                np.savez_compressed(os.path.join(HERE, "cfg1_run.npz"), **run_fixture(d, ("U.f64", "P.f64", "T.f32")))
+                continue
+            d = run_harness(name, tmp, "setup")
+            np.savez_compressed(os.path.join(HERE, f"{name}_setup.npz"), **setup_fixture(name, d))
+            d = run_harness(name, tmp, "ops", "7")
+            np.savez_compressed(os.path.join(HERE, f"{name}_ops.npz"), **ops_fixture(name, d))
+            d = run_harness(name, tmp, "run", "3")
+            np.savez_compressed(os.path.join(HERE, f"{name}_run.npz"), **run_fixture(d))
+            print("wrote", name)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:] or ["tiny", "tiny_tdep", "cfg1"])

@@ -39,3 +39,45 @@ def full_rows_from_half(K1, K2, K3, MAP, nox, noy, noz):
             tslot = 26 - slot
             full[m, c, tslot, r] = Kd[ok, r, s]
     return full.reshape(nno, 243)
+
+
+class XorShift:
+    """The seeded generator of ref_harness.c (xorshift64, uniform(-1,1))."""
+
+    M = (1 << 64) - 1
+
+    def __init__(self, seed: int):
+        self.s = 88172645463325252 ^ ((seed * 0x9E3779B97F4A7C15) & self.M)
+
+    def draw(self, n: int) -> np.ndarray:
+        out = np.empty(n)
+        s, M = self.s, self.M
+        for i in range(n):
+            s ^= (s << 13) & M
+            s ^= s >> 7
+            s ^= (s << 17) & M
+            out[i] = ((s >> 11) / 9007199254740992.0) * 2.0 - 1.0
+        self.s = s
+        return out
+
+
+def harness_op_inputs(problem, seed: int, strip):
+    """Inputs ref_harness draws in ``ops`` mode, per level (finest first):
+    u, F (stripped), d0 (stripped), p (one zero entry), coarse vector."""
+    rng = XorShift(seed)
+    out = {}
+    for lev in range(problem.levels - 1, -1, -1):
+        nox, noy, noz = problem.dims(lev)
+        nno = nox * noy * noz
+        nel = (nox - 1) * (noy - 1) * (noz - 1)
+        u = rng.draw(3 * nno)
+        F = strip(lev, rng.draw(3 * nno))
+        d0 = strip(lev, rng.draw(3 * nno))
+        p = rng.draw(nel)
+        p[nel // 3] = 0.0
+        c = None
+        if lev > 0:
+            cx, cy, cz = problem.dims(lev - 1)
+            c = rng.draw(3 * cx * cy * cz)
+        out[lev] = {"u": u, "F": F, "d0": d0, "p": p, "c": c}
+    return out

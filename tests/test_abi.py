@@ -1,0 +1,37 @@
+"""The C ABI: every entry point include/citcom_hip.h declares is exported by
+libcitcom_hip.so, and the library refuses to run without a GPU (no CPU
+fallback)."""
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    text = open(os.path.join(ROOT, "include", "citcom_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(chip_[a-z_A-Z0-9]+)\s*\(", text)))
+
+
+def test_header_symbols_exported():
+    from citcom_amd import hip
+    assert os.path.exists(hip.LIB_PATH), "build the library first"
+    out = subprocess.run(["nm", "-D", "--defined-only", hip.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (chip_\w+)", out))
+    missing = [s for s in declared() if s not in exported]
+    assert not missing, missing
+    assert len(declared()) >= 20
+
+
+def test_no_cpu_fallback_without_device():
+    import torch  # noqa: F401  (same process state as the product)
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from citcom_amd import deck
+    from citcom_amd.hip import ChipError, Citcom
+    p = deck.problem_from_deck({k: str(v) for k, v in deck.workload("tiny").items()})
+    with pytest.raises(ChipError, match="no HIP device"):
+        Citcom(p)
