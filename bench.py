@@ -86,6 +86,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mixed", action="store_true", help="FP32 multigrid inside FP64 refinement")
+    ap.add_argument("--piterations", type=int, default=None,
+                    help="cap on Uzawa iterations (profiling runs only: changes the workload)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: the path shards by subdomain "
@@ -97,6 +99,8 @@ def main():
     from citcom_amd.hip import Citcom
 
     p = make_problem(args.config)
+    if args.piterations is not None:
+        p.p_iterations = args.piterations
     nox, noy, noz = p.dims()
     log(f"config {args.config}: {nox - 1}x{noy - 1}x{noz - 1} elements, {p.levels} levels")
     t0 = time.perf_counter()

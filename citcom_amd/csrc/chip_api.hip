@@ -555,7 +555,7 @@ int chip_apply(chip_ctx *c, const char *op, int lev, const double *in, const dou
         }
         const Level &C = c->L[lev - 1];
         CHIP_CHECK(up(a, in, nv));
-        project(q, L, C, a, C.AU);
+        project(q, L, C, a, C.AU, false);
         CHIP_CHECK(down(out, C.AU, 3L * C.g.nno));
     } else if (o == "interp") {
         if (lev == c->levmin) {
@@ -564,7 +564,7 @@ int chip_apply(chip_ctx *c, const char *op, int lev, const double *in, const dou
         }
         const Level &C = c->L[lev - 1];
         CHIP_CHECK(up(C.AU, in, 3L * C.g.nno));
-        interp(q, C, L, C.AU, a);
+        interp(q, C, L, C.AU, a, false);
         CHIP_CHECK(down(out, a, nv));
     } else if (o == "div") {
         CHIP_CHECK(up(a, in, nv));

@@ -24,8 +24,8 @@ void gs_colour(hipStream_t s, const Level &L, int colour, double *d, const doubl
 // every sweep of a small level in one workgroup, then Ad = K d
 void gs_small(hipStream_t s, const Level &L, double *d, const double *F, double *Ad, int sweeps, int guess);
 bool gs_small_ok(const Level &L);
-void project(hipStream_t s, const Level &fine, const Level &coarse, const double *AU, double *AD);
-void interp(hipStream_t s, const Level &coarse, const Level &fine, const double *AD, double *AU);
+void project(hipStream_t s, const Level &fine, const Level &coarse, const double *AU, double *AD, bool do_strip = true);
+void interp(hipStream_t s, const Level &coarse, const Level &fine, const double *AD, double *AU, bool do_strip = true);
 void div(hipStream_t s, const Level &L, const double *U, double *divU);
 void grad(hipStream_t s, const Level &L, const double *P, double *gradP);
 void strip(hipStream_t s, const Level &L, double *v);

@@ -242,10 +242,11 @@ void gs_small(hipStream_t s, const Level &L, double *d, const double *F, double 
 // ---------------------------------------------------------------------------
 // grid transfers
 
-// project_vector (Solver_multigrid.c:72-159) + strip of the coarse result
+// project_vector (Solver_multigrid.c:72-159), optionally followed by the
+// strip_bcs_from_residual that multi_grid applies to its result
 __global__ void __launch_bounds__(TPB) k_project(Geo gf, Geo gc, const float *__restrict__ TWW,
                                                  const float *__restrict__ MASS, const unsigned *__restrict__ NODE,
-                                                 const double *__restrict__ AU, double *__restrict__ AD) {
+                                                 const double *__restrict__ AU, double *__restrict__ AD, int do_strip) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= gc.nno) return;
     int k, i, j;
@@ -279,19 +280,20 @@ __global__ void __launch_bounds__(TPB) k_project(Geo gf, Geo gc, const float *__
     acc[0] = acc[0] * m;
     acc[1] = acc[1] * m;
     acc[2] = acc[2] * m;
-    strip3(NODE[n], acc);
+    if (do_strip) strip3(NODE[n], acc);
     AD[3 * n] = acc[0];
     AD[3 * n + 1] = acc[1];
     AD[3 * n + 2] = acc[2];
 }
 
-void project(hipStream_t s, const Level &fine, const Level &coarse, const double *AU, double *AD) {
+void project(hipStream_t s, const Level &fine, const Level &coarse, const double *AU, double *AD, bool do_strip) {
     hipLaunchKernelGGL(k_project, dim3(nblk(coarse.g.nno)), dim3(TPB), 0, s, fine.g, coarse.g, coarse.TWW,
-                       coarse.MASS, coarse.NODE, AU, AD);
+                       coarse.MASS, coarse.NODE, AU, AD, (int)do_strip);
 }
 
 // interp_vector + un_inject_vector (Solver_multigrid.c:173-298, 581-634):
-// injection, then linear fills along x, z, y in the reference's order, + strip
+// injection, then linear fills along x, z, y in the reference's order,
+// optionally followed by multi_grid's strip_bcs_from_residual
 __device__ __forceinline__ void line_x(const Geo &gc, const float *wx, const double *AD, int k, int i, int j,
                                        double v[3]) {
     // k, j even fine indices; i fine x index
@@ -324,7 +326,7 @@ __device__ __forceinline__ void plane_xz(const Geo &gc, const float *wx, const f
 
 __global__ void __launch_bounds__(TPB) k_interp(Geo gc, Geo gf, const float *__restrict__ iw,
                                                 const unsigned *__restrict__ NODE, const double *__restrict__ AD,
-                                                double *__restrict__ AU) {
+                                                double *__restrict__ AU, int do_strip) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= gf.nno) return;
     int k, i, j;
@@ -341,15 +343,15 @@ __global__ void __launch_bounds__(TPB) k_interp(Geo gc, Geo gf, const float *__r
 #pragma unroll
         for (int d = 0; d < 3; d++) v[d] = n1 * a[d] + n2 * b[d];
     }
-    strip3(NODE[n], v);
+    if (do_strip) strip3(NODE[n], v);
     AU[3 * n] = v[0];
     AU[3 * n + 1] = v[1];
     AU[3 * n + 2] = v[2];
 }
 
-void interp(hipStream_t s, const Level &coarse, const Level &fine, const double *AD, double *AU) {
+void interp(hipStream_t s, const Level &coarse, const Level &fine, const double *AD, double *AU, bool do_strip) {
     hipLaunchKernelGGL(k_interp, dim3(nblk(fine.g.nno)), dim3(TPB), 0, s, coarse.g, fine.g, fine.iw, fine.NODE, AD,
-                       AU);
+                       AU, (int)do_strip);
 }
 
 // ---------------------------------------------------------------------------
