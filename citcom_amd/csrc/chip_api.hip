@@ -470,7 +470,7 @@ int chip_upload(chip_ctx *c, const char *which, int lev, const void *host, long 
 int chip_download(chip_ctx *c, const char *which, int lev, void *host, long count) {
     const std::string w(which);
     if (w == "K") {
-        // device [slot*9 + r*3 + c][pitch] colour order -> host [n][r*81 + slot*3 + c]
+        // device tiles [q/64][slot*9 + r*3 + c][q%64], colour order -> host [n][r*81 + slot*3 + c]
         const Level &L = c->L[lev < 0 ? c->levmax : lev];
         const Geo &g = L.g;
         if (count != (long)g.nno * KROW) {
@@ -490,7 +490,8 @@ int chip_download(chip_ctx *c, const char *which, int lev, void *host, long coun
                 for (int s = 0; s < 27; s++)
                     for (int r = 0; r < 3; r++)
                         for (int cc = 0; cc < 3; cc++)
-                            out[(size_t)n * KROW + r * 81 + s * 3 + cc] = dev[(size_t)(s * 9 + r * 3 + cc) * g.pitch + q];
+                            out[(size_t)n * KROW + r * 81 + s * 3 + cc] =
+                                dev[(size_t)(q >> 6) * KROW * 64 + (size_t)(s * 9 + r * 3 + cc) * 64 + (q & 63)];
             }
         return 0;
     }

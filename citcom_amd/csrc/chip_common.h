@@ -35,7 +35,7 @@ struct Geo {
     int nox, noy, noz;
     int elx, ely, elz;
     int nno, nel;
-    int pitch;     // stride of one stiffness coefficient run (nno rounded up)
+    int pitch;     // nno rounded up to whole 64-node stiffness tiles
     int coff[9];   // first colour-ordered index of each colour, coff[8] = nno
     int cz[8], cx[8]; // nodes of colour c along z and x
 };
@@ -43,7 +43,7 @@ struct Geo {
 struct Level {
     Geo g;
     // operator
-    float *K = nullptr;      // [243][pitch], colour order
+    float *K = nullptr;      // [pitch/64][243][64]: 64-node tiles, colour order
     double *BI = nullptr;    // [3 nno]
     double *BPI = nullptr;   // [nel]
     float *EG = nullptr;     // [nel][24]

@@ -100,36 +100,56 @@ __device__ __forceinline__ int colour_index(const Geo &g, int k, int i, int j) {
     return g.coff[c] + (k >> 1) + g.cz[c] * ((i >> 1) + g.cx[c] * (j >> 1));
 }
 
+// Stiffness storage: 64-node tiles in colour order, each tile holding its
+// 243 coefficient runs of 64 floats back to back, so one wavefront reads
+// 243 contiguous 256-byte lines (62 KB) per row product.
+__device__ __forceinline__ size_t ktile(int q) { return (size_t)(q >> 6) * (KROW * 64) + (q & 63); }
+
 // full-row product of the node at colour index q, (k,i,j), with u.  The
 // neighbour order (dy, dx, dz ascending) and the per-block expression are
-// those of the CPU restatement.  Missing neighbours carry zero coefficients
-// and read the node's own entries instead.
-__device__ __forceinline__ void row_product(const Geo &g, const float *__restrict__ K, int q, int k, int i, int j,
-                                            const double *__restrict__ u, double out[3]) {
-    const size_t P = (size_t)g.pitch;
-    const float *kp = K + q;
+// those of the CPU restatement.  Loads are issued one dy-plane (9 blocks) at
+// a time so that each thread keeps 81 coefficients and 27 vector entries in
+// flight.  Missing neighbours carry zero coefficients and read the node's
+// own entries instead.
+template <typename KA, typename UA>
+__device__ __forceinline__ void row_product_t(const Geo &g, KA kat, int k, int i, int j, UA u, double out[3]) {
     const int self = nid(g, k, i, j);
     out[0] = out[1] = out[2] = 0.0;
 #pragma unroll
     for (int dy = -1; dy <= 1; dy++) {
         const bool oy = (unsigned)(j + dy) < (unsigned)g.noy;
+        float kv[81];
+        double uv[27];
 #pragma unroll
         for (int dx = -1; dx <= 1; dx++) {
             const bool ox = oy && (unsigned)(i + dx) < (unsigned)g.nox;
 #pragma unroll
             for (int dz = -1; dz <= 1; dz++) {
+                const int b = (dz + 1) + 3 * (dx + 1);
                 const bool ok = ox && (unsigned)(k + dz) < (unsigned)g.noz;
                 const int m = ok ? self + dz + g.noz * (dx + g.nox * dy) : self;
-                const double u0 = u[3 * m], u1 = u[3 * m + 1], u2 = u[3 * m + 2];
+                uv[3 * b] = u(3 * m);
+                uv[3 * b + 1] = u(3 * m + 1);
+                uv[3 * b + 2] = u(3 * m + 2);
+                const int s0 = 9 * (b + 9 * (dy + 1));
 #pragma unroll
-                for (int r = 0; r < 3; r++) {
-                    const float a0 = kp[0], a1 = kp[P], a2 = kp[2 * P];
-                    kp += 3 * P;
-                    out[r] += a0 * u0 + a1 * u1 + a2 * u2;
-                }
+                for (int t = 0; t < 9; t++) kv[9 * b + t] = kat(s0 + t);
             }
         }
+#pragma unroll
+        for (int b = 0; b < 9; b++)
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+                out[r] += kv[9 * b + 3 * r] * uv[3 * b] + kv[9 * b + 3 * r + 1] * uv[3 * b + 1] +
+                          kv[9 * b + 3 * r + 2] * uv[3 * b + 2];
     }
+}
+
+__device__ __forceinline__ void row_product(const Geo &g, const float *__restrict__ K, int q, int k, int i, int j,
+                                            const double *__restrict__ u, double out[3]) {
+    const float *kp = K + ktile(q);
+    row_product_t(
+        g, [kp](int ci) { return kp[ci * 64]; }, k, i, j, [u](int idx) { return u[idx]; }, out);
 }
 
 __device__ __forceinline__ void strip3(unsigned f, double v[3]) {
@@ -235,7 +255,87 @@ __global__ void __launch_bounds__(SMALL_TPB) k_gs_small(Geo g, const float *__re
     }
 }
 
+// The smallest levels: stiffness, iterate, right-hand side and BI staged in
+// LDS once, then every sweep and colour pass runs from LDS in one workgroup.
+constexpr int LDS_TPB = 512;
+
+__host__ __device__ inline size_t lds_bytes(int nno) {
+    return (((size_t)KROW * nno * 4 + 15) / 16) * 16 + (size_t)9 * nno * 8;
+}
+
+__global__ void __launch_bounds__(LDS_TPB) k_gs_lds(Geo g, const float *__restrict__ K, const double *__restrict__ BI,
+                                                    const unsigned *__restrict__ NODE, double *d,
+                                                    const double *__restrict__ F, double *__restrict__ Ad, int sweeps,
+                                                    int guess) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int nno = g.nno, tid = threadIdx.x;
+    float *Ks = (float *)smem;
+    double *ds = (double *)(smem + (((size_t)KROW * nno * 4 + 15) / 16) * 16);
+    double *Fs = ds + 3 * nno, *Bs = Fs + 3 * nno;
+    for (int idx = tid; idx < KROW * nno; idx += blockDim.x) {
+        const int ci = idx / nno, q = idx - ci * nno;
+        Ks[idx] = K[ktile(q) + ci * 64];
+    }
+    for (int idx = tid; idx < 3 * nno; idx += blockDim.x) {
+        ds[idx] = guess ? d[idx] : 0.0;
+        Fs[idx] = F[idx];
+        Bs[idx] = BI[idx];
+    }
+    __syncthreads();
+    for (int cyc = 0; cyc < sweeps; cyc++)
+        for (int c = 0; c < 8; c++) {
+            for (int q = g.coff[c] + tid; q < g.coff[c + 1]; q += blockDim.x) {
+                int k, i, j;
+                colour_kij(g, c, q, k, i, j);
+                double out[3];
+                row_product_t(
+                    g, [Ks, q, nno](int ci) { return Ks[ci * nno + q]; }, k, i, j,
+                    [ds](int idx) { return ds[idx]; }, out);
+                const int n = nid(g, k, i, j);
+#pragma unroll
+                for (int r = 0; r < 3; r++) {
+                    const float t = (float)((Fs[3 * n + r] - out[r]) * Bs[3 * n + r]);
+                    ds[3 * n + r] += t;
+                }
+            }
+            __syncthreads();
+        }
+    for (int q = tid; q < nno; q += blockDim.x) {
+        int k, i, j;
+        colour_kij(g, colour_of(g, q), q, k, i, j);
+        double out[3];
+        row_product_t(
+            g, [Ks, q, nno](int ci) { return Ks[ci * nno + q]; }, k, i, j, [ds](int idx) { return ds[idx]; }, out);
+        const int n = nid(g, k, i, j);
+        strip3(NODE[n], out);
+        Ad[3 * n] = out[0];
+        Ad[3 * n + 1] = out[1];
+        Ad[3 * n + 2] = out[2];
+    }
+    for (int idx = tid; idx < 3 * nno; idx += blockDim.x) d[idx] = ds[idx];
+}
+
+static size_t lds_limit() {
+    static size_t lim = 0;
+    if (!lim) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+            lim = (size_t)v;
+        else
+            lim = 65536;
+        if (lim > 65536)
+            (void)hipFuncSetAttribute((const void *)k_gs_lds, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim);
+    }
+    return lim;
+}
+
 void gs_small(hipStream_t s, const Level &L, double *d, const double *F, double *Ad, int sweeps, int guess) {
+    const size_t need = lds_bytes(L.g.nno);
+    if (need <= lds_limit()) {
+        hipLaunchKernelGGL(k_gs_lds, dim3(1), dim3(LDS_TPB), need, s, L.g, L.K, L.BI, L.NODE, d, F, Ad, sweeps, guess);
+        return;
+    }
     hipLaunchKernelGGL(k_gs_small, dim3(1), dim3(SMALL_TPB), 0, s, L.g, L.K, L.BI, L.NODE, d, F, Ad, sweeps, guess);
 }
 
@@ -856,7 +956,7 @@ __global__ void __launch_bounds__(128) k_node_ks(Geo g, const float *__restrict_
     int k, i, j;
     colour_kij(g, colour_of(g, q), q, k, i, j);
     const int n = nid(g, k, i, j);
-    const size_t P = (size_t)g.pitch;
+    float *kq = K + ktile(q);
     double diag[3] = {0.0, 0.0, 0.0};
     for (int dy = -1; dy <= 1; dy++)
         for (int dx = -1; dx <= 1; dx++)
@@ -917,7 +1017,7 @@ __global__ void __launch_bounds__(128) k_node_ks(Geo g, const float *__restrict_
 #pragma unroll
                 for (int r = 0; r < 3; r++)
 #pragma unroll
-                    for (int c = 0; c < 3; c++) K[(size_t)(s * 9 + r * 3 + c) * P + q] = acc[r][c];
+                    for (int c = 0; c < 3; c++) kq[(s * 9 + r * 3 + c) * 64] = acc[r][c];
             }
     BI[3 * n] = (float)1.0 / diag[0];
     BI[3 * n + 1] = (float)1.0 / diag[1];
