@@ -264,7 +264,14 @@ def workload(name: str) -> dict[str, object]:
     cfg3  128x128x64 elements in a 2x2x1 box, 6 levels, Ra=1e7,
           eta = exp(-ln(1e4) T) clipped to [1e-4, 1], CFL 0.5
     cfg5  256x256x128 elements, 7 levels, Ra=1e7, contrast 1e5, top layer x100
+
+    ``name@acc`` is the same workload with ``accuracy=acc`` (e.g. cfg1@1e-8).
     """
+    if "@" in name:
+        base, acc = name.split("@", 1)
+        d = workload(base)
+        d["accuracy"] = float(acc)
+        return d
     d = dict(COMMON)
     if name == "tiny":
         d.update(_grid((1.0, 1.0, 0.5), (2, 2, 1), 3))

@@ -106,6 +106,9 @@ def test_matches_reference_operators(pair):
 
 
 def test_stokes_against_reference(pair):
+    """accuracy 1e-6 on both sides: the 8-colour and the lexicographic
+    smoother stop at different iterates, each within its relative-increment
+    tolerance of the discrete solution, so they agree to O(1e-6)."""
     name, p, c, o, T = pair
     g = golden(f"{name}_run.npz")
     c.upload("U", np.zeros(3 * c.nno()))
@@ -113,12 +116,30 @@ def test_stokes_against_reference(pair):
     c.thermal_buoyancy()
     c.assemble_forces()
     st = c.solve_constrained_flow_iterative()
-    say(name, "uzawa", st)
     U, P = c.download("U"), c.download("P")
-    assert rel_l2(U, g["step0_U"]) < 1e-6, rel_l2(U, g["step0_U"])
-    assert rel_l2(P, g["step0_P"]) < 1e-6, rel_l2(P, g["step0_P"])
+    eu, ep = rel_l2(U, g["step0_U"]), rel_l2(P, g["step0_P"])
+    say(name, "uzawa", st["iterations"], "its; rel L2 vs reference at accuracy 1e-6: U", eu, "P", ep)
+    assert eu < 1e-5 and ep < 1e-5
     V = c.download("V")
     assert np.array_equal(V, U.reshape(-1, 3).T.astype(np.float32).reshape(-1))
+
+
+@pytest.mark.parametrize("name", DECKS)
+def test_stokes_converged_parity(name):
+    """Both solvers converged well below the bar (accuracy 1e-9): velocity and
+    pressure agree with the reference's to 1e-6 relative L2."""
+    from citcom_amd import deck
+    from citcom_amd.hip import Citcom
+    p = problem(f"{name}@1e-9")
+    g = golden(f"{name}_at_1e-9_run.npz")
+    c = Citcom(p)
+    c.initialize(deck.perturbed_temperature(p))
+    st = c.general_stokes_solver()
+    U, P = c.download("U"), c.download("P")
+    eu, ep = rel_l2(U, g["step0_U"]), rel_l2(P, g["step0_P"])
+    say(name, "accuracy 1e-9:", st["iterations"], "Uzawa its; rel L2 U", eu, "P", ep)
+    assert eu < 1e-6 and ep < 1e-6
+    c.close()
 
 
 def test_pg_and_nu_against_reference(pair):
