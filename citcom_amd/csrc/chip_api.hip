@@ -122,7 +122,8 @@ void do_gs(chip_ctx *c, int lev, double *d, const double *F, double *Ad, int swe
 // side on entry and the residual on exit; the correction is left in
 // L[levmax].vel.  Coarse right-hand sides share storage with fl[] (rhs[lev]
 // is a copy of fl[lev] in the reference, and fl[lev] is not read again).
-int multi_grid(chip_ctx *c, double *F, double *residual) {
+// This enqueues the whole fixed kernel sequence; |F|^2 lands in dots[D_TMP].
+void enqueue_multi_grid(chip_ctx *c, double *F) {
     const int levmin = c->levmin, levmax = c->levmax;
     const chip_problem &p = c->p;
     Level *L = c->L;
@@ -155,8 +156,25 @@ int multi_grid(chip_ctx *c, double *F, double *residual) {
     const long neq = 3L * L[levmax].g.nno;
     copy(S(c), F, L[levmax].res, neq);
     dots(S(c), c, (int)neq, 1, F, F, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, c->dots + D_TMP);
+}
+
+// F is always c->sr here, so one captured graph serves every call
+int multi_grid(chip_ctx *c, double *F, double *residual) {
+    if (c->use_graphs && !c->prof.kernel && F == c->sr) {
+        if (!c->mg_graph) {
+            hipGraph_t graph;
+            CHIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            enqueue_multi_grid(c, F);
+            CHIP_CHECK(hipStreamEndCapture(c->stream, &graph));
+            CHIP_CHECK(hipGraphInstantiate(&c->mg_graph, graph, nullptr, nullptr, 0));
+            CHIP_CHECK(hipGraphDestroy(graph));
+        }
+        CHIP_CHECK(hipGraphLaunch(c->mg_graph, c->stream));
+    } else {
+        enqueue_multi_grid(c, F);
+    }
     if (sync_dots(c)) return -1;
-    *residual = std::sqrt(c->h_dots[D_TMP] / neq);
+    *residual = std::sqrt(c->h_dots[D_TMP] / (3.0 * c->L[c->levmax].g.nno));
     return 0;
 }
 
@@ -379,6 +397,7 @@ int chip_create(const chip_problem *p, chip_ctx **out) {
         delete c;
         return -1;
     }
+    (void)lds_limit();
     if (hipStreamCreate(&c->stream) != hipSuccess) {
         set_error("chip_create: stream");
         delete c;
@@ -396,6 +415,7 @@ int chip_create(const chip_problem *p, chip_ctx **out) {
 void chip_destroy(chip_ctx *c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->mg_graph) (void)hipGraphExecDestroy(c->mg_graph);
     for (void *ptr : c->allocs) (void)hipFree(ptr);
     if (c->h_dots) (void)hipHostFree(c->h_dots);
     if (c->h_minmax) (void)hipHostFree(c->h_minmax);
@@ -656,5 +676,15 @@ int chip_profile_read(chip_ctx *c, long *launches, double *total_ms, double *byt
 }
 
 long chip_device_bytes(chip_ctx *c) { return c->dev_bytes; }
+
+int chip_set_option(chip_ctx *c, const char *name, int value) {
+    const std::string n(name ? name : "");
+    if (n == "graphs") {
+        c->use_graphs = value != 0;
+        return 0;
+    }
+    set_error("chip_set_option: unknown option " + n);
+    return -1;
+}
 
 }  // extern "C"

@@ -107,6 +107,8 @@ struct chip_ctx {
     float elapsed_time = 0.0f;
     int solution_cycles = 0;
     bool built = false;      // operators constructed at least once
+    bool use_graphs = true;  // replay multi_grid as a captured HIP graph
+    hipGraphExec_t mg_graph = nullptr;
     chip::Prof prof;
     long dev_bytes = 0;
     std::vector<void *> allocs;

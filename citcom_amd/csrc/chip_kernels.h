@@ -8,6 +8,9 @@ namespace chip {
 
 // master-element constants into __constant__ memory
 int upload_constants();
+// largest LDS allocation per workgroup; also raises the kernels' dynamic LDS
+// limit once, outside any stream capture
+size_t lds_limit();
 
 // ---- stiffness rebuild --------------------------------------------------
 void visc_from_T(hipStream_t s, const chip_ctx *c, const float *T, float *evi);

@@ -218,104 +218,133 @@ void gs_colour(hipStream_t s, const Level &L, int colour, double *d, const doubl
     hipLaunchKernelGGL(k_gs_colour, dim3(nblk(n)), dim3(TPB), 0, s, L.g, colour, L.K, L.BI, d, F);
 }
 
-constexpr int SMALL_TPB = 512;
+// levels up to this size run every sweep in one workgroup (gs_small)
 constexpr int SMALL_NNO = 6144;
 
 bool gs_small_ok(const Level &L) { return L.g.nno <= SMALL_NNO; }
 
-__global__ void __launch_bounds__(SMALL_TPB) k_gs_small(Geo g, const float *__restrict__ K,
-                                                        const double *__restrict__ BI,
-                                                        const unsigned *__restrict__ NODE, double *d,
-                                                        const double *__restrict__ F, double *__restrict__ Ad,
-                                                        int sweeps, int guess) {
-    const int tid = threadIdx.x;
-    if (!guess) {
-        for (int n = tid; n < 3 * g.nno; n += blockDim.x) d[n] = 0.0;
-        __syncthreads();
-    }
-    for (int cyc = 0; cyc < sweeps; cyc++)
-        for (int c = 0; c < 8; c++) {
-            for (int q = g.coff[c] + tid; q < g.coff[c + 1]; q += blockDim.x) {
-                int k, i, j;
-                colour_kij(g, c, q, k, i, j);
-                node_update(g, K, BI, q, k, i, j, d, F);
-            }
-            __syncthreads();
-        }
-    for (int q = tid; q < g.nno; q += blockDim.x) {
-        int k, i, j;
-        colour_kij(g, colour_of(g, q), q, k, i, j);
-        double out[3];
-        row_product(g, K, q, k, i, j, d, out);
-        const int n = nid(g, k, i, j);
-        strip3(NODE[n], out);
-        Ad[3 * n] = out[0];
-        Ad[3 * n + 1] = out[1];
-        Ad[3 * n + 2] = out[2];
-    }
-}
+// Small levels run all their sweeps in one workgroup.  A colour pass there
+// has only a few nodes, so each row product is spread over 27 lanes (one per
+// neighbour block); three lanes then add the 27 block terms in the
+// neighbour order of row_product, which keeps the sums bit-identical to the
+// one-thread-per-node kernels.  On the smallest levels the stiffness,
+// iterate, right-hand side and BI are staged in LDS first.
+constexpr int WIDE_TPB = 1024;
+constexpr int WIDE_GRP = 32;   // lanes per node (27 used)
 
-// The smallest levels: stiffness, iterate, right-hand side and BI staged in
-// LDS once, then every sweep and colour pass runs from LDS in one workgroup.
-constexpr int LDS_TPB = 512;
-
-__host__ __device__ inline size_t lds_bytes(int nno) {
+__host__ __device__ inline size_t wide_scratch_bytes() { return (size_t)(WIDE_TPB / WIDE_GRP) * 27 * 3 * 8; }
+__host__ __device__ inline size_t lds_stage_bytes(int nno) {
     return (((size_t)KROW * nno * 4 + 15) / 16) * 16 + (size_t)9 * nno * 8;
 }
 
-__global__ void __launch_bounds__(LDS_TPB) k_gs_lds(Geo g, const float *__restrict__ K, const double *__restrict__ BI,
-                                                    const unsigned *__restrict__ NODE, double *d,
-                                                    const double *__restrict__ F, double *__restrict__ Ad, int sweeps,
-                                                    int guess) {
+template <bool STAGED>
+__global__ void __launch_bounds__(WIDE_TPB) k_gs_wide(Geo g, const float *__restrict__ K, const double *__restrict__ BI,
+                                                      const unsigned *__restrict__ NODE, double *d,
+                                                      const double *__restrict__ F, double *__restrict__ Ad,
+                                                      int sweeps, int guess) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int nno = g.nno, tid = threadIdx.x;
-    float *Ks = (float *)smem;
-    double *ds = (double *)(smem + (((size_t)KROW * nno * 4 + 15) / 16) * 16);
-    double *Fs = ds + 3 * nno, *Bs = Fs + 3 * nno;
-    for (int idx = tid; idx < KROW * nno; idx += blockDim.x) {
-        const int ci = idx / nno, q = idx - ci * nno;
-        Ks[idx] = K[ktile(q) + ci * 64];
-    }
-    for (int idx = tid; idx < 3 * nno; idx += blockDim.x) {
-        ds[idx] = guess ? d[idx] : 0.0;
-        Fs[idx] = F[idx];
-        Bs[idx] = BI[idx];
+    const int grp = tid / WIDE_GRP, lane = tid % WIDE_GRP, ngrp = blockDim.x / WIDE_GRP;
+    double *T = (double *)smem;  // [ngrp][27][3]
+    float *Ks = nullptr;
+    double *dv = d;
+    const double *Fv = F, *Bv = BI;
+    if (STAGED) {
+        unsigned char *base = smem + wide_scratch_bytes();
+        Ks = (float *)base;
+        double *ds = (double *)(base + (((size_t)KROW * nno * 4 + 15) / 16) * 16);
+        double *Fs = ds + 3 * nno, *Bs = Fs + 3 * nno;
+        for (int idx = tid; idx < KROW * nno; idx += blockDim.x) {
+            const int ci = idx / nno, q = idx - ci * nno;
+            Ks[idx] = K[ktile(q) + ci * 64];
+        }
+        for (int idx = tid; idx < 3 * nno; idx += blockDim.x) {
+            ds[idx] = guess ? d[idx] : 0.0;
+            Fs[idx] = F[idx];
+            Bs[idx] = BI[idx];
+        }
+        dv = ds;
+        Fv = Fs;
+        Bv = Bs;
+    } else if (!guess) {
+        for (int idx = tid; idx < 3 * nno; idx += blockDim.x) d[idx] = 0.0;
     }
     __syncthreads();
+    const int s = lane, dz = s % 3 - 1, dx = (s / 3) % 3 - 1, dy = s / 9 - 1;
+    // one round: groups take consecutive colour-ordered nodes [q0, q1)
+    auto block_terms = [&](int q, int k, int i, int j) {
+        if (lane < 27) {
+            const bool ok = (unsigned)(k + dz) < (unsigned)g.noz && (unsigned)(i + dx) < (unsigned)g.nox &&
+                            (unsigned)(j + dy) < (unsigned)g.noy;
+            const int self = nid(g, k, i, j);
+            const int m = ok ? self + dz + g.noz * (dx + g.nox * dy) : self;
+            const double u0 = dv[3 * m], u1 = dv[3 * m + 1], u2 = dv[3 * m + 2];
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                float a0, a1, a2;
+                if (STAGED) {
+                    a0 = Ks[(s * 9 + 3 * r) * nno + q];
+                    a1 = Ks[(s * 9 + 3 * r + 1) * nno + q];
+                    a2 = Ks[(s * 9 + 3 * r + 2) * nno + q];
+                } else {
+                    const float *kp = K + ktile(q);
+                    a0 = kp[(s * 9 + 3 * r) * 64];
+                    a1 = kp[(s * 9 + 3 * r + 1) * 64];
+                    a2 = kp[(s * 9 + 3 * r + 2) * 64];
+                }
+                T[(grp * 27 + s) * 3 + r] = a0 * u0 + a1 * u1 + a2 * u2;
+            }
+        }
+    };
+    auto row_sum = [&](int r) {
+        double out = 0.0;
+        for (int t = 0; t < 27; t++) out += T[(grp * 27 + t) * 3 + r];
+        return out;
+    };
     for (int cyc = 0; cyc < sweeps; cyc++)
         for (int c = 0; c < 8; c++) {
-            for (int q = g.coff[c] + tid; q < g.coff[c + 1]; q += blockDim.x) {
-                int k, i, j;
-                colour_kij(g, c, q, k, i, j);
-                double out[3];
-                row_product_t(
-                    g, [Ks, q, nno](int ci) { return Ks[ci * nno + q]; }, k, i, j,
-                    [ds](int idx) { return ds[idx]; }, out);
-                const int n = nid(g, k, i, j);
-#pragma unroll
-                for (int r = 0; r < 3; r++) {
-                    const float t = (float)((Fs[3 * n + r] - out[r]) * Bs[3 * n + r]);
-                    ds[3 * n + r] += t;
+            for (int q0 = g.coff[c]; q0 < g.coff[c + 1]; q0 += ngrp) {
+                const int q = q0 + grp;
+                const bool act = q < g.coff[c + 1];
+                int k = 0, i = 0, j = 0;
+                if (act) {
+                    colour_kij(g, c, q, k, i, j);
+                    block_terms(q, k, i, j);
                 }
+                __syncthreads();
+                if (act && lane < 3) {
+                    const double out = row_sum(lane);
+                    const int n = nid(g, k, i, j);
+                    const float t = (float)((Fv[3 * n + lane] - out) * Bv[3 * n + lane]);
+                    dv[3 * n + lane] += t;
+                }
+                __syncthreads();
             }
-            __syncthreads();
         }
-    for (int q = tid; q < nno; q += blockDim.x) {
-        int k, i, j;
-        colour_kij(g, colour_of(g, q), q, k, i, j);
-        double out[3];
-        row_product_t(
-            g, [Ks, q, nno](int ci) { return Ks[ci * nno + q]; }, k, i, j, [ds](int idx) { return ds[idx]; }, out);
-        const int n = nid(g, k, i, j);
-        strip3(NODE[n], out);
-        Ad[3 * n] = out[0];
-        Ad[3 * n + 1] = out[1];
-        Ad[3 * n + 2] = out[2];
+    // Ad = K d, BC rows stripped
+    for (int q0 = 0; q0 < nno; q0 += ngrp) {
+        const int q = q0 + grp;
+        const bool act = q < nno;
+        int k = 0, i = 0, j = 0;
+        if (act) {
+            colour_kij(g, colour_of(g, q), q, k, i, j);
+            block_terms(q, k, i, j);
+        }
+        __syncthreads();
+        if (act && lane < 3) {
+            double out = row_sum(lane);
+            const int n = nid(g, k, i, j);
+            const unsigned f = NODE[n];
+            if ((lane == 0 && (f & VBX)) || (lane == 1 && (f & VBY)) || (lane == 2 && (f & VBZ))) out = 0.0;
+            Ad[3 * n + lane] = out;
+        }
+        __syncthreads();
     }
-    for (int idx = tid; idx < 3 * nno; idx += blockDim.x) d[idx] = ds[idx];
+    if (STAGED)
+        for (int idx = tid; idx < 3 * nno; idx += blockDim.x) d[idx] = dv[idx];
 }
 
-static size_t lds_limit() {
+size_t lds_limit() {
     static size_t lim = 0;
     if (!lim) {
         int dev = 0, v = 0;
@@ -324,19 +353,23 @@ static size_t lds_limit() {
             lim = (size_t)v;
         else
             lim = 65536;
-        if (lim > 65536)
-            (void)hipFuncSetAttribute((const void *)k_gs_lds, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim);
+        if (lim > 65536) {
+            (void)hipFuncSetAttribute((const void *)k_gs_wide<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim);
+            (void)hipFuncSetAttribute((const void *)k_gs_wide<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim);
+        }
     }
     return lim;
 }
 
 void gs_small(hipStream_t s, const Level &L, double *d, const double *F, double *Ad, int sweeps, int guess) {
-    const size_t need = lds_bytes(L.g.nno);
-    if (need <= lds_limit()) {
-        hipLaunchKernelGGL(k_gs_lds, dim3(1), dim3(LDS_TPB), need, s, L.g, L.K, L.BI, L.NODE, d, F, Ad, sweeps, guess);
+    const size_t staged = wide_scratch_bytes() + lds_stage_bytes(L.g.nno);
+    if (staged <= lds_limit()) {
+        hipLaunchKernelGGL(k_gs_wide<true>, dim3(1), dim3(WIDE_TPB), staged, s, L.g, L.K, L.BI, L.NODE, d, F, Ad,
+                           sweeps, guess);
         return;
     }
-    hipLaunchKernelGGL(k_gs_small, dim3(1), dim3(SMALL_TPB), 0, s, L.g, L.K, L.BI, L.NODE, d, F, Ad, sweeps, guess);
+    hipLaunchKernelGGL(k_gs_wide<false>, dim3(1), dim3(WIDE_TPB), wide_scratch_bytes(), s, L.g, L.K, L.BI, L.NODE, d,
+                       F, Ad, sweeps, guess);
 }
 
 // ---------------------------------------------------------------------------

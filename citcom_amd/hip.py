@@ -99,6 +99,7 @@ def lib():
         L.chip_general_stokes_solver.argtypes = [P, C.POINTER(StokesStats)]
         L.chip_profile_enable.argtypes = [P, C.c_char_p, C.c_int]
         L.chip_profile_read.argtypes = [P, C.POINTER(C.c_long), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.chip_set_option.argtypes = [P, C.c_char_p, C.c_int]
         L.chip_device_bytes.restype = C.c_long
         L.chip_device_bytes.argtypes = [P]
         _lib = L
@@ -274,6 +275,9 @@ class Citcom:
 
     def set_stream(self, handle: int) -> None:
         _check(lib().chip_set_stream(self.h, C.c_void_p(handle)), "set_stream")
+
+    def set_option(self, name: str, value: int) -> None:
+        _check(lib().chip_set_option(self.h, name.encode(), int(value)), f"set_option {name}")
 
     def device_bytes(self) -> int:
         return lib().chip_device_bytes(self.h)

@@ -157,6 +157,9 @@ int chip_profile_enable(chip_ctx *c, const char *kernel, int on);
 int chip_profile_read(chip_ctx *c, long *launches, double *total_ms, double *bytes);
 /* device memory held by the context, bytes */
 long chip_device_bytes(chip_ctx *c);
+/* "graphs": 1 (default) replays each multigrid call as one captured HIP
+ * graph, 0 launches its kernels one by one; results are identical. */
+int chip_set_option(chip_ctx *c, const char *name, int value);
 
 #ifdef __cplusplus
 }
