@@ -86,6 +86,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mixed", action="store_true", help="FP32 multigrid inside FP64 refinement")
+    ap.add_argument("--no-graphs", action="store_true", help="launch multigrid kernels one by one")
     ap.add_argument("--piterations", type=int, default=None,
                     help="cap on Uzawa iterations (profiling runs only: changes the workload)")
     args = ap.parse_args()
@@ -105,6 +106,8 @@ def main():
     log(f"config {args.config}: {nox - 1}x{noy - 1}x{noz - 1} elements, {p.levels} levels")
     t0 = time.perf_counter()
     c = Citcom(p, mixed=args.mixed)
+    if args.no_graphs:
+        c.set_option("graphs", 0)
     T = deck.seeded_temperature(p, seed=args.seed) if p.tdepv else deck.perturbed_temperature(p)
     c.initialize(T)
     log(f"setup {time.perf_counter() - t0:.1f} s, device memory {c.device_bytes() / 2**30:.2f} GiB")
