@@ -112,7 +112,7 @@ void do_gs(chip_ctx *c, int lev, double *d, const double *F, double *Ad, int swe
     if (!guess) fill(S(c), d, 0.0, 3L * L.g.nno);
     for (int cyc = 0; cyc < sweeps; cyc++)
         for (int col = 0; col < 8; col++) {
-            ProfGuard pg(c, 1, GS_BYTES * (L.g.coff[col + 1] - L.g.coff[col]));
+            ProfGuard pg(c, 1, GS_BYTES * L.g.cn[col]);
             gs_colour(S(c), L, col, d, F);
         }
     do_matvec(c, lev, d, Ad);
@@ -502,8 +502,8 @@ int chip_download(chip_ctx *c, const char *which, int lev, void *host, long coun
         CHIP_CHECK(hipStreamSynchronize(c->stream));
         float *out = (float *)host;
         for (int col = 0; col < 8; col++)
-            for (int q = g.coff[col]; q < g.coff[col + 1]; q++) {
-                const int r0 = q - g.coff[col], kk = r0 % g.cz[col], t = r0 / g.cz[col];
+            for (int q = g.cs[col]; q < g.cs[col] + g.cn[col]; q++) {
+                const int r0 = q - g.cs[col], kk = r0 % g.cz[col], t = r0 / g.cz[col];
                 const int k = 2 * kk + (col & 1), i = 2 * (t % g.cx[col]) + ((col >> 1) & 1),
                           j = 2 * (t / g.cx[col]) + ((col >> 2) & 1);
                 const int n = k + g.noz * (i + g.nox * j);

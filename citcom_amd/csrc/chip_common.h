@@ -35,8 +35,9 @@ struct Geo {
     int nox, noy, noz;
     int elx, ely, elz;
     int nno, nel;
-    int pitch;     // nno rounded up to whole 64-node stiffness tiles
-    int coff[9];   // first colour-ordered index of each colour, coff[8] = nno
+    int pitch;     // size of the colour-ordered index space (whole 64-node tiles)
+    int cs[9];     // first colour-ordered index of colour c (64-aligned), cs[8] = pitch
+    int cn[8];     // nodes of colour c; indices cs[c]+cn[c] .. cs[c+1]-1 are padding
     int cz[8], cx[8]; // nodes of colour c along z and x
 };
 

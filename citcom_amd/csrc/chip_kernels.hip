@@ -79,7 +79,7 @@ __device__ __forceinline__ void node_kij(const Geo &g, int n, int &k, int &i, in
 }
 
 __device__ __forceinline__ void colour_kij(const Geo &g, int c, int q, int &k, int &i, int &j) {
-    const int r = q - g.coff[c];
+    const int r = q - g.cs[c];
     const int nz = g.cz[c], nx = g.cx[c];
     const int kk = r % nz, t = r / nz;
     k = 2 * kk + (c & 1);
@@ -90,14 +90,15 @@ __device__ __forceinline__ void colour_kij(const Geo &g, int c, int q, int &k, i
 __device__ __forceinline__ int colour_of(const Geo &g, int q) {
     int c = 0;
 #pragma unroll
-    for (int m = 1; m < 8; m++) c += (q >= g.coff[m]);
+    for (int m = 1; m < 8; m++) c += (q >= g.cs[m]);
     return c;
 }
 
-// colour-ordered index of node (k,i,j)
-__device__ __forceinline__ int colour_index(const Geo &g, int k, int i, int j) {
-    const int c = (k & 1) | ((i & 1) << 1) | ((j & 1) << 2);
-    return g.coff[c] + (k >> 1) + g.cz[c] * ((i >> 1) + g.cx[c] * (j >> 1));
+// colour of q, or -1 for a padding slot
+__device__ __forceinline__ int colour_valid(const Geo &g, int q) {
+    if (q >= g.pitch) return -1;
+    const int c = colour_of(g, q);
+    return (q - g.cs[c] < g.cn[c]) ? c : -1;
 }
 
 // Stiffness storage: 64-node tiles in colour order, each tile holding its
@@ -159,63 +160,95 @@ __device__ __forceinline__ void strip3(unsigned f, double v[3]) {
 }
 
 // ---------------------------------------------------------------------------
-// matvec: n_assemble_del2_u (Element_calculations.c:552-621) with strip
+// Large levels: one workgroup per 64-node tile of one colour, three
+// wavefronts per workgroup, wavefront r forming row r of the block-row
+// product of its 64 nodes.  Each wavefront streams the 81 coefficient runs of
+// its row (81 x 256 contiguous bytes); the three share the gathered vector
+// entries through L1.  The row sums follow row_product's neighbour order, so
+// results are bit-identical to a one-thread-per-node evaluation.
 
-__global__ void __launch_bounds__(TPB) k_matvec(Geo g, const float *__restrict__ K, const unsigned *__restrict__ NODE,
+__device__ __forceinline__ double row_sum(const Geo &g, const float *__restrict__ kp, int r, int k, int i, int j,
+                                          const double *__restrict__ u) {
+    const int self = nid(g, k, i, j);
+    double out = 0.0;
+#pragma unroll
+    for (int dy = -1; dy <= 1; dy++) {
+        const bool oy = (unsigned)(j + dy) < (unsigned)g.noy;
+        float kv[27];
+        double uv[27];
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) {
+            const bool ox = oy && (unsigned)(i + dx) < (unsigned)g.nox;
+#pragma unroll
+            for (int dz = -1; dz <= 1; dz++) {
+                const int b = (dz + 1) + 3 * (dx + 1);
+                const bool ok = ox && (unsigned)(k + dz) < (unsigned)g.noz;
+                const int m = ok ? self + dz + g.noz * (dx + g.nox * dy) : self;
+                uv[3 * b] = u[3 * m];
+                uv[3 * b + 1] = u[3 * m + 1];
+                uv[3 * b + 2] = u[3 * m + 2];
+                const int ci = 9 * (b + 9 * (dy + 1)) + 3 * r;
+                kv[3 * b] = kp[ci * 64];
+                kv[3 * b + 1] = kp[(ci + 1) * 64];
+                kv[3 * b + 2] = kp[(ci + 2) * 64];
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < 9; b++) out += kv[3 * b] * uv[3 * b] + kv[3 * b + 1] * uv[3 * b + 1] + kv[3 * b + 2] * uv[3 * b + 2];
+    }
+    return out;
+}
+
+// matvec: n_assemble_del2_u (Element_calculations.c:552-621) with strip;
+// optionally res = rhs - Au
+__global__ void __launch_bounds__(192) k_matvec(Geo g, const float *__restrict__ K, const unsigned *__restrict__ NODE,
                                                 const double *__restrict__ u, double *__restrict__ Au,
                                                 const double *__restrict__ rhs, double *__restrict__ res) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= g.nno) return;
+    const int q = blockIdx.x * 64 + (threadIdx.x & 63), r = threadIdx.x >> 6;
+    const int col = colour_valid(g, q);
+    if (col < 0) return;
     int k, i, j;
-    colour_kij(g, colour_of(g, q), q, k, i, j);
-    double out[3];
-    row_product(g, K, q, k, i, j, u, out);
+    colour_kij(g, col, q, k, i, j);
+    double out = row_sum(g, K + ktile(q), r, k, i, j, u);
     const int n = nid(g, k, i, j);
-    strip3(NODE[n], out);
-    Au[3 * n] = out[0];
-    Au[3 * n + 1] = out[1];
-    Au[3 * n + 2] = out[2];
-    if (res) {
-        res[3 * n] = rhs[3 * n] - out[0];
-        res[3 * n + 1] = rhs[3 * n + 1] - out[1];
-        res[3 * n + 2] = rhs[3 * n + 2] - out[2];
-    }
+    const unsigned f = NODE[n];
+    if ((r == 0 && (f & VBX)) || (r == 1 && (f & VBY)) || (r == 2 && (f & VBZ))) out = 0.0;
+    Au[3 * n + r] = out;
+    if (res) res[3 * n + r] = rhs[3 * n + r] - out;
 }
 
 void matvec(hipStream_t s, const Level &L, const double *u, double *Au, const double *rhs, double *res) {
-    hipLaunchKernelGGL(k_matvec, dim3(nblk(L.g.nno)), dim3(TPB), 0, s, L.g, L.K, L.NODE, u, Au, rhs, res);
+    hipLaunchKernelGGL(k_matvec, dim3(L.g.pitch / 64), dim3(192), 0, s, L.g, L.K, L.NODE, u, Au, rhs, res);
 }
 
-// ---------------------------------------------------------------------------
 // Gauss-Seidel: gauss_seidel (General_matrix_functions.c:1160-1361) as eight
 // colour passes; the node-block update t = (F - (K d)_node) * BI, rounded to
-// float as the reference's temp[] is, then d += t.
-
-__device__ __forceinline__ void node_update(const Geo &g, const float *__restrict__ K, const double *__restrict__ BI,
-                                            int q, int k, int i, int j, double *d, const double *__restrict__ F) {
-    double out[3];
-    row_product(g, K, q, k, i, j, d, out);
-    const int n = nid(g, k, i, j);
-#pragma unroll
-    for (int r = 0; r < 3; r++) {
-        const float t = (float)((F[3 * n + r] - out[r]) * BI[3 * n + r]);
+// float as the reference's temp[] is, then d += t.  The three rows of a node
+// are read before any is written (workgroup barrier), as the reference
+// updates all three unknowns of a node from the same Ad.
+__global__ void __launch_bounds__(192) k_gs_colour(Geo g, int c, const float *__restrict__ K,
+                                                   const double *__restrict__ BI, double *d,
+                                                   const double *__restrict__ F) {
+    const int q = g.cs[c] + blockIdx.x * 64 + (threadIdx.x & 63), r = threadIdx.x >> 6;
+    const bool act = q < g.cs[c] + g.cn[c];
+    double out = 0.0;
+    int n = 0;
+    if (act) {
+        int k, i, j;
+        colour_kij(g, c, q, k, i, j);
+        out = row_sum(g, K + ktile(q), r, k, i, j, d);
+        n = nid(g, k, i, j);
+    }
+    __syncthreads();
+    if (act) {
+        const float t = (float)((F[3 * n + r] - out) * BI[3 * n + r]);
         d[3 * n + r] += t;
     }
 }
 
-__global__ void __launch_bounds__(TPB) k_gs_colour(Geo g, int c, const float *__restrict__ K,
-                                                   const double *__restrict__ BI, double *d,
-                                                   const double *__restrict__ F) {
-    const int q = g.coff[c] + blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= g.coff[c + 1]) return;
-    int k, i, j;
-    colour_kij(g, c, q, k, i, j);
-    node_update(g, K, BI, q, k, i, j, d, F);
-}
-
 void gs_colour(hipStream_t s, const Level &L, int colour, double *d, const double *F) {
-    const int n = L.g.coff[colour + 1] - L.g.coff[colour];
-    hipLaunchKernelGGL(k_gs_colour, dim3(nblk(n)), dim3(TPB), 0, s, L.g, colour, L.K, L.BI, d, F);
+    const int tiles = (L.g.cn[colour] + 63) / 64;
+    hipLaunchKernelGGL(k_gs_colour, dim3(tiles), dim3(192), 0, s, L.g, colour, L.K, L.BI, d, F);
 }
 
 // levels up to this size run every sweep in one workgroup (gs_small)
@@ -224,40 +257,42 @@ constexpr int SMALL_NNO = 6144;
 bool gs_small_ok(const Level &L) { return L.g.nno <= SMALL_NNO; }
 
 // Small levels run all their sweeps in one workgroup.  A colour pass there
-// has only a few nodes, so each row product is spread over 27 lanes (one per
-// neighbour block); three lanes then add the 27 block terms in the
-// neighbour order of row_product, which keeps the sums bit-identical to the
-// one-thread-per-node kernels.  On the smallest levels the stiffness,
-// iterate, right-hand side and BI are staged in LDS first.
-constexpr int WIDE_TPB = 1024;
-constexpr int WIDE_GRP = 32;   // lanes per node (27 used)
+// has few nodes, so each node is given three lanes, one per row: lane r
+// forms row r of the block-row product alone, in the neighbour order of
+// row_product (bit-identical sums), and updates its own unknown.  The
+// smallest levels first stage the stiffness, iterate, right-hand side and
+// BI in LDS; the others stage the vectors when they fit.
+constexpr int ROWS_TPB = 1024;
 
-__host__ __device__ inline size_t wide_scratch_bytes() { return (size_t)(WIDE_TPB / WIDE_GRP) * 27 * 3 * 8; }
-__host__ __device__ inline size_t lds_stage_bytes(int nno) {
-    return (((size_t)KROW * nno * 4 + 15) / 16) * 16 + (size_t)9 * nno * 8;
-}
+__host__ __device__ inline size_t kstage_bytes(int pitch) { return (((size_t)KROW * pitch * 4 + 15) / 16) * 16; }
+__host__ __device__ inline size_t vstage_bytes(int nno) { return (size_t)9 * nno * 8; }
 
-template <bool STAGED>
-__global__ void __launch_bounds__(WIDE_TPB) k_gs_wide(Geo g, const float *__restrict__ K, const double *__restrict__ BI,
+template <bool SK, bool SV>
+__global__ void __launch_bounds__(ROWS_TPB) k_gs_rows(Geo g, const float *__restrict__ K, const double *__restrict__ BI,
                                                       const unsigned *__restrict__ NODE, double *d,
                                                       const double *__restrict__ F, double *__restrict__ Ad,
                                                       int sweeps, int guess) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int nno = g.nno, tid = threadIdx.x;
-    const int grp = tid / WIDE_GRP, lane = tid % WIDE_GRP, ngrp = blockDim.x / WIDE_GRP;
-    double *T = (double *)smem;  // [ngrp][27][3]
-    float *Ks = nullptr;
+    // the three lanes of a node sit in one wavefront (21 nodes per wave,
+    // lane 63 idle), so each lane reads its node's old values before any of
+    // the three stores its update
+    const int lane = tid & 63, r = lane % 3;
+    const int slot = (lane < 63) ? (tid >> 6) * 21 + lane / 3 : (1 << 30), nslot = (blockDim.x >> 6) * 21;
+    const float *Ks = (const float *)smem;
     double *dv = d;
     const double *Fv = F, *Bv = BI;
-    if (STAGED) {
-        unsigned char *base = smem + wide_scratch_bytes();
-        Ks = (float *)base;
-        double *ds = (double *)(base + (((size_t)KROW * nno * 4 + 15) / 16) * 16);
-        double *Fs = ds + 3 * nno, *Bs = Fs + 3 * nno;
-        for (int idx = tid; idx < KROW * nno; idx += blockDim.x) {
-            const int ci = idx / nno, q = idx - ci * nno;
-            Ks[idx] = K[ktile(q) + ci * 64];
+    const int pitch = g.pitch;
+    if (SK) {
+        float *kw = (float *)smem;
+        for (int idx = tid; idx < KROW * pitch; idx += blockDim.x) {
+            const int ci = idx / pitch, q = idx - ci * pitch;
+            kw[idx] = K[ktile(q) + ci * 64];
         }
+    }
+    if (SV) {
+        double *ds = (double *)(smem + (SK ? kstage_bytes(pitch) : 0));
+        double *Fs = ds + 3 * nno, *Bs = Fs + 3 * nno;
         for (int idx = tid; idx < 3 * nno; idx += blockDim.x) {
             ds[idx] = guess ? d[idx] : 0.0;
             Fs[idx] = F[idx];
@@ -270,78 +305,51 @@ __global__ void __launch_bounds__(WIDE_TPB) k_gs_wide(Geo g, const float *__rest
         for (int idx = tid; idx < 3 * nno; idx += blockDim.x) d[idx] = 0.0;
     }
     __syncthreads();
-    const int s = lane, dz = s % 3 - 1, dx = (s / 3) % 3 - 1, dy = s / 9 - 1;
-    // one round: groups take consecutive colour-ordered nodes [q0, q1)
-    auto block_terms = [&](int q, int k, int i, int j) {
-        if (lane < 27) {
+    // row r of the block-row product of the node at colour index q
+    auto row = [&](int q, int k, int i, int j) {
+        const int self = nid(g, k, i, j);
+        const float *kp = SK ? Ks + q : K + ktile(q);
+        const int kst = SK ? pitch : 64;
+        double out = 0.0;
+#pragma unroll 3
+        for (int s = 0; s < 27; s++) {
+            const int dz = s % 3 - 1, dx = (s / 3) % 3 - 1, dy = s / 9 - 1;
             const bool ok = (unsigned)(k + dz) < (unsigned)g.noz && (unsigned)(i + dx) < (unsigned)g.nox &&
                             (unsigned)(j + dy) < (unsigned)g.noy;
-            const int self = nid(g, k, i, j);
             const int m = ok ? self + dz + g.noz * (dx + g.nox * dy) : self;
-            const double u0 = dv[3 * m], u1 = dv[3 * m + 1], u2 = dv[3 * m + 2];
-#pragma unroll
-            for (int r = 0; r < 3; r++) {
-                float a0, a1, a2;
-                if (STAGED) {
-                    a0 = Ks[(s * 9 + 3 * r) * nno + q];
-                    a1 = Ks[(s * 9 + 3 * r + 1) * nno + q];
-                    a2 = Ks[(s * 9 + 3 * r + 2) * nno + q];
-                } else {
-                    const float *kp = K + ktile(q);
-                    a0 = kp[(s * 9 + 3 * r) * 64];
-                    a1 = kp[(s * 9 + 3 * r + 1) * 64];
-                    a2 = kp[(s * 9 + 3 * r + 2) * 64];
-                }
-                T[(grp * 27 + s) * 3 + r] = a0 * u0 + a1 * u1 + a2 * u2;
-            }
+            const int ci = s * 9 + 3 * r;
+            out += kp[ci * kst] * dv[3 * m] + kp[(ci + 1) * kst] * dv[3 * m + 1] + kp[(ci + 2) * kst] * dv[3 * m + 2];
         }
-    };
-    auto row_sum = [&](int r) {
-        double out = 0.0;
-        for (int t = 0; t < 27; t++) out += T[(grp * 27 + t) * 3 + r];
         return out;
     };
     for (int cyc = 0; cyc < sweeps; cyc++)
         for (int c = 0; c < 8; c++) {
-            for (int q0 = g.coff[c]; q0 < g.coff[c + 1]; q0 += ngrp) {
-                const int q = q0 + grp;
-                const bool act = q < g.coff[c + 1];
-                int k = 0, i = 0, j = 0;
-                if (act) {
-                    colour_kij(g, c, q, k, i, j);
-                    block_terms(q, k, i, j);
-                }
-                __syncthreads();
-                if (act && lane < 3) {
-                    const double out = row_sum(lane);
-                    const int n = nid(g, k, i, j);
-                    const float t = (float)((Fv[3 * n + lane] - out) * Bv[3 * n + lane]);
-                    dv[3 * n + lane] += t;
-                }
-                __syncthreads();
+            for (int q = g.cs[c] + slot; q < g.cs[c] + g.cn[c]; q += nslot) {
+                int k, i, j;
+                colour_kij(g, c, q, k, i, j);
+                const double out = row(q, k, i, j);
+                const int n = nid(g, k, i, j);
+                const float t = (float)((Fv[3 * n + r] - out) * Bv[3 * n + r]);
+                dv[3 * n + r] += t;
             }
+            __syncthreads();
         }
     // Ad = K d, BC rows stripped
-    for (int q0 = 0; q0 < nno; q0 += ngrp) {
-        const int q = q0 + grp;
-        const bool act = q < nno;
-        int k = 0, i = 0, j = 0;
-        if (act) {
-            colour_kij(g, colour_of(g, q), q, k, i, j);
-            block_terms(q, k, i, j);
-        }
-        __syncthreads();
-        if (act && lane < 3) {
-            double out = row_sum(lane);
-            const int n = nid(g, k, i, j);
-            const unsigned f = NODE[n];
-            if ((lane == 0 && (f & VBX)) || (lane == 1 && (f & VBY)) || (lane == 2 && (f & VBZ))) out = 0.0;
-            Ad[3 * n + lane] = out;
-        }
-        __syncthreads();
+    for (int q = slot; q < pitch; q += nslot) {
+        const int col = colour_valid(g, q);
+        if (col < 0) continue;
+        int k, i, j;
+        colour_kij(g, col, q, k, i, j);
+        double out = row(q, k, i, j);
+        const int n = nid(g, k, i, j);
+        const unsigned f = NODE[n];
+        if ((r == 0 && (f & VBX)) || (r == 1 && (f & VBY)) || (r == 2 && (f & VBZ))) out = 0.0;
+        Ad[3 * n + r] = out;
     }
-    if (STAGED)
+    if (SV) {
+        __syncthreads();
         for (int idx = tid; idx < 3 * nno; idx += blockDim.x) d[idx] = dv[idx];
+    }
 }
 
 size_t lds_limit() {
@@ -354,22 +362,26 @@ size_t lds_limit() {
         else
             lim = 65536;
         if (lim > 65536) {
-            (void)hipFuncSetAttribute((const void *)k_gs_wide<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim);
-            (void)hipFuncSetAttribute((const void *)k_gs_wide<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim);
+            (void)hipFuncSetAttribute((const void *)k_gs_rows<true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim);
+            (void)hipFuncSetAttribute((const void *)k_gs_rows<false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim);
         }
     }
     return lim;
 }
 
 void gs_small(hipStream_t s, const Level &L, double *d, const double *F, double *Ad, int sweeps, int guess) {
-    const size_t staged = wide_scratch_bytes() + lds_stage_bytes(L.g.nno);
-    if (staged <= lds_limit()) {
-        hipLaunchKernelGGL(k_gs_wide<true>, dim3(1), dim3(WIDE_TPB), staged, s, L.g, L.K, L.BI, L.NODE, d, F, Ad,
+    const int nno = L.g.nno;
+    const size_t lim = lds_limit();
+    if (kstage_bytes(L.g.pitch) + vstage_bytes(nno) <= lim) {
+        hipLaunchKernelGGL((k_gs_rows<true, true>), dim3(1), dim3(ROWS_TPB), kstage_bytes(L.g.pitch) + vstage_bytes(nno), s,
+                           L.g, L.K, L.BI, L.NODE, d, F, Ad, sweeps, guess);
+    } else if (vstage_bytes(nno) <= lim) {
+        hipLaunchKernelGGL((k_gs_rows<false, true>), dim3(1), dim3(ROWS_TPB), vstage_bytes(nno), s, L.g, L.K, L.BI,
+                           L.NODE, d, F, Ad, sweeps, guess);
+    } else {
+        hipLaunchKernelGGL((k_gs_rows<false, false>), dim3(1), dim3(ROWS_TPB), 0, s, L.g, L.K, L.BI, L.NODE, d, F, Ad,
                            sweeps, guess);
-        return;
     }
-    hipLaunchKernelGGL(k_gs_wide<false>, dim3(1), dim3(WIDE_TPB), wide_scratch_bytes(), s, L.g, L.K, L.BI, L.NODE, d,
-                       F, Ad, sweeps, guess);
 }
 
 // ---------------------------------------------------------------------------
@@ -985,9 +997,10 @@ __global__ void __launch_bounds__(128) k_node_ks(Geo g, const float *__restrict_
                                                  const unsigned *__restrict__ NODE, int aug_on, double aug,
                                                  float *__restrict__ K, double *__restrict__ BI) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= g.nno) return;
+    const int col = colour_valid(g, q);
+    if (col < 0) return;
     int k, i, j;
-    colour_kij(g, colour_of(g, q), q, k, i, j);
+    colour_kij(g, col, q, k, i, j);
     const int n = nid(g, k, i, j);
     float *kq = K + ktile(q);
     double diag[3] = {0.0, 0.0, 0.0};
@@ -1058,7 +1071,7 @@ __global__ void __launch_bounds__(128) k_node_ks(Geo g, const float *__restrict_
 }
 
 void node_ks(hipStream_t s, const Level &L, const chip_problem &p) {
-    hipLaunchKernelGGL(k_node_ks, dim3(nblk(L.g.nno, 128)), dim3(128), 0, s, L.g, L.GNX, L.GDA, L.EVI, L.EG, L.NODE,
+    hipLaunchKernelGGL(k_node_ks, dim3(nblk(L.g.pitch, 128)), dim3(128), 0, s, L.g, L.GNX, L.GDA, L.EVI, L.EG, L.NODE,
                        p.aug_lagr, p.aug_number, L.K, L.BI);
 }
 

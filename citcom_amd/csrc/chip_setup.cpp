@@ -269,18 +269,19 @@ int build_mesh(chip_ctx *c) {
         g.elz = g.noz - 1;
         g.nno = g.nox * g.noy * g.noz;
         g.nel = g.elx * g.ely * g.elz;
-        g.pitch = (g.nno + 63) / 64 * 64;
         // colour classes by index parities (k&1) | (i&1)<<1 | (j&1)<<2
         int off = 0;
         for (int cc = 0; cc < 8; cc++) {
             const int pk = cc & 1, pi = (cc >> 1) & 1, pj = (cc >> 2) & 1;
             const int nz = (g.noz - pk + 1) / 2, nx = (g.nox - pi + 1) / 2, ny = (g.noy - pj + 1) / 2;
-            g.coff[cc] = off;
+            g.cs[cc] = off;
+            g.cn[cc] = nz * nx * ny;
             g.cz[cc] = nz;
             g.cx[cc] = nx;
-            off += nz * nx * ny;
+            off += (g.cn[cc] + 63) / 64 * 64;
         }
-        g.coff[8] = off;
+        g.cs[8] = off;
+        g.pitch = off;
         // coordinates: per-axis arrays of the finest level, injected downwards
         // (Nodal_mesh.c:170-180, 244-247)
         for (int d = 0; d < 3; d++) h.X[d].resize(g.nno);
