@@ -200,25 +200,32 @@ __device__ __forceinline__ double row_sum(const Geo &g, const float *__restrict_
 }
 
 // matvec: n_assemble_del2_u (Element_calculations.c:552-621) with strip;
-// optionally res = rhs - Au
-__global__ void __launch_bounds__(192) k_matvec(Geo g, const float *__restrict__ K, const unsigned *__restrict__ NODE,
+// optionally res = rhs - Au.  One thread per node: the three rows share the
+// 27 vector gathers.
+__global__ void __launch_bounds__(256) k_matvec(Geo g, const float *__restrict__ K, const unsigned *__restrict__ NODE,
                                                 const double *__restrict__ u, double *__restrict__ Au,
                                                 const double *__restrict__ rhs, double *__restrict__ res) {
-    const int q = blockIdx.x * 64 + (threadIdx.x & 63), r = threadIdx.x >> 6;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
     const int col = colour_valid(g, q);
     if (col < 0) return;
     int k, i, j;
     colour_kij(g, col, q, k, i, j);
-    double out = row_sum(g, K + ktile(q), r, k, i, j, u);
+    double out[3];
+    row_product(g, K, q, k, i, j, u, out);
     const int n = nid(g, k, i, j);
-    const unsigned f = NODE[n];
-    if ((r == 0 && (f & VBX)) || (r == 1 && (f & VBY)) || (r == 2 && (f & VBZ))) out = 0.0;
-    Au[3 * n + r] = out;
-    if (res) res[3 * n + r] = rhs[3 * n + r] - out;
+    strip3(NODE[n], out);
+    Au[3 * n] = out[0];
+    Au[3 * n + 1] = out[1];
+    Au[3 * n + 2] = out[2];
+    if (res) {
+        res[3 * n] = rhs[3 * n] - out[0];
+        res[3 * n + 1] = rhs[3 * n + 1] - out[1];
+        res[3 * n + 2] = rhs[3 * n + 2] - out[2];
+    }
 }
 
 void matvec(hipStream_t s, const Level &L, const double *u, double *Au, const double *rhs, double *res) {
-    hipLaunchKernelGGL(k_matvec, dim3(L.g.pitch / 64), dim3(192), 0, s, L.g, L.K, L.NODE, u, Au, rhs, res);
+    hipLaunchKernelGGL(k_matvec, dim3(nblk(L.g.pitch)), dim3(256), 0, s, L.g, L.K, L.NODE, u, Au, rhs, res);
 }
 
 // Gauss-Seidel: gauss_seidel (General_matrix_functions.c:1160-1361) as eight
@@ -251,10 +258,13 @@ void gs_colour(hipStream_t s, const Level &L, int colour, double *d, const doubl
     hipLaunchKernelGGL(k_gs_colour, dim3(tiles), dim3(192), 0, s, L.g, colour, L.K, L.BI, d, F);
 }
 
-// levels up to this size run every sweep in one workgroup (gs_small)
-constexpr int SMALL_NNO = 6144;
+size_t lds_limit();
+__host__ __device__ inline size_t kstage_bytes(int nno);
+__host__ __device__ inline size_t vstage_bytes(int nno);
 
-bool gs_small_ok(const Level &L) { return L.g.nno <= SMALL_NNO; }
+// levels whose stiffness fits in LDS run every sweep in one workgroup
+// (gs_small); larger ones launch one kernel per colour pass
+bool gs_small_ok(const Level &L) { return kstage_bytes(L.g.nno) + vstage_bytes(L.g.nno) <= lds_limit(); }
 
 // Small levels run all their sweeps in one workgroup.  A colour pass there
 // has few nodes, so each node is given three lanes, one per row: lane r
@@ -264,7 +274,14 @@ bool gs_small_ok(const Level &L) { return L.g.nno <= SMALL_NNO; }
 // BI in LDS; the others stage the vectors when they fit.
 constexpr int ROWS_TPB = 1024;
 
-__host__ __device__ inline size_t kstage_bytes(int pitch) { return (((size_t)KROW * pitch * 4 + 15) / 16) * 16; }
+__host__ __device__ inline size_t kstage_bytes(int nno) { return (((size_t)KROW * nno * 4 + 15) / 16) * 16; }
+
+// compact (padding-free) index of colour-ordered index q of colour c
+__device__ __forceinline__ int compact_index(const Geo &g, int c, int q) {
+    int base = 0;
+    for (int m = 0; m < c; m++) base += g.cn[m];
+    return base + (q - g.cs[c]);
+}
 __host__ __device__ inline size_t vstage_bytes(int nno) { return (size_t)9 * nno * 8; }
 
 template <bool SK, bool SV>
@@ -284,14 +301,19 @@ __global__ void __launch_bounds__(ROWS_TPB) k_gs_rows(Geo g, const float *__rest
     const double *Fv = F, *Bv = BI;
     const int pitch = g.pitch;
     if (SK) {
+        // compact copy [ci][node], nodes in colour order without padding
         float *kw = (float *)smem;
-        for (int idx = tid; idx < KROW * pitch; idx += blockDim.x) {
-            const int ci = idx / pitch, q = idx - ci * pitch;
-            kw[idx] = K[ktile(q) + ci * 64];
+        for (int q = tid; q < pitch; q += blockDim.x) {
+            const int col = colour_valid(g, q);
+            if (col < 0) continue;
+            const int cq = compact_index(g, col, q);
+            const float *kp = K + ktile(q);
+#pragma unroll 27
+            for (int ci = 0; ci < KROW; ci++) kw[ci * nno + cq] = kp[ci * 64];
         }
     }
     if (SV) {
-        double *ds = (double *)(smem + (SK ? kstage_bytes(pitch) : 0));
+        double *ds = (double *)(smem + (SK ? kstage_bytes(nno) : 0));
         double *Fs = ds + 3 * nno, *Bs = Fs + 3 * nno;
         for (int idx = tid; idx < 3 * nno; idx += blockDim.x) {
             ds[idx] = guess ? d[idx] : 0.0;
@@ -306,10 +328,10 @@ __global__ void __launch_bounds__(ROWS_TPB) k_gs_rows(Geo g, const float *__rest
     }
     __syncthreads();
     // row r of the block-row product of the node at colour index q
-    auto row = [&](int q, int k, int i, int j) {
+    auto row = [&](int c, int q, int k, int i, int j) {
         const int self = nid(g, k, i, j);
-        const float *kp = SK ? Ks + q : K + ktile(q);
-        const int kst = SK ? pitch : 64;
+        const float *kp = SK ? Ks + compact_index(g, c, q) : K + ktile(q);
+        const int kst = SK ? nno : 64;
         double out = 0.0;
 #pragma unroll 3
         for (int s = 0; s < 27; s++) {
@@ -327,7 +349,7 @@ __global__ void __launch_bounds__(ROWS_TPB) k_gs_rows(Geo g, const float *__rest
             for (int q = g.cs[c] + slot; q < g.cs[c] + g.cn[c]; q += nslot) {
                 int k, i, j;
                 colour_kij(g, c, q, k, i, j);
-                const double out = row(q, k, i, j);
+                const double out = row(c, q, k, i, j);
                 const int n = nid(g, k, i, j);
                 const float t = (float)((Fv[3 * n + r] - out) * Bv[3 * n + r]);
                 dv[3 * n + r] += t;
@@ -340,7 +362,7 @@ __global__ void __launch_bounds__(ROWS_TPB) k_gs_rows(Geo g, const float *__rest
         if (col < 0) continue;
         int k, i, j;
         colour_kij(g, col, q, k, i, j);
-        double out = row(q, k, i, j);
+        double out = row(col, q, k, i, j);
         const int n = nid(g, k, i, j);
         const unsigned f = NODE[n];
         if ((r == 0 && (f & VBX)) || (r == 1 && (f & VBY)) || (r == 2 && (f & VBZ))) out = 0.0;
@@ -372,8 +394,8 @@ size_t lds_limit() {
 void gs_small(hipStream_t s, const Level &L, double *d, const double *F, double *Ad, int sweeps, int guess) {
     const int nno = L.g.nno;
     const size_t lim = lds_limit();
-    if (kstage_bytes(L.g.pitch) + vstage_bytes(nno) <= lim) {
-        hipLaunchKernelGGL((k_gs_rows<true, true>), dim3(1), dim3(ROWS_TPB), kstage_bytes(L.g.pitch) + vstage_bytes(nno), s,
+    if (kstage_bytes(nno) + vstage_bytes(nno) <= lim) {
+        hipLaunchKernelGGL((k_gs_rows<true, true>), dim3(1), dim3(ROWS_TPB), kstage_bytes(nno) + vstage_bytes(nno), s,
                            L.g, L.K, L.BI, L.NODE, d, F, Ad, sweeps, guess);
     } else if (vstage_bytes(nno) <= lim) {
         hipLaunchKernelGGL((k_gs_rows<false, true>), dim3(1), dim3(ROWS_TPB), vstage_bytes(nno), s, L.g, L.K, L.BI,
