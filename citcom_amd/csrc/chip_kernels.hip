@@ -266,112 +266,110 @@ __host__ __device__ inline size_t vstage_bytes(int nno);
 // (gs_small); larger ones launch one kernel per colour pass
 bool gs_small_ok(const Level &L) { return kstage_bytes(L.g.nno) + vstage_bytes(L.g.nno) <= lds_limit(); }
 
-// Small levels run all their sweeps in one workgroup.  A colour pass there
-// has few nodes, so each node is given three lanes, one per row: lane r
-// forms row r of the block-row product alone, in the neighbour order of
-// row_product (bit-identical sums), and updates its own unknown.  The
-// smallest levels first stage the stiffness, iterate, right-hand side and
-// BI in LDS; the others stage the vectors when they fit.
+// Small levels (stiffness resident in LDS) run all their sweeps in one
+// workgroup.  A colour pass there has only a few nodes, so each node gets a
+// wavefront: lane s < 27 forms the three row terms of neighbour block s, and
+// lanes r < 3 add the 27 terms of row r in the neighbour order of
+// row_product (bit-identical sums) and update unknown r.  Node coordinates
+// and neighbour indices are tabulated in LDS once per call.
 constexpr int ROWS_TPB = 1024;
 
 __host__ __device__ inline size_t kstage_bytes(int nno) { return (((size_t)KROW * nno * 4 + 15) / 16) * 16; }
-
-// compact (padding-free) index of colour-ordered index q of colour c
-__device__ __forceinline__ int compact_index(const Geo &g, int c, int q) {
-    int base = 0;
-    for (int m = 0; m < c; m++) base += g.cn[m];
-    return base + (q - g.cs[c]);
+__host__ __device__ inline size_t vstage_bytes(int nno) {
+    return (size_t)9 * nno * 8 + (size_t)27 * nno * 4 + (size_t)(ROWS_TPB / 64) * 81 * 8;
 }
-__host__ __device__ inline size_t vstage_bytes(int nno) { return (size_t)9 * nno * 8; }
 
-template <bool SK, bool SV>
-__global__ void __launch_bounds__(ROWS_TPB) k_gs_rows(Geo g, const float *__restrict__ K, const double *__restrict__ BI,
-                                                      const unsigned *__restrict__ NODE, double *d,
-                                                      const double *__restrict__ F, double *__restrict__ Ad,
-                                                      int sweeps, int guess) {
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ void __launch_bounds__(ROWS_TPB) k_gs_lds(Geo g, const float *__restrict__ K, const double *__restrict__ BI,
+                                                     const unsigned *__restrict__ NODE, double *d,
+                                                     const double *__restrict__ F, double *__restrict__ Ad, int sweeps,
+                                                     int guess) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int nno = g.nno, tid = threadIdx.x;
-    // the three lanes of a node sit in one wavefront (21 nodes per wave,
-    // lane 63 idle), so each lane reads its node's old values before any of
-    // the three stores its update
-    const int lane = tid & 63, r = lane % 3;
-    const int slot = (lane < 63) ? (tid >> 6) * 21 + lane / 3 : (1 << 30), nslot = (blockDim.x >> 6) * 21;
-    const float *Ks = (const float *)smem;
-    double *dv = d;
-    const double *Fv = F, *Bv = BI;
-    const int pitch = g.pitch;
-    if (SK) {
-        // compact copy [ci][node], nodes in colour order without padding
-        float *kw = (float *)smem;
-        for (int q = tid; q < pitch; q += blockDim.x) {
-            const int col = colour_valid(g, q);
-            if (col < 0) continue;
-            const int cq = compact_index(g, col, q);
-            const float *kp = K + ktile(q);
-#pragma unroll 27
-            for (int ci = 0; ci < KROW; ci++) kw[ci * nno + cq] = kp[ci * 64];
-        }
-    }
-    if (SV) {
-        double *ds = (double *)(smem + (SK ? kstage_bytes(nno) : 0));
-        double *Fs = ds + 3 * nno, *Bs = Fs + 3 * nno;
-        for (int idx = tid; idx < 3 * nno; idx += blockDim.x) {
-            ds[idx] = guess ? d[idx] : 0.0;
-            Fs[idx] = F[idx];
-            Bs[idx] = BI[idx];
-        }
-        dv = ds;
-        Fv = Fs;
-        Bv = Bs;
-    } else if (!guess) {
-        for (int idx = tid; idx < 3 * nno; idx += blockDim.x) d[idx] = 0.0;
-    }
-    __syncthreads();
-    // row r of the block-row product of the node at colour index q
-    auto row = [&](int c, int q, int k, int i, int j) {
+    const int nno = g.nno, tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = blockDim.x >> 6;
+    float *Ks = (float *)smem;                                   // [243][nno], compact colour order
+    double *ds = (double *)(smem + kstage_bytes(nno));           // [3 nno] natural order
+    double *Fs = ds + 3 * nno, *Bs = Fs + 3 * nno;
+    int *nb = (int *)(Bs + 3 * nno);                             // [nno][27] neighbour (or self) node
+    double *T = (double *)(nb + 27 * nno);                       // [nw][27][3]
+    // stage: compact index cq of colour-ordered q
+    for (int q = tid; q < g.pitch; q += blockDim.x) {
+        const int col = colour_valid(g, q);
+        if (col < 0) continue;
+        int base = 0;
+        for (int m = 0; m < col; m++) base += g.cn[m];
+        const int cq = base + (q - g.cs[col]);
+        int k, i, j;
+        colour_kij(g, col, q, k, i, j);
         const int self = nid(g, k, i, j);
-        const float *kp = SK ? Ks + compact_index(g, c, q) : K + ktile(q);
-        const int kst = SK ? nno : 64;
-        double out = 0.0;
-#pragma unroll 3
+        const float *kp = K + ktile(q);
+#pragma unroll 27
+        for (int ci = 0; ci < KROW; ci++) Ks[ci * nno + cq] = kp[ci * 64];
         for (int s = 0; s < 27; s++) {
             const int dz = s % 3 - 1, dx = (s / 3) % 3 - 1, dy = s / 9 - 1;
             const bool ok = (unsigned)(k + dz) < (unsigned)g.noz && (unsigned)(i + dx) < (unsigned)g.nox &&
                             (unsigned)(j + dy) < (unsigned)g.noy;
-            const int m = ok ? self + dz + g.noz * (dx + g.nox * dy) : self;
-            const int ci = s * 9 + 3 * r;
-            out += kp[ci * kst] * dv[3 * m] + kp[(ci + 1) * kst] * dv[3 * m + 1] + kp[(ci + 2) * kst] * dv[3 * m + 2];
+            nb[cq * 27 + s] = ok ? self + dz + g.noz * (dx + g.nox * dy) : self;
         }
+    }
+    for (int idx = tid; idx < 3 * nno; idx += blockDim.x) {
+        ds[idx] = guess ? d[idx] : 0.0;
+        Fs[idx] = F[idx];
+        Bs[idx] = BI[idx];
+    }
+    __syncthreads();
+    double *Tw = T + w * 81;
+    // terms of node cq into Tw (lanes < 27), then row r's sum (lanes < 3)
+    auto node_rows = [&](int cq) {
+        if (lane < 27) {
+            const int m = nb[cq * 27 + lane];
+            const double u0 = ds[3 * m], u1 = ds[3 * m + 1], u2 = ds[3 * m + 2];
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                const int ci = lane * 9 + 3 * r;
+                Tw[lane * 3 + r] = Ks[ci * nno + cq] * u0 + Ks[(ci + 1) * nno + cq] * u1 + Ks[(ci + 2) * nno + cq] * u2;
+            }
+        }
+        wave_sync_lds();
+        double out = 0.0;
+        if (lane < 3)
+            for (int t = 0; t < 27; t++) out += Tw[t * 3 + lane];
         return out;
     };
-    for (int cyc = 0; cyc < sweeps; cyc++)
+    for (int cyc = 0; cyc < sweeps; cyc++) {
+        int cb = 0;
         for (int c = 0; c < 8; c++) {
-            for (int q = g.cs[c] + slot; q < g.cs[c] + g.cn[c]; q += nslot) {
-                int k, i, j;
-                colour_kij(g, c, q, k, i, j);
-                const double out = row(c, q, k, i, j);
-                const int n = nid(g, k, i, j);
-                const float t = (float)((Fv[3 * n + r] - out) * Bv[3 * n + r]);
-                dv[3 * n + r] += t;
+            for (int t = w; t < g.cn[c]; t += nw) {
+                const int cq = cb + t;
+                const double out = node_rows(cq);
+                if (lane < 3) {
+                    const int n = nb[cq * 27 + 13];
+                    const float tt = (float)((Fs[3 * n + lane] - out) * Bs[3 * n + lane]);
+                    ds[3 * n + lane] += tt;
+                }
+                wave_sync_lds();
             }
+            cb += g.cn[c];
             __syncthreads();
         }
+    }
     // Ad = K d, BC rows stripped
-    for (int q = slot; q < pitch; q += nslot) {
-        const int col = colour_valid(g, q);
-        if (col < 0) continue;
-        int k, i, j;
-        colour_kij(g, col, q, k, i, j);
-        double out = row(col, q, k, i, j);
-        const int n = nid(g, k, i, j);
-        const unsigned f = NODE[n];
-        if ((r == 0 && (f & VBX)) || (r == 1 && (f & VBY)) || (r == 2 && (f & VBZ))) out = 0.0;
-        Ad[3 * n + r] = out;
+    for (int cq = w; cq < nno; cq += nw) {
+        double out = node_rows(cq);
+        if (lane < 3) {
+            const int n = nb[cq * 27 + 13];
+            const unsigned f = NODE[n];
+            if ((lane == 0 && (f & VBX)) || (lane == 1 && (f & VBY)) || (lane == 2 && (f & VBZ))) out = 0.0;
+            Ad[3 * n + lane] = out;
+        }
+        wave_sync_lds();
     }
-    if (SV) {
-        __syncthreads();
-        for (int idx = tid; idx < 3 * nno; idx += blockDim.x) d[idx] = dv[idx];
-    }
+    __syncthreads();
+    for (int idx = tid; idx < 3 * nno; idx += blockDim.x) d[idx] = ds[idx];
 }
 
 size_t lds_limit() {
@@ -383,27 +381,16 @@ size_t lds_limit() {
             lim = (size_t)v;
         else
             lim = 65536;
-        if (lim > 65536) {
-            (void)hipFuncSetAttribute((const void *)k_gs_rows<true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim);
-            (void)hipFuncSetAttribute((const void *)k_gs_rows<false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim);
-        }
+        if (lim > 65536)
+            (void)hipFuncSetAttribute((const void *)k_gs_lds, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim);
     }
     return lim;
 }
 
 void gs_small(hipStream_t s, const Level &L, double *d, const double *F, double *Ad, int sweeps, int guess) {
     const int nno = L.g.nno;
-    const size_t lim = lds_limit();
-    if (kstage_bytes(nno) + vstage_bytes(nno) <= lim) {
-        hipLaunchKernelGGL((k_gs_rows<true, true>), dim3(1), dim3(ROWS_TPB), kstage_bytes(nno) + vstage_bytes(nno), s,
-                           L.g, L.K, L.BI, L.NODE, d, F, Ad, sweeps, guess);
-    } else if (vstage_bytes(nno) <= lim) {
-        hipLaunchKernelGGL((k_gs_rows<false, true>), dim3(1), dim3(ROWS_TPB), vstage_bytes(nno), s, L.g, L.K, L.BI,
-                           L.NODE, d, F, Ad, sweeps, guess);
-    } else {
-        hipLaunchKernelGGL((k_gs_rows<false, false>), dim3(1), dim3(ROWS_TPB), 0, s, L.g, L.K, L.BI, L.NODE, d, F, Ad,
-                           sweeps, guess);
-    }
+    hipLaunchKernelGGL(k_gs_lds, dim3(1), dim3(ROWS_TPB), kstage_bytes(nno) + vstage_bytes(nno), s, L.g, L.K, L.BI,
+                       L.NODE, d, F, Ad, sweeps, guess);
 }
 
 // ---------------------------------------------------------------------------
