@@ -31,6 +31,7 @@ sys.path.insert(0, ROOT)
 from citcom_amd import deck  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
+COPY_PROBE_BYTES = 256 << 20
 METRIC = "s/timestep at 128x128x64 elems: FP32+learned vs FP64 GPU, same 1e-6 residual"
 
 
@@ -139,6 +140,14 @@ def main():
     c.timestep()
     n_mv, ms_mv, bytes_mv = c.profile_read()
     c.profile(None)
+    # known-size copy (256 MiB read + 256 MiB written): the reference point that
+    # scales hardware byte counters (FETCH_SIZE / WRITE_SIZE) into HBM bytes
+    src = torch.ones(COPY_PROBE_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
     avg_ms = ms_gs / max(n_gs, 1)
     achieved = (bytes_gs / max(n_gs, 1)) / (avg_ms * 1e-3) / 1e9
     log(f"gs_colour: {n_gs} launches, {ms_gs:.1f} ms total, {achieved:.0f} GB/s; "

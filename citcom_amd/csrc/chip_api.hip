@@ -56,20 +56,18 @@ int sync_minmax(chip_ctx *c, float *mn, float *mx) {
 }
 
 // ---- profiling of one kernel family -------------------------------------
+// When the profiled family matches, a and b are handed to the launch, which
+// stamps them with the kernel's own start and end (hipExtLaunchKernelGGL).
 struct ProfGuard {
-    chip_ctx *c;
-    bool on;
-    ProfGuard(chip_ctx *cc, int kernel, double bytes) : c(cc), on(cc->prof.kernel == kernel) {
-        if (!on) return;
-        hipEvent_t a, b;
-        (void)hipEventCreate(&a);
-        (void)hipEventCreate(&b);
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfGuard(chip_ctx *c, int kernel, double bytes) {
+        if (c->prof.kernel != kernel) return;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+            a = b = nullptr;
+            return;
+        }
         c->prof.ev.emplace_back(a, b);
         c->prof.bytes.push_back(bytes);
-        (void)hipEventRecord(a, c->stream);
-    }
-    ~ProfGuard() {
-        if (on) (void)hipEventRecord(c->prof.ev.back().second, c->stream);
     }
 };
 
@@ -99,7 +97,7 @@ constexpr double GS_BYTES = 243.0 * 4 + 9 * 8 + 3 * 8;
 void do_matvec(chip_ctx *c, int lev, const double *u, double *Au, const double *rhs = nullptr, double *res = nullptr) {
     const Level &L = c->L[lev];
     ProfGuard pg(c, 2, MV_BYTES * L.g.nno);
-    matvec(S(c), L, u, Au, rhs, res);
+    matvec(S(c), L, u, Au, rhs, res, pg.a, pg.b);
 }
 
 // gauss_seidel (General_matrix_functions.c:1160-1361), 8-colour; returns Ad = K d
@@ -113,7 +111,7 @@ void do_gs(chip_ctx *c, int lev, double *d, const double *F, double *Ad, int swe
     for (int cyc = 0; cyc < sweeps; cyc++)
         for (int col = 0; col < 8; col++) {
             ProfGuard pg(c, 1, GS_BYTES * L.g.cn[col]);
-            gs_colour(S(c), L, col, d, F);
+            gs_colour(S(c), L, col, d, F, pg.a, pg.b);
         }
     do_matvec(c, lev, d, Ad);
 }

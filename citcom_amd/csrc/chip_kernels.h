@@ -20,10 +20,13 @@ void node_ks(hipStream_t s, const Level &L, const chip_problem &p);
 void build_bpi(hipStream_t s, const Level &L, int precond);
 
 // ---- vector operators ---------------------------------------------------
-// Au = K u (strip BC rows); optionally res = rhs - Au as well
-void matvec(hipStream_t s, const Level &L, const double *u, double *Au, const double *rhs, double *res);
+// Au = K u (strip BC rows); optionally res = rhs - Au as well.  With events
+// e0/e1 the launch records the kernel's own start and end time in them.
+void matvec(hipStream_t s, const Level &L, const double *u, double *Au, const double *rhs, double *res,
+            hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // one colour pass of the node-block Gauss-Seidel
-void gs_colour(hipStream_t s, const Level &L, int colour, double *d, const double *F);
+void gs_colour(hipStream_t s, const Level &L, int colour, double *d, const double *F, hipEvent_t e0 = nullptr,
+               hipEvent_t e1 = nullptr);
 // every sweep of a small level in one workgroup, then Ad = K d
 void gs_small(hipStream_t s, const Level &L, double *d, const double *F, double *Ad, int sweeps, int guess);
 bool gs_small_ok(const Level &L);

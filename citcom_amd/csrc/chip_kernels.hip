@@ -8,6 +8,7 @@
 // -ffp-contract=off for that reason.  The stiffness operator is bandwidth
 // bound (2 flops per 4-byte coefficient): coefficient streams are read once,
 // coalesced, 256-byte aligned; vectors are gathered through L2.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -224,8 +225,13 @@ __global__ void __launch_bounds__(256) k_matvec(Geo g, const float *__restrict__
     }
 }
 
-void matvec(hipStream_t s, const Level &L, const double *u, double *Au, const double *rhs, double *res) {
-    hipLaunchKernelGGL(k_matvec, dim3(nblk(L.g.pitch)), dim3(256), 0, s, L.g, L.K, L.NODE, u, Au, rhs, res);
+void matvec(hipStream_t s, const Level &L, const double *u, double *Au, const double *rhs, double *res, hipEvent_t e0,
+            hipEvent_t e1) {
+    if (e0)
+        hipExtLaunchKernelGGL(k_matvec, dim3(nblk(L.g.pitch)), dim3(256), 0, s, e0, e1, 0, L.g, L.K, L.NODE, u, Au, rhs,
+                              res);
+    else
+        hipLaunchKernelGGL(k_matvec, dim3(nblk(L.g.pitch)), dim3(256), 0, s, L.g, L.K, L.NODE, u, Au, rhs, res);
 }
 
 // Gauss-Seidel: gauss_seidel (General_matrix_functions.c:1160-1361) as eight
@@ -253,9 +259,12 @@ __global__ void __launch_bounds__(192) k_gs_colour(Geo g, int c, const float *__
     }
 }
 
-void gs_colour(hipStream_t s, const Level &L, int colour, double *d, const double *F) {
+void gs_colour(hipStream_t s, const Level &L, int colour, double *d, const double *F, hipEvent_t e0, hipEvent_t e1) {
     const int tiles = (L.g.cn[colour] + 63) / 64;
-    hipLaunchKernelGGL(k_gs_colour, dim3(tiles), dim3(192), 0, s, L.g, colour, L.K, L.BI, d, F);
+    if (e0)
+        hipExtLaunchKernelGGL(k_gs_colour, dim3(tiles), dim3(192), 0, s, e0, e1, 0, L.g, colour, L.K, L.BI, d, F);
+    else
+        hipLaunchKernelGGL(k_gs_colour, dim3(tiles), dim3(192), 0, s, L.g, colour, L.K, L.BI, d, F);
 }
 
 size_t lds_limit();
