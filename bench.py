@@ -10,10 +10,10 @@ Default workload: 128x128x64 elements in a 2x2x1 box, 6 multigrid levels,
 Ra = 1e7, eta = exp(-ln(1e4) T) clipped to [1e-4, 1], seeded initial
 temperature.  The step-0 Stokes solve and the warmup steps are untimed.
 
-Prints one JSON line (see the README of the measurement contract in
-DESIGN.md): s/timestep (lower is better), the roofline of the dominant
-kernel (the 8-colour Gauss-Seidel pass) from HIP events on the library's
-stream, and a CPU baseline from the C restatement on the host.
+Prints one JSON line: s/timestep (lower is better), the roofline of the
+dominant kernel (the 8-colour Gauss-Seidel pass) from HIP events on the
+library's stream, and a CPU baseline: the unmodified reference (or, where its
+build is absent, the C restatement) timed on the host on a bounded sample.
 """
 from __future__ import annotations
 
@@ -43,12 +43,44 @@ def make_problem(name):
     return deck.problem_from_deck({k: str(v) for k, v in deck.workload(name).items()})
 
 
-def cpu_baseline(name, mg_calls_per_step, seconds_budget=30.0):
-    """Time the C restatement (one host thread, the reference's algorithm with
-    its lexicographic smoother) on a bounded sample of the same workload: the
-    stiffness rebuild, one PG step and one multi_grid call; the per-step
-    estimate scales the multi_grid time by the multigrid calls per step that
-    the GPU run measured."""
+REF_HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+
+def cpu_baseline_reference(name, seed, mg_calls_per_step):
+    """Time the unmodified reference (oracle/_ref/ref_harness, the
+    reference's own objects) on one host thread on a bounded sample of the
+    same workload: the operator build of one step, one multi_grid call and
+    one PG_timestep on the seeded initial temperature (read through the
+    reference's restart reader).  The per-step estimate scales the multi_grid
+    time by the multigrid calls per step that the GPU run measured."""
+    import subprocess
+    import tempfile
+
+    keys = deck.workload(name)
+    p = make_problem(name)
+    with tempfile.TemporaryDirectory() as tmp:
+        out = os.path.join(tmp, "out")
+        keys["datafile"] = os.path.join(out, "run")
+        keys.update({"restart": 1, "restart_timesteps": 0, "oldfile": os.path.join(out, "seed")})
+        deck.write_restart_temperature(os.path.join(out, "seed"), deck.seeded_temperature(p, seed=seed))
+        path = os.path.join(tmp, "deck.input")
+        deck.write_deck(path, keys)
+        r = subprocess.run([REF_HARNESS, path, os.path.join(tmp, "dump"), "time"], capture_output=True, text=True,
+                           timeout=600, cwd=tmp)
+    if r.returncode != 0:
+        raise RuntimeError(f"reference timing sample failed: {r.stderr[-500:]}")
+    f = r.stdout.split()
+    t = {f[k]: float(f[k + 1]) for k in range(0, len(f) - 1, 2) if f[k] in ("build", "mg", "pg")}
+    est = t["build"] + t["pg"] + t["mg"] * mg_calls_per_step
+    return {"value": est, "unit": "s/timestep", "cores": 1, "kind": "reference",
+            "sample": (f"unmodified reference (its own objects, gcc -O2, 1 MPI rank) at {name} size: operator build "
+                       f"{t['build']:.2f} s + one PG_timestep {t['pg']:.2f} s + one multi_grid call {t['mg']:.2f} s x "
+                       f"{mg_calls_per_step:.1f} multigrid calls per step (as counted on the GPU)")}
+
+
+def cpu_baseline_port(name, seed, mg_calls_per_step):
+    """The same sample on the C restatement (oracle/), where the reference
+    build is absent."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
@@ -56,7 +88,7 @@ def cpu_baseline(name, mg_calls_per_step, seconds_budget=30.0):
         pyoracle.build()
     p = make_problem(name)
     o = pyoracle.Oracle(p, smoother=0)
-    T = deck.seeded_temperature(p, seed=1)
+    T = deck.seeded_temperature(p, seed=seed)
     nno = o.nno()
     t0 = time.perf_counter()
     o.set_viscosity(o.visc_from_T(T))
@@ -68,7 +100,6 @@ def cpu_baseline(name, mg_calls_per_step, seconds_budget=30.0):
     t0 = time.perf_counter()
     Fc = F.copy()
     d = np.zeros_like(F)
-    import ctypes as C
     pyoracle.lib().orc_multi_grid(o.h, d, Fc, None, 0.0, o.levmax)
     t_mg = time.perf_counter() - t0
     est = t_rebuild + t_pg + t_mg * mg_calls_per_step
@@ -76,6 +107,26 @@ def cpu_baseline(name, mg_calls_per_step, seconds_budget=30.0):
             "sample": (f"C restatement on 1 host thread at {name} size: stiffness rebuild {t_rebuild:.2f} s + "
                        f"one PG step {t_pg:.2f} s + one multi_grid call {t_mg:.2f} s x "
                        f"{mg_calls_per_step:.1f} multigrid calls per step (as counted on the GPU)")}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/<kernel>_traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE runs,
+    scaled by what the same counters read for a copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh).get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(name, seed, mg_calls_per_step):
+    if os.path.exists(REF_HARNESS):
+        try:
+            return cpu_baseline_reference(name, seed, mg_calls_per_step)
+        except Exception as e:  # noqa: BLE001 - report and fall back to the restatement
+            log(f"reference timing sample unavailable ({e}); timing the C restatement")
+    return cpu_baseline_port(name, seed, mg_calls_per_step)
 
 
 def main():
@@ -92,8 +143,8 @@ def main():
                     help="cap on Uzawa iterations (profiling runs only: changes the workload)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU: the path shards by subdomain "
-                                   "(see DESIGN.md), and no multi-GPU figure is produced"}))
+        print(json.dumps({"error": "this benchmark measures one GPU; the solver would shard by subdomain, "
+                                   "but no multi-GPU figure is produced"}))
         sys.exit(2)
 
     import torch  # device sync for the timed region
@@ -172,12 +223,12 @@ def main():
                    "uzawa_iterations_per_step": float(np.mean([r['stokes']['iterations'] for r in recs])),
                    "mg_calls_per_step": mg_calls},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel": "k_gs_colour",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic("k_gs_colour"), "kernel": "k_gs_colour",
                      "launches": n_gs, "avg_launch_ms": avg_ms},
     }
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(args.config, mg_calls)
+        out["cpu_baseline"] = cpu_baseline(args.config, args.seed, mg_calls)
     print(json.dumps(out), flush=True)
     c.close()
 

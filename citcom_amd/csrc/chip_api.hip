@@ -103,7 +103,7 @@ void do_matvec(chip_ctx *c, int lev, const double *u, double *Au, const double *
 // gauss_seidel (General_matrix_functions.c:1160-1361), 8-colour; returns Ad = K d
 void do_gs(chip_ctx *c, int lev, double *d, const double *F, double *Ad, int sweeps, int guess) {
     const Level &L = c->L[lev];
-    if (gs_small_ok(L)) {
+    if (c->small_levels && gs_small_kind(L)) {
         gs_small(S(c), L, d, F, Ad, sweeps, guess);
         return;
     }
@@ -121,13 +121,23 @@ void do_gs(chip_ctx *c, int lev, double *d, const double *F, double *Ad, int swe
 // L[levmax].vel.  Coarse right-hand sides share storage with fl[] (rhs[lev]
 // is a copy of fl[lev] in the reference, and fl[lev] is not read again).
 // This enqueues the whole fixed kernel sequence; |F|^2 lands in dots[D_TMP].
+// the coarsest-level smoother call of multi_grid (AU[levmin] is not read
+// afterwards): the coarse matrix where there is one
+void coarse_smooth(chip_ctx *c, double *vel, const double *rhs) {
+    const Level &L = c->L[c->levmin];
+    if (L.Mc && c->small_levels)
+        coarse_apply(S(c), L, rhs, vel);
+    else
+        do_gs(c, c->levmin, vel, rhs, L.AU, c->p.v_steps_low, 0);
+}
+
 void enqueue_multi_grid(chip_ctx *c, double *F) {
     const int levmin = c->levmin, levmax = c->levmax;
     const chip_problem &p = c->p;
     Level *L = c->L;
     auto fl = [&](int lev) { return lev == levmax ? F : L[lev].fl; };
     for (int lev = levmax; lev > levmin; lev--) project(S(c), L[lev], L[lev - 1], fl(lev), fl(lev - 1));
-    do_gs(c, levmin, L[levmin].vel, fl(levmin), L[levmin].AU, p.v_steps_low, 0);
+    coarse_smooth(c, L[levmin].vel, fl(levmin));
     for (int lev = levmin + 1; lev <= levmax; lev++) {
         interp(S(c), L[lev - 1], L[lev], L[lev - 1].vel, L[lev].vel);
         for (int Vn = 1; Vn <= p.mg_cycle; Vn++) {
@@ -138,7 +148,7 @@ void enqueue_multi_grid(chip_ctx *c, double *F) {
                 sub(S(c), L[dlev].res, rhs, L[dlev].AU, 3L * L[dlev].g.nno);
                 project(S(c), L[dlev], L[dlev - 1], L[dlev].res, L[dlev - 1].fl);
             }
-            do_gs(c, levmin, L[levmin].vel, L[levmin].fl, L[levmin].AU, p.v_steps_low, 0);
+            coarse_smooth(c, L[levmin].vel, L[levmin].fl);
             for (int ulev = levmin + 1; ulev <= lev; ulev++) {
                 const int cycles = (ulev == levmax) ? p.v_steps_high : p.up_heavy;
                 const long n = 3L * L[ulev].g.nno;
@@ -284,6 +294,7 @@ int rebuild(chip_ctx *c) {
     for (int lev = c->levmin; lev <= c->levmax; lev++) elt_g(q, c->L[lev]);
     for (int lev = c->levmax; lev >= c->levmin; lev--) node_ks(q, c->L[lev], c->p);
     for (int lev = c->levmax; lev >= c->levmin; lev--) build_bpi(q, c->L[lev], c->p.precond);
+    if (c->L[c->levmin].Mc) coarse_build(q, c->L[c->levmin], c->p.v_steps_low);
     CHIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -567,6 +578,14 @@ int chip_apply(chip_ctx *c, const char *op, int lev, const double *in, const dou
         do_gs(c, lev, a, b, d, arg, guess);
         CHIP_CHECK(down(out, a, nv));
         CHIP_CHECK(down(out2, d, nv));
+    } else if (o == "coarse") {
+        if (lev != c->levmin || !L.Mc) {
+            set_error("coarse: the coarsest level has no coarse matrix");
+            return -1;
+        }
+        CHIP_CHECK(up(b, in2, nv));
+        coarse_apply(q, L, b, a);
+        CHIP_CHECK(down(out, a, nv));
     } else if (o == "project") {
         if (lev == c->levmin) {
             set_error("project: no coarser level");
@@ -679,6 +698,15 @@ int chip_set_option(chip_ctx *c, const char *name, int value) {
     const std::string n(name ? name : "");
     if (n == "graphs") {
         c->use_graphs = value != 0;
+        return 0;
+    }
+    if (n == "small_levels") {
+        c->small_levels = value != 0;
+        if (c->mg_graph) {   // the captured kernel sequence changes
+            CHIP_CHECK(hipStreamSynchronize(c->stream));
+            CHIP_CHECK(hipGraphExecDestroy(c->mg_graph));
+            c->mg_graph = nullptr;
+        }
         return 0;
     }
     set_error("chip_set_option: unknown option " + n);

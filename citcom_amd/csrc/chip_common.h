@@ -60,6 +60,8 @@ struct Level {
     float *VN = nullptr;     // [nno] nodal viscosity scratch
     // multigrid work vectors [3 nno]
     double *vel = nullptr, *res = nullptr, *fl = nullptr, *del = nullptr, *AU = nullptr;
+    // coarsest level: its smoother call as a matrix [3nno][3nno], row-major
+    double *Mc = nullptr;
     // FP32 copies for the mixed-precision cycle
     float *velf = nullptr, *resf = nullptr, *flf = nullptr, *delf = nullptr, *AUf = nullptr;
 };
@@ -109,6 +111,7 @@ struct chip_ctx {
     int solution_cycles = 0;
     bool built = false;      // operators constructed at least once
     bool use_graphs = true;  // replay multi_grid as a captured HIP graph
+    bool small_levels = true;  // single-workgroup smoother on small levels
     hipGraphExec_t mg_graph = nullptr;
     chip::Prof prof;
     long dev_bytes = 0;

@@ -29,7 +29,12 @@ void gs_colour(hipStream_t s, const Level &L, int colour, double *d, const doubl
                hipEvent_t e1 = nullptr);
 // every sweep of a small level in one workgroup, then Ad = K d
 void gs_small(hipStream_t s, const Level &L, double *d, const double *F, double *Ad, int sweeps, int guess);
-bool gs_small_ok(const Level &L);
+// 1: the level's stiffness and vectors fit in LDS (gs_small), 0: one launch
+// per colour pass
+int gs_small_kind(const Level &L);
+// coarsest level: Mc = the v_steps_low-sweep smoother as a matrix, d = Mc F
+void coarse_build(hipStream_t s, const Level &L, int sweeps);
+void coarse_apply(hipStream_t s, const Level &L, const double *F, double *d);
 void project(hipStream_t s, const Level &fine, const Level &coarse, const double *AU, double *AD, bool do_strip = true);
 void interp(hipStream_t s, const Level &coarse, const Level &fine, const double *AD, double *AU, bool do_strip = true);
 void div(hipStream_t s, const Level &L, const double *U, double *divU);

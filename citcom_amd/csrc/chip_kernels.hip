@@ -107,6 +107,14 @@ __device__ __forceinline__ int colour_valid(const Geo &g, int q) {
 // 243 contiguous 256-byte lines (62 KB) per row product.
 __device__ __forceinline__ size_t ktile(int q) { return (size_t)(q >> 6) * (KROW * 64) + (q & 63); }
 
+// Workgroups are dispatched to the 8 XCDs round-robin.  This gives XCD x a
+// contiguous range of the grid's tiles, so the vector entries that
+// neighbouring tiles share are fetched into one L2 rather than several.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    const int x = b & 7, y = b >> 3, qn = nb >> 3, rn = nb & 7;
+    return (x < rn ? x * (qn + 1) : rn * (qn + 1) + (x - rn) * qn) + y;
+}
+
 // full-row product of the node at colour index q, (k,i,j), with u.  The
 // neighbour order (dy, dx, dz ascending) and the per-block expression are
 // those of the CPU restatement.  Loads are issued one dy-plane (9 blocks) at
@@ -161,43 +169,109 @@ __device__ __forceinline__ void strip3(unsigned f, double v[3]) {
 }
 
 // ---------------------------------------------------------------------------
-// Large levels: one workgroup per 64-node tile of one colour, three
-// wavefronts per workgroup, wavefront r forming row r of the block-row
-// product of its 64 nodes.  Each wavefront streams the 81 coefficient runs of
-// its row (81 x 256 contiguous bytes); the three share the gathered vector
-// entries through L1.  The row sums follow row_product's neighbour order, so
-// results are bit-identical to a one-thread-per-node evaluation.
+// Smoother row sums.  The 8-colour smoother sums a row in a fixed order: the
+// block terms t[b] = k0*u0 + k1*u1 + k2*u2 of each y-plane p = dy+1
+// (b = (dz+1) + 3(dx+1); absent neighbours and b >= 9 give 0) are summed in a
+// pairwise tree over 16 slots (t[q] += t[q+w], w = 1, 2, 4, 8, q a multiple
+// of 2w) into P[p], and the row sum is (P[0] + P[1]) + P[2].  A thread forms
+// a plane's tree in registers (PairSum16); a 16-lane row of a wavefront
+// forms the same tree with DPP row shifts (IEEE addition is commutative, so
+// both give the same bits).  The matvec keeps row_product's sequential order.
 
-__device__ __forceinline__ double row_sum(const Geo &g, const float *__restrict__ kp, int r, int k, int i, int j,
-                                          const double *__restrict__ u) {
-    const int self = nid(g, k, i, j);
-    double out = 0.0;
+// tree over 16 slots formed term by term: push<B>(t[B]) for B = 0..15 keeps
+// one partial sum per tree level; after B = 15, total is t[0] of the tree
+struct PairSum16 {
+    double st[4];
+    double total;
+    template <int B>
+    __device__ __forceinline__ void push(double v) {
 #pragma unroll
-    for (int dy = -1; dy <= 1; dy++) {
-        const bool oy = (unsigned)(j + dy) < (unsigned)g.noy;
-        float kv[27];
-        double uv[27];
-#pragma unroll
-        for (int dx = -1; dx <= 1; dx++) {
-            const bool ox = oy && (unsigned)(i + dx) < (unsigned)g.nox;
-#pragma unroll
-            for (int dz = -1; dz <= 1; dz++) {
-                const int b = (dz + 1) + 3 * (dx + 1);
-                const bool ok = ox && (unsigned)(k + dz) < (unsigned)g.noz;
-                const int m = ok ? self + dz + g.noz * (dx + g.nox * dy) : self;
-                uv[3 * b] = u[3 * m];
-                uv[3 * b + 1] = u[3 * m + 1];
-                uv[3 * b + 2] = u[3 * m + 2];
-                const int ci = 9 * (b + 9 * (dy + 1)) + 3 * r;
-                kv[3 * b] = kp[ci * 64];
-                kv[3 * b + 1] = kp[(ci + 1) * 64];
-                kv[3 * b + 2] = kp[(ci + 2) * 64];
+        for (int l = 0; l < 4; l++) {
+            if (B & (1 << l)) {
+                v = st[l] + v;
+            } else {
+                st[l] = v;
+                return;
             }
         }
-#pragma unroll
-        for (int b = 0; b < 9; b++) out += kv[3 * b] * uv[3 * b] + kv[3 * b + 1] * uv[3 * b + 1] + kv[3 * b + 2] * uv[3 * b + 2];
+        total = v;
     }
-    return out;
+};
+
+// lane i of each 16-lane row receives lane i+W's value (row_shl:W)
+template <int W>
+__device__ __forceinline__ double dpp_row_shl(double v) {
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    const int rlo = __builtin_amdgcn_update_dpp(0, lo, 0x100 + W, 0xF, 0xF, false);
+    const int rhi = __builtin_amdgcn_update_dpp(0, hi, 0x100 + W, 0xF, 0xF, false);
+    return __hiloint2double(rhi, rlo);
+}
+
+// PairSum16 over the 16 lanes of a DPP row; lane 16k holds the row's total
+__device__ __forceinline__ double tree16_lanes(double v) {
+    v = v + dpp_row_shl<1>(v);
+    v = v + dpp_row_shl<2>(v);
+    v = v + dpp_row_shl<4>(v);
+    v = v + dpp_row_shl<8>(v);
+    return v;
+}
+
+__device__ __forceinline__ double lane_value(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+// Plane P = dy+1 of the three rows of node (k, i, j), coefficients kp (its
+// tile column), into P[r]
+template <int P>
+__device__ __forceinline__ void plane_rows(const Geo &g, const float *__restrict__ kp, int k, int i, int j, int self,
+                                           const double *__restrict__ u, double out[3]) {
+    constexpr int DY = P - 1;
+    const bool oy = (unsigned)(j + DY) < (unsigned)g.noy;
+    float kv[81];
+    double uv[27];
+    bool okb[9];
+#pragma unroll
+    for (int dx = -1; dx <= 1; dx++) {
+        const bool ox = oy && (unsigned)(i + dx) < (unsigned)g.nox;
+#pragma unroll
+        for (int dz = -1; dz <= 1; dz++) {
+            const int b = (dz + 1) + 3 * (dx + 1);
+            const bool ok = ox && (unsigned)(k + dz) < (unsigned)g.noz;
+            const int m = ok ? self + dz + g.noz * (dx + g.nox * DY) : self;
+            okb[b] = ok;
+            uv[3 * b] = u[3 * m];
+            uv[3 * b + 1] = u[3 * m + 1];
+            uv[3 * b + 2] = u[3 * m + 2];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 81; c++) kv[c] = kp[(81 * P + c) * 64];
+    PairSum16 ps[3];
+#define CHIP_TERM(B, R)                                                                                            \
+    (okb[B] ? kv[9 * (B) + 3 * (R)] * uv[3 * (B)] + kv[9 * (B) + 3 * (R) + 1] * uv[3 * (B) + 1] +                  \
+                  kv[9 * (B) + 3 * (R) + 2] * uv[3 * (B) + 2]                                                      \
+            : 0.0)
+#define CHIP_PUSH(B)                    \
+    ps[0].push<B>(CHIP_TERM(B, 0));     \
+    ps[1].push<B>(CHIP_TERM(B, 1));     \
+    ps[2].push<B>(CHIP_TERM(B, 2))
+    CHIP_PUSH(0); CHIP_PUSH(1); CHIP_PUSH(2); CHIP_PUSH(3); CHIP_PUSH(4);
+    CHIP_PUSH(5); CHIP_PUSH(6); CHIP_PUSH(7); CHIP_PUSH(8);
+#undef CHIP_PUSH
+#undef CHIP_TERM
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        ps[r].push<9>(0.0);
+        ps[r].push<10>(0.0);
+        ps[r].push<11>(0.0);
+        ps[r].push<12>(0.0);
+        ps[r].push<13>(0.0);
+        ps[r].push<14>(0.0);
+        ps[r].push<15>(0.0);
+        out[r] = ps[r].total;
+    }
 }
 
 // matvec: n_assemble_del2_u (Element_calculations.c:552-621) with strip;
@@ -206,7 +280,7 @@ __device__ __forceinline__ double row_sum(const Geo &g, const float *__restrict_
 __global__ void __launch_bounds__(256) k_matvec(Geo g, const float *__restrict__ K, const unsigned *__restrict__ NODE,
                                                 const double *__restrict__ u, double *__restrict__ Au,
                                                 const double *__restrict__ rhs, double *__restrict__ res) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int col = colour_valid(g, q);
     if (col < 0) return;
     int k, i, j;
@@ -236,26 +310,49 @@ void matvec(hipStream_t s, const Level &L, const double *u, double *Au, const do
 
 // Gauss-Seidel: gauss_seidel (General_matrix_functions.c:1160-1361) as eight
 // colour passes; the node-block update t = (F - (K d)_node) * BI, rounded to
-// float as the reference's temp[] is, then d += t.  The three rows of a node
-// are read before any is written (workgroup barrier), as the reference
-// updates all three unknowns of a node from the same Ad.
+// float as the reference's temp[] is, then d += t.  One workgroup per 64-node
+// tile of one colour: wavefront p forms y-plane p of the three rows of its 64
+// nodes (81 coalesced coefficient loads and 27 vector gathers, all issued
+// together), the partial sums meet in LDS, and wavefront r updates row r.
+// Every row is formed before any unknown is written (workgroup barrier), as
+// the reference updates all three unknowns of a node from the same Ad.
 __global__ void __launch_bounds__(192) k_gs_colour(Geo g, int c, const float *__restrict__ K,
                                                    const double *__restrict__ BI, double *d,
                                                    const double *__restrict__ F) {
-    const int q = g.cs[c] + blockIdx.x * 64 + (threadIdx.x & 63), r = threadIdx.x >> 6;
+    __shared__ double Ps[3][3][64];   // [plane][row][node]
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int q = g.cs[c] + xcd_tile(blockIdx.x, gridDim.x) * 64 + lane;
     const bool act = q < g.cs[c] + g.cn[c];
-    double out = 0.0;
-    int n = 0;
+    int k = 0, i = 0, j = 0;
+    if (act) colour_kij(g, c, q, k, i, j);
+    const int self = nid(g, k, i, j);
+    // row w's F, BI and d, read before the barrier: only this thread
+    // writes d[3 self + w] in this pass
+    double Fr = 0.0, Br = 0.0, dr = 0.0;
     if (act) {
-        int k, i, j;
-        colour_kij(g, c, q, k, i, j);
-        out = row_sum(g, K + ktile(q), r, k, i, j, d);
-        n = nid(g, k, i, j);
+        Fr = F[3 * self + w];
+        Br = BI[3 * self + w];
+        dr = d[3 * self + w];
+    }
+    if (act) {
+        double out[3];
+        const float *kp = K + ktile(q);
+        if (w == 0)
+            plane_rows<0>(g, kp, k, i, j, self, d, out);
+        else if (w == 1)
+            plane_rows<1>(g, kp, k, i, j, self, d, out);
+        else
+            plane_rows<2>(g, kp, k, i, j, self, d, out);
+        Ps[w][0][lane] = out[0];
+        Ps[w][1][lane] = out[1];
+        Ps[w][2][lane] = out[2];
     }
     __syncthreads();
     if (act) {
-        const float t = (float)((F[3 * n + r] - out) * BI[3 * n + r]);
-        d[3 * n + r] += t;
+        const int r = w;
+        const double o = (Ps[0][r][lane] + Ps[1][r][lane]) + Ps[2][r][lane];
+        const float t = (float)((Fr - o) * Br);
+        d[3 * self + r] = dr + t;
     }
 }
 
@@ -267,118 +364,118 @@ void gs_colour(hipStream_t s, const Level &L, int colour, double *d, const doubl
         hipLaunchKernelGGL(k_gs_colour, dim3(tiles), dim3(192), 0, s, L.g, colour, L.K, L.BI, d, F);
 }
 
-size_t lds_limit();
-__host__ __device__ inline size_t kstage_bytes(int nno);
-__host__ __device__ inline size_t vstage_bytes(int nno);
+// ---------------------------------------------------------------------------
+// A level whose stiffness fits in LDS (the coarsest, 4x4x2 elements at the
+// workloads here) runs every sweep of a gauss_seidel call, and the final
+// Ad = K d, in one workgroup.  One wavefront per node: lane 16p + b forms
+// block b of y-plane p for the three rows, each 16-lane row sums its plane
+// with DPP shifts, lane 0 combines the planes, lanes 0..2 update.
 
-// levels whose stiffness fits in LDS run every sweep in one workgroup
-// (gs_small); larger ones launch one kernel per colour pass
-bool gs_small_ok(const Level &L) { return kstage_bytes(L.g.nno) + vstage_bytes(L.g.nno) <= lds_limit(); }
-
-// Small levels (stiffness resident in LDS) run all their sweeps in one
-// workgroup.  A colour pass there has only a few nodes, so each node gets a
-// wavefront: lane s < 27 forms the three row terms of neighbour block s, and
-// lanes r < 3 add the 27 terms of row r in the neighbour order of
-// row_product (bit-identical sums) and update unknown r.  Node coordinates
-// and neighbour indices are tabulated in LDS once per call.
-constexpr int ROWS_TPB = 1024;
+constexpr int SMALL_TPB = 1024;
 
 __host__ __device__ inline size_t kstage_bytes(int nno) { return (((size_t)KROW * nno * 4 + 15) / 16) * 16; }
-__host__ __device__ inline size_t vstage_bytes(int nno) {
-    return (size_t)9 * nno * 8 + (size_t)27 * nno * 4 + (size_t)(ROWS_TPB / 64) * 81 * 8;
+__host__ __device__ inline size_t vstage_bytes(int nno) { return (size_t)9 * nno * 8 + (size_t)27 * nno * 4; }
+
+__device__ __forceinline__ int compact_base(const Geo &g, int col) {
+    int base = 0;
+    for (int m = 0; m < col; m++) base += g.cn[m];
+    return base;
 }
 
-__device__ __forceinline__ void wave_sync_lds() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__global__ void __launch_bounds__(ROWS_TPB) k_gs_lds(Geo g, const float *__restrict__ K, const double *__restrict__ BI,
-                                                     const unsigned *__restrict__ NODE, double *d,
-                                                     const double *__restrict__ F, double *__restrict__ Ad, int sweeps,
-                                                     int guess) {
+// unit = 1: workgroup j smooths the unit right-hand side e_j into column j
+// of the row-major [3 nno][3 nno] matrix d (the coarse operator), without Ad.
+__global__ void __launch_bounds__(SMALL_TPB) k_gs_lds(Geo g, const float *__restrict__ K, const double *__restrict__ BI,
+                                                      const unsigned *__restrict__ NODE, double *d,
+                                                      const double *__restrict__ F, double *__restrict__ Ad, int sweeps,
+                                                      int guess, int unit) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int nno = g.nno, tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = blockDim.x >> 6;
     float *Ks = (float *)smem;                                   // [243][nno], compact colour order
     double *ds = (double *)(smem + kstage_bytes(nno));           // [3 nno] natural order
     double *Fs = ds + 3 * nno, *Bs = Fs + 3 * nno;
-    int *nb = (int *)(Bs + 3 * nno);                             // [nno][27] neighbour (or self) node
-    double *T = (double *)(nb + 27 * nno);                       // [nw][27][3]
-    // stage: compact index cq of colour-ordered q
+    int *nb = (int *)(Bs + 3 * nno);                             // [nno][27] neighbour node or -1
     for (int q = tid; q < g.pitch; q += blockDim.x) {
         const int col = colour_valid(g, q);
         if (col < 0) continue;
-        int base = 0;
-        for (int m = 0; m < col; m++) base += g.cn[m];
-        const int cq = base + (q - g.cs[col]);
+        const int cq = compact_base(g, col) + (q - g.cs[col]);
         int k, i, j;
         colour_kij(g, col, q, k, i, j);
         const int self = nid(g, k, i, j);
         const float *kp = K + ktile(q);
 #pragma unroll 27
         for (int ci = 0; ci < KROW; ci++) Ks[ci * nno + cq] = kp[ci * 64];
-        for (int s = 0; s < 27; s++) {
-            const int dz = s % 3 - 1, dx = (s / 3) % 3 - 1, dy = s / 9 - 1;
+        for (int sl = 0; sl < 27; sl++) {
+            const int dz = sl % 3 - 1, dx = (sl / 3) % 3 - 1, dy = sl / 9 - 1;
             const bool ok = (unsigned)(k + dz) < (unsigned)g.noz && (unsigned)(i + dx) < (unsigned)g.nox &&
                             (unsigned)(j + dy) < (unsigned)g.noy;
-            nb[cq * 27 + s] = ok ? self + dz + g.noz * (dx + g.nox * dy) : self;
+            nb[cq * 27 + sl] = ok ? self + dz + g.noz * (dx + g.nox * dy) : -1;
         }
     }
     for (int idx = tid; idx < 3 * nno; idx += blockDim.x) {
         ds[idx] = guess ? d[idx] : 0.0;
-        Fs[idx] = F[idx];
+        Fs[idx] = unit ? (idx == (int)blockIdx.x ? 1.0 : 0.0) : F[idx];
         Bs[idx] = BI[idx];
     }
     __syncthreads();
-    double *Tw = T + w * 81;
-    // terms of node cq into Tw (lanes < 27), then row r's sum (lanes < 3)
-    auto node_rows = [&](int cq) {
-        if (lane < 27) {
-            const int m = nb[cq * 27 + lane];
-            const double u0 = ds[3 * m], u1 = ds[3 * m + 1], u2 = ds[3 * m + 2];
-#pragma unroll
-            for (int r = 0; r < 3; r++) {
-                const int ci = lane * 9 + 3 * r;
-                Tw[lane * 3 + r] = Ks[ci * nno + cq] * u0 + Ks[(ci + 1) * nno + cq] * u1 + Ks[(ci + 2) * nno + cq] * u2;
-            }
-        }
-        wave_sync_lds();
-        double out = 0.0;
-        if (lane < 3)
-            for (int t = 0; t < 27; t++) out += Tw[t * 3 + lane];
-        return out;
-    };
+    const int p = lane >> 4, b = lane & 15;
+    const int sl = (p < 3 && b < 9) ? 9 * p + b : -1;   // neighbour slot of this lane
     for (int cyc = 0; cyc < sweeps; cyc++) {
         int cb = 0;
         for (int c = 0; c < 8; c++) {
-            for (int t = w; t < g.cn[c]; t += nw) {
+            const int cn = g.cn[c];
+            for (int t = w; t < cn; t += nw) {   // wave-uniform
                 const int cq = cb + t;
-                const double out = node_rows(cq);
+                double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+                if (sl >= 0) {
+                    const int m = nb[cq * 27 + sl];
+                    if (m >= 0) {
+                        const double u0 = ds[3 * m], u1 = ds[3 * m + 1], u2 = ds[3 * m + 2];
+                        const float *kc = Ks + (sl * 9) * nno + cq;
+                        v0 = kc[0] * u0 + kc[nno] * u1 + kc[2 * nno] * u2;
+                        v1 = kc[3 * nno] * u0 + kc[4 * nno] * u1 + kc[5 * nno] * u2;
+                        v2 = kc[6 * nno] * u0 + kc[7 * nno] * u1 + kc[8 * nno] * u2;
+                    }
+                }
+                v0 = tree16_lanes(v0);
+                v1 = tree16_lanes(v1);
+                v2 = tree16_lanes(v2);
+                const double o0 = (lane_value(v0, 0) + lane_value(v0, 16)) + lane_value(v0, 32);
+                const double o1 = (lane_value(v1, 0) + lane_value(v1, 16)) + lane_value(v1, 32);
+                const double o2 = (lane_value(v2, 0) + lane_value(v2, 16)) + lane_value(v2, 32);
                 if (lane < 3) {
                     const int n = nb[cq * 27 + 13];
-                    const float tt = (float)((Fs[3 * n + lane] - out) * Bs[3 * n + lane]);
+                    const double o = lane == 0 ? o0 : (lane == 1 ? o1 : o2);
+                    const float tt = (float)((Fs[3 * n + lane] - o) * Bs[3 * n + lane]);
                     ds[3 * n + lane] += tt;
                 }
-                wave_sync_lds();
             }
-            cb += g.cn[c];
+            cb += cn;
             __syncthreads();
         }
     }
-    // Ad = K d, BC rows stripped
-    for (int cq = w; cq < nno; cq += nw) {
-        double out = node_rows(cq);
-        if (lane < 3) {
-            const int n = nb[cq * 27 + 13];
-            const unsigned f = NODE[n];
-            if ((lane == 0 && (f & VBX)) || (lane == 1 && (f & VBY)) || (lane == 2 && (f & VBZ))) out = 0.0;
-            Ad[3 * n + lane] = out;
+    // Ad = K d in row_product's sequential order, BC rows stripped
+    for (int cq = tid; Ad && cq < nno; cq += blockDim.x) {
+        const int n = nb[cq * 27 + 13];
+        double out[3] = {0.0, 0.0, 0.0};
+        for (int s2 = 0; s2 < 27; s2++) {
+            const int m = nb[cq * 27 + s2];
+            const int mm = m >= 0 ? m : n;   // absent neighbours have zero coefficients
+            const double u0 = ds[3 * mm], u1 = ds[3 * mm + 1], u2 = ds[3 * mm + 2];
+            const float *kc = Ks + (s2 * 9) * nno + cq;
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+                out[r] += kc[(3 * r) * nno] * u0 + kc[(3 * r + 1) * nno] * u1 + kc[(3 * r + 2) * nno] * u2;
         }
-        wave_sync_lds();
+        strip3(NODE[n], out);
+        Ad[3 * n] = out[0];
+        Ad[3 * n + 1] = out[1];
+        Ad[3 * n + 2] = out[2];
     }
     __syncthreads();
-    for (int idx = tid; idx < 3 * nno; idx += blockDim.x) d[idx] = ds[idx];
+    if (unit)
+        for (int idx = tid; idx < 3 * nno; idx += blockDim.x) d[(size_t)idx * 3 * nno + blockIdx.x] = ds[idx];
+    else
+        for (int idx = tid; idx < 3 * nno; idx += blockDim.x) d[idx] = ds[idx];
 }
 
 size_t lds_limit() {
@@ -396,10 +493,50 @@ size_t lds_limit() {
     return lim;
 }
 
+int gs_small_kind(const Level &L) {
+    const size_t nno = L.g.nno;
+    return kstage_bytes(nno) + vstage_bytes(nno) <= lds_limit() ? 1 : 0;
+}
+
 void gs_small(hipStream_t s, const Level &L, double *d, const double *F, double *Ad, int sweeps, int guess) {
     const int nno = L.g.nno;
-    hipLaunchKernelGGL(k_gs_lds, dim3(1), dim3(ROWS_TPB), kstage_bytes(nno) + vstage_bytes(nno), s, L.g, L.K, L.BI,
-                       L.NODE, d, F, Ad, sweeps, guess);
+    hipLaunchKernelGGL(k_gs_lds, dim3(1), dim3(SMALL_TPB), kstage_bytes(nno) + vstage_bytes(nno), s, L.g, L.K, L.BI,
+                       L.NODE, d, F, Ad, sweeps, guess, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Coarsest level as a matrix.  The gauss_seidel call on the coarsest level of
+// multi_grid (v_steps_low sweeps from a zero guess, General_matrix_functions.c
+// :574, :613) is linear in its right-hand side apart from the FP32 rounding
+// of each increment, and its Ad is not read afterwards.  Column j of Mc is
+// the smoother's result for e_j (one workgroup per column, rebuilt with the
+// stiffness); a call is then d = Mc F.
+
+void coarse_build(hipStream_t s, const Level &L, int sweeps) {
+    const int nno = L.g.nno;
+    hipLaunchKernelGGL(k_gs_lds, dim3(3 * nno), dim3(SMALL_TPB), kstage_bytes(nno) + vstage_bytes(nno), s, L.g, L.K,
+                       L.BI, L.NODE, L.Mc, nullptr, nullptr, sweeps, 0, 1);
+}
+
+// d = Mc F: sixteen lanes per row i, lane t summing j = t, t+16, ... in
+// order into S[t]; the S[t] meet in a pairwise tree (S[q] += S[q+w],
+// w = 1, 2, 4, 8) formed by a butterfly over the 16 lanes
+__global__ void __launch_bounds__(256) k_coarse_apply(const double *__restrict__ M, const double *__restrict__ F,
+                                                      double *__restrict__ d, int n3) {
+    const int t = threadIdx.x & 15, i = blockIdx.x * 16 + (threadIdx.x >> 4);
+    double S = 0.0;
+    if (i < n3)
+        for (int j = t; j < n3; j += 16) S = S + M[(size_t)i * n3 + j] * F[j];
+    S = S + __shfl_xor(S, 1, 64);
+    S = S + __shfl_xor(S, 2, 64);
+    S = S + __shfl_xor(S, 4, 64);
+    S = S + __shfl_xor(S, 8, 64);
+    if (i < n3 && t == 0) d[i] = S;
+}
+
+void coarse_apply(hipStream_t s, const Level &L, const double *F, double *d) {
+    const int n3 = 3 * L.g.nno;
+    hipLaunchKernelGGL(k_coarse_apply, dim3((n3 + 15) / 16), dim3(256), 0, s, L.Mc, F, d, n3);
 }
 
 // ---------------------------------------------------------------------------

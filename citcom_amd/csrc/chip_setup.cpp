@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "chip_common.h"
+#include "chip_kernels.h"
 
 namespace chip {
 
@@ -369,6 +370,10 @@ int build_mesh(chip_ctx *c) {
         // isoviscous until a viscosity is set
         std::vector<float> one((size_t)g.nel * 8, 1.0f);
         CHIP_CHECK(hipMemcpy(L.EVI, one.data(), one.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    {
+        Level &C = c->L[c->levmin];
+        if (gs_small_kind(C) && dev_alloc(c, &C.Mc, 9 * (size_t)C.g.nno * C.g.nno)) return -1;
     }
     const Geo &g = c->L[c->levmax].g;
     const size_t nno = g.nno, nel = g.nel, neq = 3 * nno;

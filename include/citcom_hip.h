@@ -113,6 +113,8 @@ int chip_construct_stiffness_B_matrix(chip_ctx *c);
  *     "gs0"/"gs1" gauss_seidel guess 0/1 with arg sweeps, 8-colour
  *         (General_matrix_functions.c:1160-1361); in = d0 (gs1), in2 = F,
  *         out = d0 after, out2 = A d0
+ *     "coarse" (coarsest level) the multigrid's coarsest-level smoother call
+ *         (v_steps_low sweeps from zero) applied as its matrix: out = Mc in2
  *     "project" project_vector lev -> lev-1 (Solver_multigrid.c:72-159)
  *     "interp"  interp_vector lev-1 -> lev (Solver_multigrid.c:173-298)
  *     "div" assemble_div_u (Element_calculations.c:691-720)
@@ -158,7 +160,10 @@ int chip_profile_read(chip_ctx *c, long *launches, double *total_ms, double *byt
 /* device memory held by the context, bytes */
 long chip_device_bytes(chip_ctx *c);
 /* "graphs": 1 (default) replays each multigrid call as one captured HIP
- * graph, 0 launches its kernels one by one; results are identical. */
+ * graph, 0 launches its kernels one by one; results are identical.
+ * "small_levels": 1 (default) smooths levels whose stiffness fits in LDS in
+ * one single-workgroup kernel per gauss_seidel call, 0 launches one kernel
+ * per colour pass on every level; results are identical. */
 int chip_set_option(chip_ctx *c, const char *name, int value);
 
 #ifdef __cplusplus

@@ -72,6 +72,7 @@ struct orc
 	float elapsed_time;
 	/* work */
 	double *mg_vel[MAXL], *mg_res[MAXL], *mg_rhs[MAXL], *mg_fl[MAXL], *mg_del[MAXL], *mg_AU[MAXL];
+	double *Mc;	/* coarsest-level smoother as a matrix (smoother 1), column-major */
 };
 
 static double lpoly(int p, double y)
@@ -470,6 +471,7 @@ void orc_free(orc *o)
 	}
 	for(d = 0; d < 3; d++)
 		free(o->TB[d]);
+	free(o->Mc);
 	free(o->mat);
 	free(o->expansivity);
 	free(o->diffusivity);
@@ -853,6 +855,50 @@ static void build_bpi(orc *o, int lev)
 }
 
 /* construct_stiffness_B_matrix, Construct_arrays.c:834-889 */
+/* The coarsest level's gauss_seidel call in multi_grid (v_steps_low sweeps
+ * from a zero guess, General_matrix_functions.c:574, :613) is linear in its
+ * right-hand side apart from the FP32 rounding of each increment.  With the
+ * 8-colour smoother it is applied as a matrix: column j is the smoother's
+ * result for the unit right-hand side e_j, built whenever the operators are
+ * rebuilt.  Mc F then equals the sweeps on F up to that FP32 rounding. */
+static void build_coarse_operator(orc *o)
+{
+	const int lev = o->levmin, n3 = o->neq[lev];
+	double *e = calloc(n3, sizeof(double)), *Ad = malloc(sizeof(double) * n3);
+	int j;
+	free(o->Mc);
+	o->Mc = malloc(sizeof(double) * (size_t)n3 * n3);
+	for(j = 0; j < n3; j++)
+	{
+		e[j] = 1.0;
+		orc_gauss_seidel(o, lev, o->Mc + (size_t)j * n3, e, Ad, o->p.v_steps_low, 0);
+		e[j] = 0.0;
+	}
+	free(e);
+	free(Ad);
+}
+
+/* d = Mc F; row i sums j = t, t+16, t+32, ... into S[t] (t < 16), then the
+ * S[t] meet in a pairwise tree: S[q] += S[q+w] for w = 1, 2, 4, 8 and q a
+ * multiple of 2w; d[i] = S[0] */
+void orc_coarse_solve(orc *o, const double *F, double *d)
+{
+	const int n3 = o->neq[o->levmin];
+	int i, j, w, q;
+	for(i = 0; i < n3; i++)
+	{
+		double S[16];
+		for(q = 0; q < 16; q++)
+			S[q] = 0.0;
+		for(j = 0; j < n3; j++)
+			S[j & 15] = S[j & 15] + o->Mc[(size_t)j * n3 + i] * F[j];
+		for(w = 1; w < 16; w *= 2)
+			for(q = 0; q < 16; q += 2 * w)
+				S[q] = S[q] + S[q + w];
+		d[i] = S[0];
+	}
+}
+
 void orc_build_operators(orc *o)
 {
 	int lev, e;
@@ -866,6 +912,8 @@ void orc_build_operators(orc *o)
 		build_bpi(o, lev);
 	/* rebuild_BI_on_boundary (:892-952) changes only OFFSIDE nodes, which a
 	 * single subdomain does not have */
+	if(o->p.smoother == 1)
+		build_coarse_operator(o);
 }
 
 void orc_set_viscosity(orc *o, const float *evi)
@@ -926,12 +974,59 @@ void orc_matvec(orc *o, int lev, const double *u, double *Au, int strip)
 		orc_strip(o, lev, Au);
 }
 
+/* The same row product summed in the 8-colour smoother's fixed order: the
+ * block terms t[p][b] = k0*u0 + k1*u1 + k2*u2 of each y-plane p = dy+1
+ * (b = (dz+1) + 3(dx+1), absent neighbours and b >= 9 are 0) are summed in a
+ * pairwise tree over 16 slots, t[q] += t[q+w] for w = 1, 2, 4, 8 and q a
+ * multiple of 2w, giving P[p] = t[p][0]; the row sum is (P[0] + P[1]) + P[2].
+ * (On the GPU one wavefront handles a plane, or a 16-lane row of lanes
+ * does, which is why the planes are summed separately.)  The matvec keeps
+ * the sequential order of row_apply. */
+static inline void row_apply_tree(const orc *o, int lev, int n, const double *u, double out[3])
+{
+	const float *row = o->K[lev] + (size_t)n * KROW;
+	const int noz = o->noz[lev], nox = o->nox[lev], noy = o->noy[lev];
+	const int k = n % noz, i = (n / noz) % nox, j = n / (noz * nox);
+	double P[3][3];
+	int p, b, r, w, q;
+	for(p = 0; p < 3; p++)
+	{
+		double t[3][16];
+		for(r = 0; r < 3; r++)
+			for(b = 0; b < 16; b++)
+				t[r][b] = 0.0;
+		for(b = 0; b < 9; b++)
+		{
+			const int dz = b % 3 - 1, dx = b / 3 - 1, dy = p - 1;
+			const int kk = k + dz, ii = i + dx, jj = j + dy;
+			const int s = nslot(dz, dx, dy);
+			int m;
+			if(kk < 0 || kk >= noz || ii < 0 || ii >= nox || jj < 0 || jj >= noy)
+				continue;
+			m = nidx(o, lev, kk, ii, jj);
+			for(r = 0; r < 3; r++)
+				t[r][b] = row[r * 81 + s * 3 + 0] * u[3 * m] + row[r * 81 + s * 3 + 1] * u[3 * m + 1] + row[r * 81 + s * 3 + 2] * u[3 * m + 2];
+		}
+		for(w = 1; w < 16; w *= 2)
+			for(q = 0; q < 16; q += 2 * w)
+				for(r = 0; r < 3; r++)
+					t[r][q] = t[r][q] + t[r][q + w];
+		for(r = 0; r < 3; r++)
+			P[r][p] = t[r][0];
+	}
+	for(r = 0; r < 3; r++)
+		out[r] = (P[r][0] + P[r][1]) + P[r][2];
+}
+
 static void node_update(orc *o, int lev, int n, double *d0, const double *F)
 {
 	double Ad[3];
 	float t[3];
 	int r;
-	row_apply(o, lev, n, d0, Ad);
+	if(o->p.smoother == 0)
+		row_apply(o, lev, n, d0, Ad);
+	else
+		row_apply_tree(o, lev, n, d0, Ad);
 	/* temp is higher_precision (float) in the reference (:1169, :1236-1238) */
 	for(r = 0; r < 3; r++)
 		t[r] = (float)((F[3 * n + r] - Ad[r]) * o->BI[lev][3 * n + r]);
@@ -945,7 +1040,8 @@ static void node_update(orc *o, int lev, int n, double *d0, const double *F)
  * gives the same iterate up to the float rounding of the reference's
  * scattered Ad increments.
  * smoother 1: the same node-block update in 8 colour classes of the node
- * index parities (k%2, i%2, j%2), colour c = (k&1) | (i&1)<<1 | (j&1)<<2.
+ * index parities (k%2, i%2, j%2), colour c = (k&1) | (i&1)<<1 | (j&1)<<2,
+ * with the row sums of row_apply_tree.
  * Both return Ad = K d0 (BC rows stripped). */
 void orc_gauss_seidel(orc *o, int lev, double *d0, const double *F, double *Ad, int cycles, int guess)
 {
@@ -1121,6 +1217,16 @@ static double pdot(const orc *o, int lev, const double *A, const double *B)
 /* ------------------------------------------------------------------------ */
 /* multigrid and Uzawa                                                       */
 
+/* the coarsest-level smoother call of multi_grid (AU[levmin] is not read
+ * afterwards) */
+static void coarse_smooth(orc *o, double *vel, const double *rhs, double *AU)
+{
+	if(o->p.smoother == 1)
+		orc_coarse_solve(o, rhs, vel);
+	else
+		orc_gauss_seidel(o, o->levmin, vel, rhs, AU, o->p.v_steps_low, 0);
+}
+
 /* multi_grid, General_matrix_functions.c:525-653 */
 double orc_multi_grid(orc *o, double *d1, double *F, double *Au, double acc, int hl)
 {
@@ -1136,7 +1242,7 @@ double orc_multi_grid(orc *o, double *d1, double *F, double *Au, double acc, int
 		orc_project(o, lev, fl[lev], fl[lev - 1]);
 		orc_strip(o, lev - 1, fl[lev - 1]);
 	}
-	orc_gauss_seidel(o, levmin, vel[levmin], fl[levmin], AU[levmin], o->p.v_steps_low, 0);
+	coarse_smooth(o, vel[levmin], fl[levmin], AU[levmin]);
 	for(lev = levmin + 1; lev <= levmax; lev++)
 	{
 		orc_interp(o, lev, vel[lev - 1], vel[lev]);
@@ -1154,7 +1260,7 @@ double orc_multi_grid(orc *o, double *d1, double *F, double *Au, double acc, int
 				orc_project(o, dlev, res[dlev], rhs[dlev - 1]);
 				orc_strip(o, dlev - 1, rhs[dlev - 1]);
 			}
-			orc_gauss_seidel(o, levmin, vel[levmin], rhs[levmin], AU[levmin], o->p.v_steps_low, 0);
+			coarse_smooth(o, vel[levmin], rhs[levmin], AU[levmin]);
 			for(ulev = levmin + 1; ulev <= lev; ulev++)
 			{
 				const int cycles = (ulev == levmax) ? o->p.v_steps_high : o->p.up_heavy;

@@ -60,7 +60,8 @@ typedef struct orc_params
 	float gamma;
 	double Q0;
 	float fixed_timestep;
-	/* smoother: 0 = lexicographic node-block GS (reference), 1 = 8-colour */
+	/* smoother: 0 = lexicographic node-block GS (reference), 1 = 8-colour
+	 * (the coarsest level of multi_grid applied as a matrix, orc_coarse_solve) */
 	int smoother;
 } orc_params;
 
@@ -77,6 +78,8 @@ void orc_set_viscosity(orc *o, const float *evi_finest);	/* EVI[levmax], then pr
 void orc_build_operators(orc *o);	/* project_viscosity + K + BI + EG + BPI on every level */
 void orc_matvec(orc *o, int lev, const double *u, double *Au, int strip);
 void orc_gauss_seidel(orc *o, int lev, double *d0, const double *F, double *Ad, int cycles, int guess);
+/* smoother 1: the coarsest-level smoother of multi_grid as a matrix, d = Mc F */
+void orc_coarse_solve(orc *o, const double *F, double *d);
 void orc_project(orc *o, int lev, const double *fine, double *coarse);
 void orc_interp(orc *o, int lev, const double *coarse, double *fine);
 void orc_div(orc *o, int lev, const double *U, double *divU);

@@ -80,6 +80,7 @@ def lib():
         L.orc_build_operators.argtypes = [P]
         L.orc_matvec.argtypes = [P, C.c_int, dp, dp, C.c_int]
         L.orc_gauss_seidel.argtypes = [P, C.c_int, dp, dp, dp, C.c_int, C.c_int]
+        L.orc_coarse_solve.argtypes = [P, dp, dp]
         L.orc_project.argtypes = [P, C.c_int, dp, dp]
         L.orc_interp.argtypes = [P, C.c_int, dp, dp]
         L.orc_div.argtypes = [P, C.c_int, dp, dp]
@@ -155,6 +156,12 @@ class Oracle:
         Ad = np.zeros_like(d)
         lib().orc_gauss_seidel(self.h, lev, d, np.ascontiguousarray(F), Ad, cycles, int(guess))
         return d, Ad
+
+    def coarse_solve(self, F):
+        """smoother 1: the coarsest-level smoother of multi_grid as a matrix"""
+        d = np.zeros(3 * self.nno(0))
+        lib().orc_coarse_solve(self.h, np.ascontiguousarray(F, dtype=np.float64), d)
+        return d
 
     def project(self, lev, fine):
         out = np.zeros(3 * self.nno(lev - 1))

@@ -14,6 +14,10 @@
  *   ref_harness <deck> <outdir> run <nsteps>
  *       the time loop of Citcom.c:97-161 for nsteps steps; per-step scalars
  *       to steps.txt and final T, Tdot, U, P, V, Nu to outdir.
+ *   ref_harness <deck> <outdir> time
+ *       a bounded timing sample of the reference on the deck (CPU baseline):
+ *       the operator build of one step, one multi_grid call and one
+ *       PG_timestep, printed as "build <s> mg <s> pg <s>".
  *
  * Every array is written raw (native endianness) to <outdir>/<name>; the
  * 1-based reference arrays are written from index 1 unless noted.
@@ -279,6 +283,32 @@ static void run_steps(struct All_variables *E, int nsteps)
 	dump_fields(E, "final");
 }
 
+/* the reference's own routines, timed with its own clock (CPU_time0,
+ * MPI_Wtime): build_operators as general_stokes_solver does it
+ * (Drive_solvers.c:105-126), multi_grid (General_matrix_functions.c:525)
+ * on the assembled forces, PG_timestep (Advection_diffusion.c:251) */
+static void time_sample(struct All_variables *E)
+{
+	const int neq = E->lmesh.neq;
+	double *d = calloc(neq + 2, sizeof(double)), *F = calloc(neq + 2, sizeof(double));
+	double *Au = calloc(neq + 2, sizeof(double));
+	double t0, t_build, t_mg, t_pg;
+	t0 = CPU_time0();
+	build_operators(E);
+	t_build = CPU_time0() - t0;
+	memcpy(F, E->F, neq * sizeof(double));
+	t0 = CPU_time0();
+	multi_grid(E, d, F, Au, E->control.accuracy, E->mesh.levmax);
+	t_mg = CPU_time0() - t0;
+	t0 = CPU_time0();
+	PG_timestep(E);
+	t_pg = CPU_time0() - t0;
+	printf("build %.6f mg %.6f pg %.6f\n", t_build, t_mg, t_pg);
+	free(d);
+	free(F);
+	free(Au);
+}
+
 int main(int argc, char **argv)
 {
 	struct All_variables E;
@@ -286,7 +316,7 @@ int main(int argc, char **argv)
 
 	if(argc < 4)
 	{
-		fprintf(stderr, "usage: %s deck outdir setup|ops <seed>|run <nsteps>\n", argv[0]);
+		fprintf(stderr, "usage: %s deck outdir setup|ops <seed>|run <nsteps>|time\n", argv[0]);
 		return 2;
 	}
 	g_out = argv[2];
@@ -321,6 +351,10 @@ int main(int argc, char **argv)
 	else if(strcmp(mode, "run") == 0)
 	{
 		run_steps(&E, argc > 4 ? atoi(argv[4]) : 1);
+	}
+	else if(strcmp(mode, "time") == 0)
+	{
+		time_sample(&E);
 	}
 	else
 	{

@@ -200,3 +200,40 @@ def test_graph_replay_matches_eager():
     say("graph replays per run:", out[0][3])
     for k in range(3):
         assert np.array_equal(out[0][k], out[1][k]) and np.array_equal(out[2][k], out[1][k])
+
+
+@pytest.mark.parametrize("name", ["tiny_tdep", "cfg1"])
+@pytest.mark.parametrize("small", [1, 0])
+def test_smoother_kernels_bitwise(name, small, pyoracle):
+    """The three smoother implementations (whole level in LDS, one workgroup
+    with the stiffness in L2, one launch per colour pass; cfg1's levels use
+    one each) give the restatement's 8-colour iterate bit for bit, and so do
+    per-colour launches on every level (small_levels = 0)."""
+    from citcom_amd import deck
+    from citcom_amd.hip import Citcom
+    p = problem(name)
+    c = Citcom(p)
+    c.set_option("small_levels", small)
+    o = pyoracle.Oracle(p, smoother=1)
+    T = deck.perturbed_temperature(p)
+    c.initialize(T)
+    c.get_system_viscosity()
+    c.construct_stiffness_B_matrix()
+    o.set_viscosity(o.visc_from_T(T))
+    inp = refdump.harness_op_inputs(p, 11, o.strip)
+    for lev in range(p.levels):
+        x = inp[lev]
+        d, Ad = c.apply("gs0", lev, None, x["F"], sweeps=3)
+        do, Ado = o.gauss_seidel(lev, np.zeros_like(x["F"]), x["F"], 3, 0)
+        assert np.array_equal(d, do) and np.array_equal(Ad, Ado), (lev, "gs0")
+        d, Ad = c.apply("gs1", lev, x["d0"], x["F"], sweeps=2)
+        do, Ado = o.gauss_seidel(lev, x["d0"], x["F"], 2, 1)
+        assert np.array_equal(d, do) and np.array_equal(Ad, Ado), (lev, "gs1")
+    if small:
+        # the coarsest level's smoother call of multi_grid, as a matrix
+        F = inp[0]["F"]
+        assert np.array_equal(c.apply("coarse", 0, None, F), o.coarse_solve(F))
+        d, _ = o.gauss_seidel(0, np.zeros_like(F), F, p.v_steps_low, 0)
+        assert rel_l2(c.apply("coarse", 0, None, F), d) < 1e-6
+    say(name, "small_levels", small, "smoother bitwise on", p.levels, "levels")
+    c.close()
