@@ -91,29 +91,49 @@ int prof_flush(chip_ctx *c) {
 
 // algorithmic bytes: 243 FP32 coefficients per node plus the vectors
 // (matvec: read u, write Au; colour pass: read/write d, read F and BI)
-constexpr double MV_BYTES = 243.0 * 4 + 6 * 8;
-constexpr double GS_BYTES = 243.0 * 4 + 9 * 8 + 3 * 8;
+template <typename V>
+constexpr double mv_bytes() { return 243.0 * 4 + 6 * sizeof(V); }
+template <typename V>
+constexpr double gs_bytes() { return 243.0 * 4 + 12 * sizeof(V); }
 
-void do_matvec(chip_ctx *c, int lev, const double *u, double *Au, const double *rhs = nullptr, double *res = nullptr) {
+// the multigrid work vectors of a level in one precision
+template <typename V>
+struct LV {
+    V *vel, *res, *fl, *del, *AU;
+    const V *BI;
+};
+template <typename V>
+LV<V> vecs(const Level &L);
+template <>
+LV<double> vecs<double>(const Level &L) { return {L.vel, L.res, L.fl, L.del, L.AU, L.BI}; }
+template <>
+LV<float> vecs<float>(const Level &L) { return {L.velf, L.resf, L.flf, L.delf, L.AUf, L.BIf}; }
+
+template <typename V>
+void do_matvec(chip_ctx *c, int lev, const V *u, V *Au, const V *rhs = nullptr, V *res = nullptr) {
     const Level &L = c->L[lev];
-    ProfGuard pg(c, 2, MV_BYTES * L.g.nno);
+    ProfGuard pg(c, 2, mv_bytes<V>() * L.g.nno);
     matvec(S(c), L, u, Au, rhs, res, pg.a, pg.b);
 }
 
 // gauss_seidel (General_matrix_functions.c:1160-1361), 8-colour; returns Ad = K d
-void do_gs(chip_ctx *c, int lev, double *d, const double *F, double *Ad, int sweeps, int guess) {
+template <typename V>
+void do_gs(chip_ctx *c, int lev, V *d, const V *F, V *Ad, int sweeps, int guess) {
     const Level &L = c->L[lev];
-    if (c->small_levels && gs_small_kind(L)) {
-        gs_small(S(c), L, d, F, Ad, sweeps, guess);
-        return;
+    if constexpr (sizeof(V) == sizeof(double)) {
+        if (c->small_levels && gs_small_kind(L)) {
+            gs_small(S(c), L, d, F, Ad, sweeps, guess);
+            return;
+        }
     }
-    if (!guess) fill(S(c), d, 0.0, 3L * L.g.nno);
+    if (!guess) fill(S(c), d, V(0), 3L * L.g.nno);
+    const V *BI = vecs<V>(L).BI;
     for (int cyc = 0; cyc < sweeps; cyc++)
         for (int col = 0; col < 8; col++) {
-            ProfGuard pg(c, 1, GS_BYTES * L.g.cn[col]);
-            gs_colour(S(c), L, col, d, F, pg.a, pg.b);
+            ProfGuard pg(c, 1, gs_bytes<V>() * L.g.cn[col]);
+            gs_colour(S(c), L, BI, col, d, F, pg.a, pg.b);
         }
-    do_matvec(c, lev, d, Ad);
+    do_matvec(c, lev, (const V *)d, Ad);
 }
 
 // multi_grid (General_matrix_functions.c:525-653).  F holds the right-hand
@@ -123,47 +143,73 @@ void do_gs(chip_ctx *c, int lev, double *d, const double *F, double *Ad, int swe
 // This enqueues the whole fixed kernel sequence; |F|^2 lands in dots[D_TMP].
 // the coarsest-level smoother call of multi_grid (AU[levmin] is not read
 // afterwards): the coarse matrix where there is one
-void coarse_smooth(chip_ctx *c, double *vel, const double *rhs) {
+template <typename V>
+void coarse_smooth(chip_ctx *c, V *vel, const V *rhs) {
     const Level &L = c->L[c->levmin];
     if (L.Mc && c->small_levels)
         coarse_apply(S(c), L, rhs, vel);
     else
-        do_gs(c, c->levmin, vel, rhs, L.AU, c->p.v_steps_low, 0);
+        do_gs(c, c->levmin, vel, rhs, vecs<V>(L).AU, c->p.v_steps_low, 0);
 }
 
-void enqueue_multi_grid(chip_ctx *c, double *F) {
+// the V-cycles of multi_grid in one precision; F is the finest right-hand
+// side, the correction is left in the finest vel and the finest residual in
+// its res
+template <typename V>
+void enqueue_cycles(chip_ctx *c, V *F) {
     const int levmin = c->levmin, levmax = c->levmax;
     const chip_problem &p = c->p;
     Level *L = c->L;
-    auto fl = [&](int lev) { return lev == levmax ? F : L[lev].fl; };
-    for (int lev = levmax; lev > levmin; lev--) project(S(c), L[lev], L[lev - 1], fl(lev), fl(lev - 1));
-    coarse_smooth(c, L[levmin].vel, fl(levmin));
+    auto fl = [&](int lev) { return lev == levmax ? F : vecs<V>(L[lev]).fl; };
+    for (int lev = levmax; lev > levmin; lev--) project(S(c), L[lev], L[lev - 1], (const V *)fl(lev), fl(lev - 1));
+    coarse_smooth(c, vecs<V>(L[levmin]).vel, (const V *)fl(levmin));
     for (int lev = levmin + 1; lev <= levmax; lev++) {
-        interp(S(c), L[lev - 1], L[lev], L[lev - 1].vel, L[lev].vel);
+        interp(S(c), L[lev - 1], L[lev], (const V *)vecs<V>(L[lev - 1]).vel, vecs<V>(L[lev]).vel);
         for (int Vn = 1; Vn <= p.mg_cycle; Vn++) {
             for (int dlev = lev; dlev >= levmin + 1; dlev--) {
+                const LV<V> v = vecs<V>(L[dlev]);
                 const int cycles = (dlev == levmax) ? p.v_steps_high : p.down_heavy;
-                double *rhs = (dlev == lev) ? fl(dlev) : L[dlev].fl;
-                do_gs(c, dlev, L[dlev].vel, rhs, L[dlev].AU, cycles, dlev == lev);
-                sub(S(c), L[dlev].res, rhs, L[dlev].AU, 3L * L[dlev].g.nno);
-                project(S(c), L[dlev], L[dlev - 1], L[dlev].res, L[dlev - 1].fl);
+                V *rhs = (dlev == lev) ? fl(dlev) : v.fl;
+                do_gs(c, dlev, v.vel, (const V *)rhs, v.AU, cycles, dlev == lev);
+                sub(S(c), v.res, rhs, v.AU, 3L * L[dlev].g.nno);
+                project(S(c), L[dlev], L[dlev - 1], (const V *)v.res, vecs<V>(L[dlev - 1]).fl);
             }
-            coarse_smooth(c, L[levmin].vel, L[levmin].fl);
+            coarse_smooth(c, vecs<V>(L[levmin]).vel, (const V *)vecs<V>(L[levmin]).fl);
             for (int ulev = levmin + 1; ulev <= lev; ulev++) {
+                const LV<V> v = vecs<V>(L[ulev]);
                 const int cycles = (ulev == levmax) ? p.v_steps_high : p.up_heavy;
                 const long n = 3L * L[ulev].g.nno;
-                interp(S(c), L[ulev - 1], L[ulev], L[ulev - 1].vel, L[ulev].del);
-                do_gs(c, ulev, L[ulev].del, L[ulev].res, L[ulev].AU, cycles, 1);
-                dots(S(c), c, (int)n, 2, L[ulev].AU, L[ulev].AU, L[ulev].AU, L[ulev].res, nullptr, nullptr, nullptr,
+                interp(S(c), L[ulev - 1], L[ulev], (const V *)vecs<V>(L[ulev - 1]).vel, v.del);
+                do_gs(c, ulev, v.del, (const V *)v.res, v.AU, cycles, 1);
+                dots(S(c), c, (int)n, 2, (const V *)v.AU, (const V *)v.AU, v.AU, v.res, nullptr, nullptr, nullptr,
                      nullptr, c->dots + D_MG);
-                mg_update(S(c), L[ulev].vel, L[ulev].del, ulev == levmax ? L[ulev].res : nullptr, L[ulev].AU,
-                          c->dots + D_MG, n);
+                mg_update(S(c), v.vel, v.del, ulev == levmax ? v.res : nullptr, v.AU, c->dots + D_MG, n);
             }
         }
     }
-    const long neq = 3L * L[levmax].g.nno;
-    copy(S(c), F, L[levmax].res, neq);
-    dots(S(c), c, (int)neq, 1, F, F, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, c->dots + D_TMP);
+}
+
+// one multi_grid call on F (= c->sr, FP64): F holds the right-hand side on
+// entry and the residual on exit; the correction is left in L[levmax].vel
+// (FP64).  Plain form: the reference's FP64 cycles, residual from the line
+// search.  Mixed form (FP32 multigrid inside FP64 refinement): the cycles run
+// on an FP32 copy of F, the FP32 correction is added to the FP64 solution
+// (u1, so vel is not used) and the residual F = Ah - K u1 is formed in FP64;
+// solve_del2_u always solves K u1 = Ah.  |F|^2 lands in dots[D_TMP].
+void enqueue_multi_grid(chip_ctx *c, double *F) {
+    const Level &L = c->L[c->levmax];
+    const long neq = 3L * L.g.nno;
+    if (!c->p.mixed) {
+        enqueue_cycles<double>(c, F);
+        copy(S(c), F, L.res, neq);
+    } else {
+        to_f32(S(c), L.flf, F, neq);
+        enqueue_cycles<float>(c, L.flf);
+        add_f32(S(c), c->u1, L.velf, neq);
+        do_matvec<double>(c, c->levmax, (const double *)c->u1, c->sd, (const double *)c->Ah, F);
+    }
+    dots(S(c), c, (int)neq, 1, (const double *)F, (const double *)F, nullptr, nullptr, nullptr, nullptr, nullptr,
+         nullptr, c->dots + D_TMP);
 }
 
 // F is always c->sr here, so one captured graph serves every call
@@ -189,6 +235,10 @@ int multi_grid(chip_ctx *c, double *F, double *residual) {
 // solve_del2_u (General_matrix_functions.c:368-520, multigrid branch):
 // d0 = K^-1 F; F is not modified
 int solve_del2_u(chip_ctx *c, double *d0, const double *F, double acc, int *valid, int *count) {
+    if (c->p.mixed && (d0 != c->u1 || F != c->Ah)) {
+        set_error("solve_del2_u: the mixed form solves K u1 = Ah only");
+        return -1;
+    }
     const Level &L = c->L[c->levmax];
     const long neq = 3L * L.g.nno;
     copy(S(c), c->sr, F, neq);
@@ -202,7 +252,7 @@ int solve_del2_u(chip_ctx *c, double *d0, const double *F, double acc, int *vali
     int cnt = 0;
     while (residual > acc) {
         if (multi_grid(c, c->sr, &residual)) return -1;
-        axpy_scalar(S(c), d0, c->L[c->levmax].vel, 1.0, neq);
+        if (!c->p.mixed) axpy_scalar(S(c), d0, c->L[c->levmax].vel, 1.0, neq);
         cnt++;
         if (cnt > 10000) {
             set_error("solve_del2_u: multigrid does not converge");
@@ -295,6 +345,8 @@ int rebuild(chip_ctx *c) {
     for (int lev = c->levmax; lev >= c->levmin; lev--) node_ks(q, c->L[lev], c->p);
     for (int lev = c->levmax; lev >= c->levmin; lev--) build_bpi(q, c->L[lev], c->p.precond);
     if (c->L[c->levmin].Mc) coarse_build(q, c->L[c->levmin], c->p.v_steps_low);
+    if (c->p.mixed)
+        for (int lev = c->levmin; lev <= c->levmax; lev++) to_f32(q, c->L[lev].BIf, c->L[lev].BI, 3L * c->L[lev].g.nno);
     CHIP_CHECK(hipGetLastError());
     return 0;
 }

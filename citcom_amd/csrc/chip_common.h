@@ -64,6 +64,7 @@ struct Level {
     double *Mc = nullptr;
     // FP32 copies for the mixed-precision cycle
     float *velf = nullptr, *resf = nullptr, *flf = nullptr, *delf = nullptr, *AUf = nullptr;
+    float *BIf = nullptr;
 };
 
 struct Prof {

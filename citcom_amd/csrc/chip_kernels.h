@@ -20,12 +20,16 @@ void node_ks(hipStream_t s, const Level &L, const chip_problem &p);
 void build_bpi(hipStream_t s, const Level &L, int precond);
 
 // ---- vector operators ---------------------------------------------------
+// The multigrid operators take FP64 (plain form) or FP32 (mixed form)
+// vectors; the stiffness is FP32 in both.
 // Au = K u (strip BC rows); optionally res = rhs - Au as well.  With events
 // e0/e1 the launch records the kernel's own start and end time in them.
-void matvec(hipStream_t s, const Level &L, const double *u, double *Au, const double *rhs, double *res,
-            hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
-// one colour pass of the node-block Gauss-Seidel
-void gs_colour(hipStream_t s, const Level &L, int colour, double *d, const double *F, hipEvent_t e0 = nullptr,
+template <typename V>
+void matvec(hipStream_t s, const Level &L, const V *u, V *Au, const V *rhs, V *res, hipEvent_t e0 = nullptr,
+            hipEvent_t e1 = nullptr);
+// one colour pass of the node-block Gauss-Seidel (BI in the vector type)
+template <typename V>
+void gs_colour(hipStream_t s, const Level &L, const V *BI, int colour, V *d, const V *F, hipEvent_t e0 = nullptr,
                hipEvent_t e1 = nullptr);
 // every sweep of a small level in one workgroup, then Ad = K d
 void gs_small(hipStream_t s, const Level &L, double *d, const double *F, double *Ad, int sweeps, int guess);
@@ -34,24 +38,39 @@ void gs_small(hipStream_t s, const Level &L, double *d, const double *F, double 
 int gs_small_kind(const Level &L);
 // coarsest level: Mc = the v_steps_low-sweep smoother as a matrix, d = Mc F
 void coarse_build(hipStream_t s, const Level &L, int sweeps);
-void coarse_apply(hipStream_t s, const Level &L, const double *F, double *d);
-void project(hipStream_t s, const Level &fine, const Level &coarse, const double *AU, double *AD, bool do_strip = true);
-void interp(hipStream_t s, const Level &coarse, const Level &fine, const double *AD, double *AU, bool do_strip = true);
+template <typename V>
+void coarse_apply(hipStream_t s, const Level &L, const V *F, V *d);
+template <typename V>
+void project(hipStream_t s, const Level &fine, const Level &coarse, const V *AU, V *AD, bool do_strip = true);
+template <typename V>
+void interp(hipStream_t s, const Level &coarse, const Level &fine, const V *AD, V *AU, bool do_strip = true);
 void div(hipStream_t s, const Level &L, const double *U, double *divU);
 void grad(hipStream_t s, const Level &L, const double *P, double *gradP);
 void strip(hipStream_t s, const Level &L, double *v);
 
-// ---- reductions: out[m] = sum a_m . b_m over n entries (m < nd <= 4) -----
-void dots(hipStream_t s, chip_ctx *c, int n, int nd, const double *a0, const double *b0, const double *a1,
-          const double *b1, const double *a2, const double *b2, const double *a3, const double *b3, double *out);
+// ---- reductions: out[m] = sum a_m . b_m over n entries (m < nd <= 4), FP64
+// (only a0, b0 fix the vector type; the others may be nullptr)
+template <typename T>
+struct same_t {
+    using type = T;
+};
+template <typename V>
+void dots(hipStream_t s, chip_ctx *c, int n, int nd, const V *a0, const V *b0, typename same_t<const V *>::type a1,
+          typename same_t<const V *>::type b1, typename same_t<const V *>::type a2, typename same_t<const V *>::type b2,
+          typename same_t<const V *>::type a3, typename same_t<const V *>::type b3, double *out);
 
 // ---- elementwise ------------------------------------------------------------
 void fill(hipStream_t s, double *v, double val, long n);
+void fill(hipStream_t s, float *v, float val, long n);
+void to_f32(hipStream_t s, float *dst, const double *src, long n);
+void add_f32(hipStream_t s, double *y, const float *x, long n);   // y += x
 void copy(hipStream_t s, double *dst, const double *src, long n);
 void axpy_scalar(hipStream_t s, double *y, const double *x, double a, long n);   // y += a x
 void sub(hipStream_t s, double *r, const double *a, const double *b, long n);    // r = a - b
+void sub(hipStream_t s, float *r, const float *a, const float *b, long n);
 // vel += alpha del, alpha = sc[1]/sc[0]; if res: res -= alpha AU
 void mg_update(hipStream_t s, double *vel, const double *del, double *res, const double *AU, const double *sc, long n);
+void mg_update(hipStream_t s, float *vel, const float *del, float *res, const float *AU, const double *sc, long n);
 // Uzawa pieces (Stokes_flow_Incomp.c:393-464)
 void uz_z(hipStream_t s, double *z1, const double *BPI, const double *r1, long n);
 void uz_search(hipStream_t s, double *s2, const double *z1, const double *s1, const double *num, const double *den,

@@ -121,15 +121,15 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 // a time so that each thread keeps 81 coefficients and 27 vector entries in
 // flight.  Missing neighbours carry zero coefficients and read the node's
 // own entries instead.
-template <typename KA, typename UA>
-__device__ __forceinline__ void row_product_t(const Geo &g, KA kat, int k, int i, int j, UA u, double out[3]) {
+template <typename A, typename KA, typename UA>
+__device__ __forceinline__ void row_product_t(const Geo &g, KA kat, int k, int i, int j, UA u, A out[3]) {
     const int self = nid(g, k, i, j);
-    out[0] = out[1] = out[2] = 0.0;
+    out[0] = out[1] = out[2] = A(0);
 #pragma unroll
     for (int dy = -1; dy <= 1; dy++) {
         const bool oy = (unsigned)(j + dy) < (unsigned)g.noy;
         float kv[81];
-        double uv[27];
+        A uv[27];
 #pragma unroll
         for (int dx = -1; dx <= 1; dx++) {
             const bool ox = oy && (unsigned)(i + dx) < (unsigned)g.nox;
@@ -155,17 +155,19 @@ __device__ __forceinline__ void row_product_t(const Geo &g, KA kat, int k, int i
     }
 }
 
+template <typename V>
 __device__ __forceinline__ void row_product(const Geo &g, const float *__restrict__ K, int q, int k, int i, int j,
-                                            const double *__restrict__ u, double out[3]) {
+                                            const V *__restrict__ u, V out[3]) {
     const float *kp = K + ktile(q);
-    row_product_t(
+    row_product_t<V>(
         g, [kp](int ci) { return kp[ci * 64]; }, k, i, j, [u](int idx) { return u[idx]; }, out);
 }
 
-__device__ __forceinline__ void strip3(unsigned f, double v[3]) {
-    if (f & VBX) v[0] = 0.0;
-    if (f & VBY) v[1] = 0.0;
-    if (f & VBZ) v[2] = 0.0;
+template <typename V>
+__device__ __forceinline__ void strip3(unsigned f, V v[3]) {
+    if (f & VBX) v[0] = V(0);
+    if (f & VBY) v[1] = V(0);
+    if (f & VBZ) v[2] = V(0);
 }
 
 // ---------------------------------------------------------------------------
@@ -180,11 +182,12 @@ __device__ __forceinline__ void strip3(unsigned f, double v[3]) {
 
 // tree over 16 slots formed term by term: push<B>(t[B]) for B = 0..15 keeps
 // one partial sum per tree level; after B = 15, total is t[0] of the tree
+template <typename V>
 struct PairSum16 {
-    double st[4];
-    double total;
+    V st[4];
+    V total;
     template <int B>
-    __device__ __forceinline__ void push(double v) {
+    __device__ __forceinline__ void push(V v) {
 #pragma unroll
         for (int l = 0; l < 4; l++) {
             if (B & (1 << l)) {
@@ -223,14 +226,14 @@ __device__ __forceinline__ double lane_value(double v, int lane) {
 }
 
 // Plane P = dy+1 of the three rows of node (k, i, j), coefficients kp (its
-// tile column), into P[r]
-template <int P>
+// tile column), into P[r]; arithmetic in the vector type V
+template <int P, typename V>
 __device__ __forceinline__ void plane_rows(const Geo &g, const float *__restrict__ kp, int k, int i, int j, int self,
-                                           const double *__restrict__ u, double out[3]) {
+                                           const V *__restrict__ u, V out[3]) {
     constexpr int DY = P - 1;
     const bool oy = (unsigned)(j + DY) < (unsigned)g.noy;
     float kv[81];
-    double uv[27];
+    V uv[27];
     bool okb[9];
 #pragma unroll
     for (int dx = -1; dx <= 1; dx++) {
@@ -248,28 +251,28 @@ __device__ __forceinline__ void plane_rows(const Geo &g, const float *__restrict
     }
 #pragma unroll
     for (int c = 0; c < 81; c++) kv[c] = kp[(81 * P + c) * 64];
-    PairSum16 ps[3];
+    PairSum16<V> ps[3];
 #define CHIP_TERM(B, R)                                                                                            \
     (okb[B] ? kv[9 * (B) + 3 * (R)] * uv[3 * (B)] + kv[9 * (B) + 3 * (R) + 1] * uv[3 * (B) + 1] +                  \
                   kv[9 * (B) + 3 * (R) + 2] * uv[3 * (B) + 2]                                                      \
-            : 0.0)
-#define CHIP_PUSH(B)                    \
-    ps[0].push<B>(CHIP_TERM(B, 0));     \
-    ps[1].push<B>(CHIP_TERM(B, 1));     \
-    ps[2].push<B>(CHIP_TERM(B, 2))
+            : V(0))
+#define CHIP_PUSH(B)                                  \
+    ps[0].template push<B>(CHIP_TERM(B, 0));          \
+    ps[1].template push<B>(CHIP_TERM(B, 1));          \
+    ps[2].template push<B>(CHIP_TERM(B, 2))
     CHIP_PUSH(0); CHIP_PUSH(1); CHIP_PUSH(2); CHIP_PUSH(3); CHIP_PUSH(4);
     CHIP_PUSH(5); CHIP_PUSH(6); CHIP_PUSH(7); CHIP_PUSH(8);
 #undef CHIP_PUSH
 #undef CHIP_TERM
 #pragma unroll
     for (int r = 0; r < 3; r++) {
-        ps[r].push<9>(0.0);
-        ps[r].push<10>(0.0);
-        ps[r].push<11>(0.0);
-        ps[r].push<12>(0.0);
-        ps[r].push<13>(0.0);
-        ps[r].push<14>(0.0);
-        ps[r].push<15>(0.0);
+        ps[r].template push<9>(V(0));
+        ps[r].template push<10>(V(0));
+        ps[r].template push<11>(V(0));
+        ps[r].template push<12>(V(0));
+        ps[r].template push<13>(V(0));
+        ps[r].template push<14>(V(0));
+        ps[r].template push<15>(V(0));
         out[r] = ps[r].total;
     }
 }
@@ -277,15 +280,16 @@ __device__ __forceinline__ void plane_rows(const Geo &g, const float *__restrict
 // matvec: n_assemble_del2_u (Element_calculations.c:552-621) with strip;
 // optionally res = rhs - Au.  One thread per node: the three rows share the
 // 27 vector gathers.
+template <typename V>
 __global__ void __launch_bounds__(256) k_matvec(Geo g, const float *__restrict__ K, const unsigned *__restrict__ NODE,
-                                                const double *__restrict__ u, double *__restrict__ Au,
-                                                const double *__restrict__ rhs, double *__restrict__ res) {
+                                                const V *__restrict__ u, V *__restrict__ Au,
+                                                const V *__restrict__ rhs, V *__restrict__ res) {
     const int q = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int col = colour_valid(g, q);
     if (col < 0) return;
     int k, i, j;
     colour_kij(g, col, q, k, i, j);
-    double out[3];
+    V out[3];
     row_product(g, K, q, k, i, j, u, out);
     const int n = nid(g, k, i, j);
     strip3(NODE[n], out);
@@ -299,14 +303,18 @@ __global__ void __launch_bounds__(256) k_matvec(Geo g, const float *__restrict__
     }
 }
 
-void matvec(hipStream_t s, const Level &L, const double *u, double *Au, const double *rhs, double *res, hipEvent_t e0,
-            hipEvent_t e1) {
+template <typename V>
+void matvec(hipStream_t s, const Level &L, const V *u, V *Au, const V *rhs, V *res, hipEvent_t e0, hipEvent_t e1) {
     if (e0)
-        hipExtLaunchKernelGGL(k_matvec, dim3(nblk(L.g.pitch)), dim3(256), 0, s, e0, e1, 0, L.g, L.K, L.NODE, u, Au, rhs,
-                              res);
+        hipExtLaunchKernelGGL(k_matvec<V>, dim3(nblk(L.g.pitch)), dim3(256), 0, s, e0, e1, 0, L.g, L.K, L.NODE, u, Au,
+                              rhs, res);
     else
-        hipLaunchKernelGGL(k_matvec, dim3(nblk(L.g.pitch)), dim3(256), 0, s, L.g, L.K, L.NODE, u, Au, rhs, res);
+        hipLaunchKernelGGL(k_matvec<V>, dim3(nblk(L.g.pitch)), dim3(256), 0, s, L.g, L.K, L.NODE, u, Au, rhs, res);
 }
+template void matvec<double>(hipStream_t, const Level &, const double *, double *, const double *, double *,
+                             hipEvent_t, hipEvent_t);
+template void matvec<float>(hipStream_t, const Level &, const float *, float *, const float *, float *, hipEvent_t,
+                            hipEvent_t);
 
 // Gauss-Seidel: gauss_seidel (General_matrix_functions.c:1160-1361) as eight
 // colour passes; the node-block update t = (F - (K d)_node) * BI, rounded to
@@ -316,10 +324,10 @@ void matvec(hipStream_t s, const Level &L, const double *u, double *Au, const do
 // together), the partial sums meet in LDS, and wavefront r updates row r.
 // Every row is formed before any unknown is written (workgroup barrier), as
 // the reference updates all three unknowns of a node from the same Ad.
+template <typename V>
 __global__ void __launch_bounds__(192) k_gs_colour(Geo g, int c, const float *__restrict__ K,
-                                                   const double *__restrict__ BI, double *d,
-                                                   const double *__restrict__ F) {
-    __shared__ double Ps[3][3][64];   // [plane][row][node]
+                                                   const V *__restrict__ BI, V *d, const V *__restrict__ F) {
+    __shared__ V Ps[3][3][64];   // [plane][row][node]
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int q = g.cs[c] + xcd_tile(blockIdx.x, gridDim.x) * 64 + lane;
     const bool act = q < g.cs[c] + g.cn[c];
@@ -328,21 +336,21 @@ __global__ void __launch_bounds__(192) k_gs_colour(Geo g, int c, const float *__
     const int self = nid(g, k, i, j);
     // row w's F, BI and d, read before the barrier: only this thread
     // writes d[3 self + w] in this pass
-    double Fr = 0.0, Br = 0.0, dr = 0.0;
+    V Fr = V(0), Br = V(0), dr = V(0);
     if (act) {
         Fr = F[3 * self + w];
         Br = BI[3 * self + w];
         dr = d[3 * self + w];
     }
     if (act) {
-        double out[3];
+        V out[3];
         const float *kp = K + ktile(q);
         if (w == 0)
-            plane_rows<0>(g, kp, k, i, j, self, d, out);
+            plane_rows<0>(g, kp, k, i, j, self, (const V *)d, out);
         else if (w == 1)
-            plane_rows<1>(g, kp, k, i, j, self, d, out);
+            plane_rows<1>(g, kp, k, i, j, self, (const V *)d, out);
         else
-            plane_rows<2>(g, kp, k, i, j, self, d, out);
+            plane_rows<2>(g, kp, k, i, j, self, (const V *)d, out);
         Ps[w][0][lane] = out[0];
         Ps[w][1][lane] = out[1];
         Ps[w][2][lane] = out[2];
@@ -350,19 +358,24 @@ __global__ void __launch_bounds__(192) k_gs_colour(Geo g, int c, const float *__
     __syncthreads();
     if (act) {
         const int r = w;
-        const double o = (Ps[0][r][lane] + Ps[1][r][lane]) + Ps[2][r][lane];
+        const V o = (Ps[0][r][lane] + Ps[1][r][lane]) + Ps[2][r][lane];
         const float t = (float)((Fr - o) * Br);
         d[3 * self + r] = dr + t;
     }
 }
 
-void gs_colour(hipStream_t s, const Level &L, int colour, double *d, const double *F, hipEvent_t e0, hipEvent_t e1) {
+template <typename V>
+void gs_colour(hipStream_t s, const Level &L, const V *BI, int colour, V *d, const V *F, hipEvent_t e0, hipEvent_t e1) {
     const int tiles = (L.g.cn[colour] + 63) / 64;
     if (e0)
-        hipExtLaunchKernelGGL(k_gs_colour, dim3(tiles), dim3(192), 0, s, e0, e1, 0, L.g, colour, L.K, L.BI, d, F);
+        hipExtLaunchKernelGGL(k_gs_colour<V>, dim3(tiles), dim3(192), 0, s, e0, e1, 0, L.g, colour, L.K, BI, d, F);
     else
-        hipLaunchKernelGGL(k_gs_colour, dim3(tiles), dim3(192), 0, s, L.g, colour, L.K, L.BI, d, F);
+        hipLaunchKernelGGL(k_gs_colour<V>, dim3(tiles), dim3(192), 0, s, L.g, colour, L.K, BI, d, F);
 }
+template void gs_colour<double>(hipStream_t, const Level &, const double *, int, double *, const double *, hipEvent_t,
+                                hipEvent_t);
+template void gs_colour<float>(hipStream_t, const Level &, const float *, int, float *, const float *, hipEvent_t,
+                               hipEvent_t);
 
 // ---------------------------------------------------------------------------
 // A level whose stiffness fits in LDS (the coarsest, 4x4x2 elements at the
@@ -521,37 +534,42 @@ void coarse_build(hipStream_t s, const Level &L, int sweeps) {
 // d = Mc F: sixteen lanes per row i, lane t summing j = t, t+16, ... in
 // order into S[t]; the S[t] meet in a pairwise tree (S[q] += S[q+w],
 // w = 1, 2, 4, 8) formed by a butterfly over the 16 lanes
-__global__ void __launch_bounds__(256) k_coarse_apply(const double *__restrict__ M, const double *__restrict__ F,
-                                                      double *__restrict__ d, int n3) {
+template <typename V>
+__global__ void __launch_bounds__(256) k_coarse_apply(const double *__restrict__ M, const V *__restrict__ F,
+                                                      V *__restrict__ d, int n3) {
     const int t = threadIdx.x & 15, i = blockIdx.x * 16 + (threadIdx.x >> 4);
     double S = 0.0;
     if (i < n3)
-        for (int j = t; j < n3; j += 16) S = S + M[(size_t)i * n3 + j] * F[j];
+        for (int j = t; j < n3; j += 16) S = S + M[(size_t)i * n3 + j] * (double)F[j];
     S = S + __shfl_xor(S, 1, 64);
     S = S + __shfl_xor(S, 2, 64);
     S = S + __shfl_xor(S, 4, 64);
     S = S + __shfl_xor(S, 8, 64);
-    if (i < n3 && t == 0) d[i] = S;
+    if (i < n3 && t == 0) d[i] = (V)S;
 }
 
-void coarse_apply(hipStream_t s, const Level &L, const double *F, double *d) {
+template <typename V>
+void coarse_apply(hipStream_t s, const Level &L, const V *F, V *d) {
     const int n3 = 3 * L.g.nno;
-    hipLaunchKernelGGL(k_coarse_apply, dim3((n3 + 15) / 16), dim3(256), 0, s, L.Mc, F, d, n3);
+    hipLaunchKernelGGL(k_coarse_apply<V>, dim3((n3 + 15) / 16), dim3(256), 0, s, L.Mc, F, d, n3);
 }
+template void coarse_apply<double>(hipStream_t, const Level &, const double *, double *);
+template void coarse_apply<float>(hipStream_t, const Level &, const float *, float *);
 
 // ---------------------------------------------------------------------------
 // grid transfers
 
 // project_vector (Solver_multigrid.c:72-159), optionally followed by the
 // strip_bcs_from_residual that multi_grid applies to its result
+template <typename V>
 __global__ void __launch_bounds__(TPB) k_project(Geo gf, Geo gc, const float *__restrict__ TWW,
                                                  const float *__restrict__ MASS, const unsigned *__restrict__ NODE,
-                                                 const double *__restrict__ AU, double *__restrict__ AD, int do_strip) {
+                                                 const V *__restrict__ AU, V *__restrict__ AD, int do_strip) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= gc.nno) return;
     int k, i, j;
     node_kij(gc, n, k, i, j);
-    double acc[3] = {0.0, 0.0, 0.0};
+    V acc[3] = {V(0), V(0), V(0)};
     for (int ey = j - 1; ey <= j; ey++) {
         if ((unsigned)ey >= (unsigned)gc.ely) continue;
         for (int ex = i - 1; ex <= i; ex++) {
@@ -561,7 +579,7 @@ __global__ void __launch_bounds__(TPB) k_project(Geo gf, Geo gc, const float *__
                 const int a = cLoc[(k - ez) + 2 * (i - ex) + 4 * (j - ey)];
                 const int el = ez + gc.elz * (ex + gc.elx * ey);
                 const int fz = 2 * ez + cOffz[a], fx = 2 * ex + cOffx[a], fy = 2 * ey + cOffy[a];
-                double av0 = 0.0, av1 = 0.0, av2 = 0.0;
+                V av0 = V(0), av1 = V(0), av2 = V(0);
 #pragma unroll
                 for (int b = 1; b <= 8; b++) {
                     const int fn = nid(gf, fz + cOffz[b], fx + cOffx[b], fy + cOffy[b]);
@@ -586,16 +604,19 @@ __global__ void __launch_bounds__(TPB) k_project(Geo gf, Geo gc, const float *__
     AD[3 * n + 2] = acc[2];
 }
 
-void project(hipStream_t s, const Level &fine, const Level &coarse, const double *AU, double *AD, bool do_strip) {
-    hipLaunchKernelGGL(k_project, dim3(nblk(coarse.g.nno)), dim3(TPB), 0, s, fine.g, coarse.g, coarse.TWW,
+template <typename V>
+void project(hipStream_t s, const Level &fine, const Level &coarse, const V *AU, V *AD, bool do_strip) {
+    hipLaunchKernelGGL(k_project<V>, dim3(nblk(coarse.g.nno)), dim3(TPB), 0, s, fine.g, coarse.g, coarse.TWW,
                        coarse.MASS, coarse.NODE, AU, AD, (int)do_strip);
 }
+template void project<double>(hipStream_t, const Level &, const Level &, const double *, double *, bool);
+template void project<float>(hipStream_t, const Level &, const Level &, const float *, float *, bool);
 
 // interp_vector + un_inject_vector (Solver_multigrid.c:173-298, 581-634):
 // injection, then linear fills along x, z, y in the reference's order,
 // optionally followed by multi_grid's strip_bcs_from_residual
-__device__ __forceinline__ void line_x(const Geo &gc, const float *wx, const double *AD, int k, int i, int j,
-                                       double v[3]) {
+template <typename V>
+__device__ __forceinline__ void line_x(const Geo &gc, const float *wx, const V *AD, int k, int i, int j, V v[3]) {
     // k, j even fine indices; i fine x index
     if (!(i & 1)) {
         const int cn = nid(gc, k >> 1, i >> 1, j >> 1);
@@ -610,12 +631,13 @@ __device__ __forceinline__ void line_x(const Geo &gc, const float *wx, const dou
     }
 }
 
-__device__ __forceinline__ void plane_xz(const Geo &gc, const float *wx, const float *wz, const double *AD, int k, int i,
-                                         int j, double v[3]) {
+template <typename V>
+__device__ __forceinline__ void plane_xz(const Geo &gc, const float *wx, const float *wz, const V *AD, int k, int i,
+                                         int j, V v[3]) {
     if (!(k & 1)) {
         line_x(gc, wx, AD, k, i, j, v);
     } else {
-        double a[3], b[3];
+        V a[3], b[3];
         line_x(gc, wx, AD, k - 1, i, j, a);
         line_x(gc, wx, AD, k + 1, i, j, b);
         const float n1 = wz[2 * k], n2 = wz[2 * k + 1];
@@ -624,19 +646,20 @@ __device__ __forceinline__ void plane_xz(const Geo &gc, const float *wx, const f
     }
 }
 
+template <typename V>
 __global__ void __launch_bounds__(TPB) k_interp(Geo gc, Geo gf, const float *__restrict__ iw,
-                                                const unsigned *__restrict__ NODE, const double *__restrict__ AD,
-                                                double *__restrict__ AU, int do_strip) {
+                                                const unsigned *__restrict__ NODE, const V *__restrict__ AD,
+                                                V *__restrict__ AU, int do_strip) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= gf.nno) return;
     int k, i, j;
     node_kij(gf, n, k, i, j);
     const float *wx = iw, *wz = iw + 2 * gf.nox, *wy = iw + 2 * (gf.nox + gf.noz);
-    double v[3];
+    V v[3];
     if (!(j & 1)) {
         plane_xz(gc, wx, wz, AD, k, i, j, v);
     } else {
-        double a[3], b[3];
+        V a[3], b[3];
         plane_xz(gc, wx, wz, AD, k, i, j - 1, a);
         plane_xz(gc, wx, wz, AD, k, i, j + 1, b);
         const float n1 = wy[2 * j], n2 = wy[2 * j + 1];
@@ -649,10 +672,13 @@ __global__ void __launch_bounds__(TPB) k_interp(Geo gc, Geo gf, const float *__r
     AU[3 * n + 2] = v[2];
 }
 
-void interp(hipStream_t s, const Level &coarse, const Level &fine, const double *AD, double *AU, bool do_strip) {
-    hipLaunchKernelGGL(k_interp, dim3(nblk(fine.g.nno)), dim3(TPB), 0, s, coarse.g, fine.g, fine.iw, fine.NODE, AD,
+template <typename V>
+void interp(hipStream_t s, const Level &coarse, const Level &fine, const V *AD, V *AU, bool do_strip) {
+    hipLaunchKernelGGL(k_interp<V>, dim3(nblk(fine.g.nno)), dim3(TPB), 0, s, coarse.g, fine.g, fine.iw, fine.NODE, AD,
                        AU, (int)do_strip);
 }
+template void interp<double>(hipStream_t, const Level &, const Level &, const double *, double *, bool);
+template void interp<float>(hipStream_t, const Level &, const Level &, const float *, float *, bool);
 
 // ---------------------------------------------------------------------------
 // divergence and gradient (Element_calculations.c:691-769)
@@ -756,11 +782,11 @@ __device__ void block_sum(double v[ND], double *sh /* [ND][DOT_TPB/64] */) {
         }
 }
 
-template <int ND>
-__global__ void __launch_bounds__(DOT_TPB) k_dots(int n, const double *__restrict__ a0, const double *__restrict__ b0,
-                                                  const double *__restrict__ a1, const double *__restrict__ b1,
-                                                  const double *__restrict__ a2, const double *__restrict__ b2,
-                                                  const double *__restrict__ a3, const double *__restrict__ b3,
+template <int ND, typename V>
+__global__ void __launch_bounds__(DOT_TPB) k_dots(int n, const V *__restrict__ a0, const V *__restrict__ b0,
+                                                  const V *__restrict__ a1, const V *__restrict__ b1,
+                                                  const V *__restrict__ a2, const V *__restrict__ b2,
+                                                  const V *__restrict__ a3, const V *__restrict__ b3,
                                                   double *partial, unsigned *counter, double *out) {
     __shared__ double sh[4 * 16];
     __shared__ bool last;
@@ -768,10 +794,10 @@ __global__ void __launch_bounds__(DOT_TPB) k_dots(int n, const double *__restric
 #pragma unroll
     for (int m = 0; m < ND; m++) v[m] = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        v[0] += a0[i] * b0[i];
-        if (ND > 1) v[1] += a1[i] * b1[i];
-        if (ND > 2) v[2] += a2[i] * b2[i];
-        if (ND > 3) v[3] += a3[i] * b3[i];
+        v[0] += (double)a0[i] * (double)b0[i];
+        if (ND > 1) v[1] += (double)a1[i] * (double)b1[i];
+        if (ND > 2) v[2] += (double)a2[i] * (double)b2[i];
+        if (ND > 3) v[3] += (double)a3[i] * (double)b3[i];
     }
     block_sum<ND>(v, sh);
     if (threadIdx.x == 0) {
@@ -798,25 +824,47 @@ __global__ void __launch_bounds__(DOT_TPB) k_dots(int n, const double *__restric
     }
 }
 
-void dots(hipStream_t s, chip_ctx *c, int n, int nd, const double *a0, const double *b0, const double *a1,
-          const double *b1, const double *a2, const double *b2, const double *a3, const double *b3, double *out) {
+template <typename V>
+void dots(hipStream_t s, chip_ctx *c, int n, int nd, const V *a0, const V *b0, typename same_t<const V *>::type a1,
+          typename same_t<const V *>::type b1, typename same_t<const V *>::type a2, typename same_t<const V *>::type b2,
+          typename same_t<const V *>::type a3, typename same_t<const V *>::type b3, double *out) {
     const int nb = std::min(DOT_MAXB, std::max(1, (n + DOT_TPB * 4 - 1) / (DOT_TPB * 4)));
     switch (nd) {
-    case 1: hipLaunchKernelGGL(k_dots<1>, dim3(nb), dim3(DOT_TPB), 0, s, n, a0, b0, a1, b1, a2, b2, a3, b3, c->partial, c->counter, out); break;
-    case 2: hipLaunchKernelGGL(k_dots<2>, dim3(nb), dim3(DOT_TPB), 0, s, n, a0, b0, a1, b1, a2, b2, a3, b3, c->partial, c->counter, out); break;
-    case 3: hipLaunchKernelGGL(k_dots<3>, dim3(nb), dim3(DOT_TPB), 0, s, n, a0, b0, a1, b1, a2, b2, a3, b3, c->partial, c->counter, out); break;
-    default: hipLaunchKernelGGL(k_dots<4>, dim3(nb), dim3(DOT_TPB), 0, s, n, a0, b0, a1, b1, a2, b2, a3, b3, c->partial, c->counter, out); break;
+    case 1: hipLaunchKernelGGL((k_dots<1, V>), dim3(nb), dim3(DOT_TPB), 0, s, n, a0, b0, a1, b1, a2, b2, a3, b3, c->partial, c->counter, out); break;
+    case 2: hipLaunchKernelGGL((k_dots<2, V>), dim3(nb), dim3(DOT_TPB), 0, s, n, a0, b0, a1, b1, a2, b2, a3, b3, c->partial, c->counter, out); break;
+    case 3: hipLaunchKernelGGL((k_dots<3, V>), dim3(nb), dim3(DOT_TPB), 0, s, n, a0, b0, a1, b1, a2, b2, a3, b3, c->partial, c->counter, out); break;
+    default: hipLaunchKernelGGL((k_dots<4, V>), dim3(nb), dim3(DOT_TPB), 0, s, n, a0, b0, a1, b1, a2, b2, a3, b3, c->partial, c->counter, out); break;
     }
 }
+template void dots<double>(hipStream_t, chip_ctx *, int, int, const double *, const double *, const double *,
+                           const double *, const double *, const double *, const double *, const double *, double *);
+template void dots<float>(hipStream_t, chip_ctx *, int, int, const float *, const float *, const float *,
+                          const float *, const float *, const float *, const float *, const float *, double *);
 
 // ---------------------------------------------------------------------------
 // elementwise
 
-__global__ void k_fill(double *v, double val, long n) {
+template <typename V>
+__global__ void k_fill(V *v, V val, long n) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) v[i] = val;
 }
-void fill(hipStream_t s, double *v, double val, long n) { hipLaunchKernelGGL(k_fill, dim3(nblk(n)), dim3(TPB), 0, s, v, val, n); }
+void fill(hipStream_t s, double *v, double val, long n) { hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n)), dim3(TPB), 0, s, v, val, n); }
+void fill(hipStream_t s, float *v, float val, long n) { hipLaunchKernelGGL(k_fill<float>, dim3(nblk(n)), dim3(TPB), 0, s, v, val, n); }
+
+// FP64 -> FP32 (the residual handed to the FP32 multigrid)
+__global__ void k_to_f32(float *d, const double *a, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = (float)a[i];
+}
+void to_f32(hipStream_t s, float *dst, const double *src, long n) { hipLaunchKernelGGL(k_to_f32, dim3(nblk(n)), dim3(TPB), 0, s, dst, src, n); }
+
+// y += x with an FP32 correction x (FP64 accumulation of the refinement)
+__global__ void k_add_f32(double *y, const float *x, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += (double)x[i];
+}
+void add_f32(hipStream_t s, double *y, const float *x, long n) { hipLaunchKernelGGL(k_add_f32, dim3(nblk(n)), dim3(TPB), 0, s, y, x, n); }
 
 __global__ void k_copy(double *d, const double *a, long n) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -836,22 +884,28 @@ __global__ void k_axpy(double *y, const double *x, double a, long n) {
 }
 void axpy_scalar(hipStream_t s, double *y, const double *x, double a, long n) { hipLaunchKernelGGL(k_axpy, dim3(nblk(n)), dim3(TPB), 0, s, y, x, a, n); }
 
-__global__ void k_sub(double *r, const double *a, const double *b, long n) {
+template <typename V>
+__global__ void k_sub(V *r, const V *a, const V *b, long n) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) r[i] = a[i] - b[i];
 }
-void sub(hipStream_t s, double *r, const double *a, const double *b, long n) { hipLaunchKernelGGL(k_sub, dim3(nblk(n)), dim3(TPB), 0, s, r, a, b, n); }
+void sub(hipStream_t s, double *r, const double *a, const double *b, long n) { hipLaunchKernelGGL(k_sub<double>, dim3(nblk(n)), dim3(TPB), 0, s, r, a, b, n); }
+void sub(hipStream_t s, float *r, const float *a, const float *b, long n) { hipLaunchKernelGGL(k_sub<float>, dim3(nblk(n)), dim3(TPB), 0, s, r, a, b, n); }
 
 // multi_grid line search update (General_matrix_functions.c:626-638)
-__global__ void k_mg_update(double *vel, const double *del, double *res, const double *AU, const double *sc, long n) {
+template <typename V>
+__global__ void k_mg_update(V *vel, const V *del, V *res, const V *AU, const double *sc, long n) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double alpha = sc[1] / sc[0];
+    const V alpha = (V)(sc[1] / sc[0]);
     vel[i] += alpha * del[i];
     if (res) res[i] -= alpha * AU[i];
 }
 void mg_update(hipStream_t s, double *vel, const double *del, double *res, const double *AU, const double *sc, long n) {
-    hipLaunchKernelGGL(k_mg_update, dim3(nblk(n)), dim3(TPB), 0, s, vel, del, res, AU, sc, n);
+    hipLaunchKernelGGL(k_mg_update<double>, dim3(nblk(n)), dim3(TPB), 0, s, vel, del, res, AU, sc, n);
+}
+void mg_update(hipStream_t s, float *vel, const float *del, float *res, const float *AU, const double *sc, long n) {
+    hipLaunchKernelGGL(k_mg_update<float>, dim3(nblk(n)), dim3(TPB), 0, s, vel, del, res, AU, sc, n);
 }
 
 __global__ void k_uz_z(double *z1, const double *BPI, const double *r1, long n) {

@@ -363,6 +363,7 @@ int build_mesh(chip_ctx *c) {
             return -1;
         if (p.mixed) {
             if (dev_alloc(c, &L.velf, 3 * (size_t)g.nno) || dev_alloc(c, &L.resf, 3 * (size_t)g.nno) ||
+                dev_alloc(c, &L.BIf, 3 * (size_t)g.nno) ||
                 dev_alloc(c, &L.flf, 3 * (size_t)g.nno) || dev_alloc(c, &L.delf, 3 * (size_t)g.nno) ||
                 dev_alloc(c, &L.AUf, 3 * (size_t)g.nno))
                 return -1;
