@@ -9,6 +9,9 @@ multigrid velocity solve to accuracy 1e-6).
 Default workload: 128x128x64 elements in a 2x2x1 box, 6 multigrid levels,
 Ra = 1e7, eta = exp(-ln(1e4) T) clipped to [1e-4, 1], seeded initial
 temperature.  The step-0 Stokes solve and the warmup steps are untimed.
+The measured form is the mixed one (FP32 multigrid cycles inside FP64
+iterative refinement, to the same 1e-6 criterion); the plain FP64 form is
+timed beside it for the ratio.
 
 Prints one JSON line: s/timestep (lower is better), the roofline of the
 dominant kernel (the 8-colour Gauss-Seidel pass) from HIP events on the
@@ -111,13 +114,15 @@ def cpu_baseline_port(name, seed, mg_calls_per_step):
 
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary
-    (profiles/<kernel>_traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE runs,
-    scaled by what the same counters read for a copy of known size), or None."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_traffic.json")
+    (profiles/k_gs_colour_traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE
+    runs, scaled by what the same counters read for a copy of known size), or
+    None."""
+    path = os.path.join(ROOT, "profiles", "k_gs_colour_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
-        return json.load(fh).get("hbm_bytes_per_launch")
+        per_kernel = json.load(fh).get("hbm_bytes_per_launch")
+    return per_kernel.get(kernel) if isinstance(per_kernel, dict) else None
 
 
 def cpu_baseline(name, seed, mg_calls_per_step):
@@ -129,6 +134,51 @@ def cpu_baseline(name, seed, mg_calls_per_step):
     return cpu_baseline_port(name, seed, mg_calls_per_step)
 
 
+def run_form(p, T, mixed, steps, warmup, graphs, profile):
+    """Timestep loop of one form: step-0 solve and warmup untimed, then `steps`
+    timed steps; with profile, one more step with kernel-stamped events on
+    every colour pass and one on every matvec."""
+    import torch
+
+    from citcom_amd.hip import Citcom
+
+    form = "mixed" if mixed else "plain"
+    t0 = time.perf_counter()
+    c = Citcom(p, mixed=mixed)
+    if not graphs:
+        c.set_option("graphs", 0)
+    c.initialize(T)
+    log(f"[{form}] setup {time.perf_counter() - t0:.1f} s, device memory {c.device_bytes() / 2**30:.2f} GiB")
+    t0 = time.perf_counter()
+    st0 = c.general_stokes_solver()
+    log(f"[{form}] step-0 Stokes solve {time.perf_counter() - t0:.2f} s, {st0['iterations']} Uzawa iterations")
+    for w in range(warmup):
+        r = c.timestep()
+        log(f"[{form}] warmup {w}: {r['stokes']['iterations']} Uzawa its, {r['stokes']['mg_cycles']} MG calls")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    recs = [c.timestep() for _ in range(steps)]
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    for k, r in enumerate(recs):
+        log(f"[{form}] step {k}: dt {r['pg']['timestep']:.3e} Nu_top {r['Nu_top']:.5f} "
+            f"Uzawa {r['stokes']['iterations']} MG {r['stokes']['mg_cycles']}")
+    out = {"s_per_step": elapsed / steps,
+           "uzawa_iterations_per_step": float(np.mean([r["stokes"]["iterations"] for r in recs])),
+           "mg_calls_per_step": float(np.mean([r["stokes"]["mg_cycles"] for r in recs])),
+           "Nu_top": float(recs[-1]["Nu_top"])}
+    if profile:
+        c.profile("gs_colour")
+        c.timestep()
+        out["gs"] = c.profile_read()
+        c.profile("matvec")
+        c.timestep()
+        out["mv"] = c.profile_read()
+        c.profile(None)
+    c.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,8 +186,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="cfg3")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--form", choices=("mixed", "plain"), default="mixed",
+                    help="mixed: FP32 multigrid inside FP64 refinement; plain: the reference's FP64 arithmetic")
+    ap.add_argument("--compare-steps", type=int, default=2,
+                    help="timed steps of the plain FP64 form run beside the mixed form (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--mixed", action="store_true", help="FP32 multigrid inside FP64 refinement")
     ap.add_argument("--no-graphs", action="store_true", help="launch multigrid kernels one by one")
     ap.add_argument("--piterations", type=int, default=None,
                     help="cap on Uzawa iterations (profiling runs only: changes the workload)")
@@ -149,48 +202,18 @@ def main():
 
     import torch  # device sync for the timed region
 
-    from citcom_amd.hip import Citcom
-
     p = make_problem(args.config)
     if args.piterations is not None:
         p.p_iterations = args.piterations
     nox, noy, noz = p.dims()
     log(f"config {args.config}: {nox - 1}x{noy - 1}x{noz - 1} elements, {p.levels} levels")
-    t0 = time.perf_counter()
-    c = Citcom(p, mixed=args.mixed)
-    if args.no_graphs:
-        c.set_option("graphs", 0)
     T = deck.seeded_temperature(p, seed=args.seed) if p.tdepv else deck.perturbed_temperature(p)
-    c.initialize(T)
-    log(f"setup {time.perf_counter() - t0:.1f} s, device memory {c.device_bytes() / 2**30:.2f} GiB")
-    t0 = time.perf_counter()
-    st0 = c.general_stokes_solver()
-    log(f"step-0 Stokes solve {time.perf_counter() - t0:.2f} s, {st0['iterations']} Uzawa iterations")
-    for w in range(args.warmup):
-        r = c.timestep()
-        log(f"warmup {w}: {r['stokes']['iterations']} Uzawa its, {r['stokes']['mg_cycles']} MG calls")
+    mixed = args.form == "mixed"
+    main_run = run_form(p, T, mixed, args.steps, args.warmup, not args.no_graphs, profile=True)
+    plain = None
+    if mixed and args.compare_steps > 0:
+        plain = run_form(p, T, False, args.compare_steps, 1, not args.no_graphs, profile=False)
 
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    recs = []
-    for k in range(args.steps):
-        recs.append(c.timestep())
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    s_per_step = elapsed / args.steps
-    for k, r in enumerate(recs):
-        log(f"step {k}: dt {r['pg']['timestep']:.3e} Nu_top {r['Nu_top']:.5f} Uzawa {r['stokes']['iterations']} "
-            f"MG {r['stokes']['mg_cycles']}")
-    mg_calls = float(np.mean([r["stokes"]["mg_cycles"] for r in recs]))
-
-    # dominant kernel: one more step with HIP events around every colour pass
-    c.profile("gs_colour")
-    c.timestep()
-    n_gs, ms_gs, bytes_gs = c.profile_read()
-    c.profile("matvec")
-    c.timestep()
-    n_mv, ms_mv, bytes_mv = c.profile_read()
-    c.profile(None)
     # known-size copy (256 MiB read + 256 MiB written): the reference point that
     # scales hardware byte counters (FETCH_SIZE / WRITE_SIZE) into HBM bytes
     src = torch.ones(COPY_PROBE_BYTES // 4, dtype=torch.float32, device="cuda")
@@ -199,11 +222,27 @@ def main():
         dst.copy_(src)
     torch.cuda.synchronize()
     del src, dst
+
+    n_gs, ms_gs, bytes_gs = main_run["gs"]
+    n_mv, ms_mv, bytes_mv = main_run["mv"]
     avg_ms = ms_gs / max(n_gs, 1)
     achieved = (bytes_gs / max(n_gs, 1)) / (avg_ms * 1e-3) / 1e9
     log(f"gs_colour: {n_gs} launches, {ms_gs:.1f} ms total, {achieved:.0f} GB/s; "
         f"matvec: {n_mv} launches, {ms_mv:.1f} ms, {(bytes_mv / max(ms_mv, 1e-9)) / 1e6:.0f} GB/s")
-
+    s_per_step = main_run["s_per_step"]
+    kernel = "k_gs_colour<float>" if mixed else "k_gs_colour<double>"
+    config = {"workload": f"{args.config}: {nox - 1}x{noy - 1}x{noz - 1} elements, {p.levels} MG levels, "
+                          f"Ra={p.rayleigh:g}, viscosity contrast 1e4, accuracy={p.accuracy:g}",
+              "form": ("FP32 multigrid cycles inside FP64 iterative refinement (FP64 residual, Uzawa CG, "
+                       "coarsest level); FP32 stiffness as in the reference") if mixed else
+                      "plain: FP64 vectors, the reference's FP32 stiffness",
+              "uzawa_iterations_per_step": main_run["uzawa_iterations_per_step"],
+              "mg_calls_per_step": main_run["mg_calls_per_step"]}
+    if plain is not None:
+        config["plain_fp64_s_per_step"] = plain["s_per_step"]
+        config["speedup_vs_plain_fp64"] = plain["s_per_step"] / s_per_step
+        config["learned_warm_start"] = "not built"
+        log(f"plain FP64 form: {plain['s_per_step']:.3f} s/step; mixed form {s_per_step:.3f} s/step")
     out = {
         "metric": METRIC,
         "value": s_per_step,
@@ -215,22 +254,17 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-mixed" if args.mixed else "f64",
+        "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic",
-        "config": {"workload": f"{args.config}: {nox - 1}x{noy - 1}x{noz - 1} elements, {p.levels} MG levels, "
-                               f"Ra={p.rayleigh:g}, accuracy={p.accuracy:g}, plain FP64 vectors with the "
-                               f"reference's FP32 stiffness", "form": "mixed" if args.mixed else "plain",
-                   "uzawa_iterations_per_step": float(np.mean([r['stokes']['iterations'] for r in recs])),
-                   "mg_calls_per_step": mg_calls},
+        "config": config,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic("k_gs_colour"), "kernel": "k_gs_colour",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(kernel), "kernel": kernel,
                      "launches": n_gs, "avg_launch_ms": avg_ms},
     }
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(args.config, args.seed, mg_calls)
+        out["cpu_baseline"] = cpu_baseline(args.config, args.seed, main_run["mg_calls_per_step"])
     print(json.dumps(out), flush=True)
-    c.close()
 
 
 if __name__ == "__main__":

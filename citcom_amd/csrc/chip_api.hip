@@ -118,6 +118,28 @@ void do_matvec(chip_ctx *c, int lev, const V *u, V *Au, const V *rhs = nullptr, 
 
 // gauss_seidel (General_matrix_functions.c:1160-1361), 8-colour; returns Ad = K d
 template <typename V>
+void do_gs(chip_ctx *c, int lev, V *d, const V *F, V *Ad, int sweeps, int guess);
+
+inline void copy_v(hipStream_t s, double *d, const double *a, long n) { copy(s, d, a, n); }
+inline void copy_v(hipStream_t s, float *d, const float *a, long n) { fcopy(s, d, a, n); }
+
+// a smoother call of multi_grid: the affine form where the level has one
+template <typename V>
+void mg_gs(chip_ctx *c, int lev, V *d, const V *F, V *Ad, int sweeps, int guess) {
+    const Level &L = c->L[lev];
+    if (L.Ma && c->small_levels && sweeps == L.affine_sweeps) {
+        V *out;
+        if constexpr (sizeof(V) == sizeof(double)) out = L.tmp;
+        else out = L.tmpf;
+        affine_apply(S(c), L, guess ? (const V *)d : nullptr, F, out);
+        do_matvec(c, lev, (const V *)out, Ad);
+        copy_v(S(c), d, (const V *)out, 3L * L.g.nno);
+        return;
+    }
+    do_gs(c, lev, d, F, Ad, sweeps, guess);
+}
+
+template <typename V>
 void do_gs(chip_ctx *c, int lev, V *d, const V *F, V *Ad, int sweeps, int guess) {
     const Level &L = c->L[lev];
     if constexpr (sizeof(V) == sizeof(double)) {
@@ -170,7 +192,7 @@ void enqueue_cycles(chip_ctx *c, V *F) {
                 const LV<V> v = vecs<V>(L[dlev]);
                 const int cycles = (dlev == levmax) ? p.v_steps_high : p.down_heavy;
                 V *rhs = (dlev == lev) ? fl(dlev) : v.fl;
-                do_gs(c, dlev, v.vel, (const V *)rhs, v.AU, cycles, dlev == lev);
+                mg_gs(c, dlev, v.vel, (const V *)rhs, v.AU, cycles, dlev == lev);
                 sub(S(c), v.res, rhs, v.AU, 3L * L[dlev].g.nno);
                 project(S(c), L[dlev], L[dlev - 1], (const V *)v.res, vecs<V>(L[dlev - 1]).fl);
             }
@@ -180,7 +202,7 @@ void enqueue_cycles(chip_ctx *c, V *F) {
                 const int cycles = (ulev == levmax) ? p.v_steps_high : p.up_heavy;
                 const long n = 3L * L[ulev].g.nno;
                 interp(S(c), L[ulev - 1], L[ulev], (const V *)vecs<V>(L[ulev - 1]).vel, v.del);
-                do_gs(c, ulev, v.del, (const V *)v.res, v.AU, cycles, 1);
+                mg_gs(c, ulev, v.del, (const V *)v.res, v.AU, cycles, 1);
                 dots(S(c), c, (int)n, 2, (const V *)v.AU, (const V *)v.AU, v.AU, v.res, nullptr, nullptr, nullptr,
                      nullptr, c->dots + D_MG);
                 mg_update(S(c), v.vel, v.del, ulev == levmax ? v.res : nullptr, v.AU, c->dots + D_MG, n);
@@ -345,6 +367,8 @@ int rebuild(chip_ctx *c) {
     for (int lev = c->levmax; lev >= c->levmin; lev--) node_ks(q, c->L[lev], c->p);
     for (int lev = c->levmax; lev >= c->levmin; lev--) build_bpi(q, c->L[lev], c->p.precond);
     if (c->L[c->levmin].Mc) coarse_build(q, c->L[c->levmin], c->p.v_steps_low);
+    for (int lev = c->levmin; lev <= c->levmax; lev++)
+        if (c->L[lev].Ma) affine_build(q, c->L[lev], c->L[lev].affine_sweeps);
     if (c->p.mixed)
         for (int lev = c->levmin; lev <= c->levmax; lev++) to_f32(q, c->L[lev].BIf, c->L[lev].BI, 3L * c->L[lev].g.nno);
     CHIP_CHECK(hipGetLastError());
@@ -630,6 +654,16 @@ int chip_apply(chip_ctx *c, const char *op, int lev, const double *in, const dou
         do_gs(c, lev, a, b, d, arg, guess);
         CHIP_CHECK(down(out, a, nv));
         CHIP_CHECK(down(out2, d, nv));
+    } else if (o == "affine0" || o == "affine1") {
+        if (!L.Ma) {
+            set_error("affine: the level has no affine smoother form");
+            return -1;
+        }
+        const int guess = (o == "affine1");
+        if (guess) CHIP_CHECK(up(a, in, nv));
+        CHIP_CHECK(up(b, in2, nv));
+        affine_apply(q, L, guess ? (const double *)a : nullptr, (const double *)b, L.tmp);
+        CHIP_CHECK(down(out, L.tmp, nv));
     } else if (o == "coarse") {
         if (lev != c->levmin || !L.Mc) {
             set_error("coarse: the coarsest level has no coarse matrix");

@@ -62,6 +62,13 @@ struct Level {
     double *vel = nullptr, *res = nullptr, *fl = nullptr, *del = nullptr, *AU = nullptr;
     // coarsest level: its smoother call as a matrix [3nno][3nno], row-major
     double *Mc = nullptr;
+    // level above the coarsest: its smoother call as d = Ma din + Mb F
+    // ([3nno][3nno] each, row-major), for affine_sweeps sweeps; scratch
+    // vectors for the result in both precisions
+    double *Ma = nullptr, *Mb = nullptr;
+    int affine_sweeps = 0;
+    double *tmp = nullptr;
+    float *tmpf = nullptr;
     // FP32 copies for the mixed-precision cycle
     float *velf = nullptr, *resf = nullptr, *flf = nullptr, *delf = nullptr, *AUf = nullptr;
     float *BIf = nullptr;

@@ -36,6 +36,11 @@ void gs_small(hipStream_t s, const Level &L, double *d, const double *F, double 
 // 1: the level's stiffness and vectors fit in LDS (gs_small), 0: one launch
 // per colour pass
 int gs_small_kind(const Level &L);
+// level above the coarsest: its smoother call as d = Ma din + Mb F
+bool affine_ok(const Level &L);
+void affine_build(hipStream_t s, const Level &L, int sweeps);
+template <typename V>
+void affine_apply(hipStream_t s, const Level &L, const V *din, const V *F, V *out);
 // coarsest level: Mc = the v_steps_low-sweep smoother as a matrix, d = Mc F
 void coarse_build(hipStream_t s, const Level &L, int sweeps);
 template <typename V>

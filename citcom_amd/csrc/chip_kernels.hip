@@ -491,6 +491,12 @@ __global__ void __launch_bounds__(SMALL_TPB) k_gs_lds(Geo g, const float *__rest
         for (int idx = tid; idx < 3 * nno; idx += blockDim.x) d[idx] = ds[idx];
 }
 
+// smoother calls applied as matrices on levels of at most this many dofs
+// (two FP64 matrices of 18 MB at most)
+constexpr long AFFINE_MAX_DOF = 1500;
+
+bool affine_ok(const Level &L) { return 3L * L.g.nno <= AFFINE_MAX_DOF; }
+
 size_t lds_limit() {
     static size_t lim = 0;
     if (!lim) {
@@ -525,11 +531,106 @@ void gs_small(hipStream_t s, const Level &L, double *d, const double *F, double 
 // the smoother's result for e_j (one workgroup per column, rebuilt with the
 // stiffness); a call is then d = Mc F.
 
+__global__ void k_gs_unit(Geo g, const float *__restrict__ K, const double *__restrict__ BI, double *__restrict__ Ma,
+                          double *__restrict__ Mb, int sweeps);
+
 void coarse_build(hipStream_t s, const Level &L, int sweeps) {
     const int nno = L.g.nno;
-    hipLaunchKernelGGL(k_gs_lds, dim3(3 * nno), dim3(SMALL_TPB), kstage_bytes(nno) + vstage_bytes(nno), s, L.g, L.K,
-                       L.BI, L.NODE, L.Mc, nullptr, nullptr, sweeps, 0, 1);
+    if (gs_small_kind(L))
+        hipLaunchKernelGGL(k_gs_lds, dim3(3 * nno), dim3(SMALL_TPB), kstage_bytes(nno) + vstage_bytes(nno), s, L.g,
+                           L.K, L.BI, L.NODE, L.Mc, nullptr, nullptr, sweeps, 0, 1);
+    else
+        hipLaunchKernelGGL(k_gs_unit, dim3(3 * nno), dim3(192), (size_t)3 * nno * 8, s, L.g, L.K, L.BI, nullptr, L.Mc,
+                           sweeps);
 }
+
+// ---------------------------------------------------------------------------
+// The level above the coarsest as an affine operator.  Its gauss_seidel calls
+// in multi_grid (down_heavy = up_heavy sweeps) are, apart from the FP32
+// rounding of each increment, d_out = Ma d_in + Mb F (guess) or Mb F (zero
+// guess).  Column j of Ma is the smoother's result for d_in = e_j, F = 0;
+// column j of Mb for d_in = 0, F = e_j; one workgroup per column (three
+// wavefronts per 64-node tile, as k_gs_colour), d kept in LDS.
+// (Ma null: the Mb columns only, one workgroup each)
+__global__ void __launch_bounds__(192) k_gs_unit(Geo g, const float *__restrict__ K, const double *__restrict__ BI,
+                                                 double *__restrict__ Ma, double *__restrict__ Mb, int sweeps) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double *ds = (double *)smem;   // [3 nno]
+    __shared__ double Ps[3][3][64];
+    const int n3 = 3 * g.nno, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const bool forA = Ma && (int)blockIdx.x < n3;
+    const int jcol = (forA || !Ma) ? blockIdx.x : blockIdx.x - n3;
+    for (int idx = threadIdx.x; idx < n3; idx += blockDim.x) ds[idx] = (forA && idx == jcol) ? 1.0 : 0.0;
+    __syncthreads();
+    for (int cyc = 0; cyc < sweeps; cyc++)
+        for (int c = 0; c < 8; c++) {
+            const int tiles = (g.cn[c] + 63) >> 6;
+            for (int t = 0; t < tiles; t++) {
+                const int q = g.cs[c] + t * 64 + lane;
+                const bool act = q < g.cs[c] + g.cn[c];
+                int k = 0, i = 0, j = 0;
+                if (act) colour_kij(g, c, q, k, i, j);
+                const int self = nid(g, k, i, j);
+                if (act) {
+                    double out[3];
+                    const float *kp = K + ktile(q);
+                    if (w == 0)
+                        plane_rows<0>(g, kp, k, i, j, self, (const double *)ds, out);
+                    else if (w == 1)
+                        plane_rows<1>(g, kp, k, i, j, self, (const double *)ds, out);
+                    else
+                        plane_rows<2>(g, kp, k, i, j, self, (const double *)ds, out);
+                    Ps[w][0][lane] = out[0];
+                    Ps[w][1][lane] = out[1];
+                    Ps[w][2][lane] = out[2];
+                }
+                __syncthreads();
+                if (act) {
+                    const int r = w, e = 3 * self + r;
+                    const double Fr = (!forA && e == jcol) ? 1.0 : 0.0;
+                    const double o = (Ps[0][r][lane] + Ps[1][r][lane]) + Ps[2][r][lane];
+                    const float tt = (float)((Fr - o) * BI[e]);
+                    ds[e] = ds[e] + tt;
+                }
+                __syncthreads();
+            }
+        }
+    double *M = forA ? Ma : Mb;
+    for (int idx = threadIdx.x; idx < n3; idx += blockDim.x) M[(size_t)idx * n3 + jcol] = ds[idx];
+}
+
+void affine_build(hipStream_t s, const Level &L, int sweeps) {
+    const int n3 = 3 * L.g.nno;
+    hipLaunchKernelGGL(k_gs_unit, dim3(2 * n3), dim3(192), (size_t)n3 * 8, s, L.g, L.K, L.BI, L.Ma, L.Mb, sweeps);
+}
+
+// out = Ma din + Mb F (din may be null: zero guess).  One wavefront per row
+// i: lane t sums the Ma terms j = t, t+64, ... in order, then the Mb terms
+// likewise, into S[t]; the S[t] meet in a pairwise tree (S[q] += S[q+w],
+// w = 1 .. 32) formed by a butterfly over the wavefront.
+template <typename V>
+__global__ void __launch_bounds__(256) k_affine_apply(const double *__restrict__ Ma, const double *__restrict__ Mb,
+                                                      const V *__restrict__ din, const V *__restrict__ F,
+                                                      V *__restrict__ out, int n3) {
+    const int t = threadIdx.x & 63, i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    double S = 0.0;
+    if (i < n3) {
+        if (din)
+            for (int j = t; j < n3; j += 64) S = S + Ma[(size_t)i * n3 + j] * (double)din[j];
+        for (int j = t; j < n3; j += 64) S = S + Mb[(size_t)i * n3 + j] * (double)F[j];
+    }
+#pragma unroll
+    for (int w = 1; w < 64; w *= 2) S = S + __shfl_xor(S, w, 64);
+    if (i < n3 && t == 0) out[i] = (V)S;
+}
+
+template <typename V>
+void affine_apply(hipStream_t s, const Level &L, const V *din, const V *F, V *out) {
+    const int n3 = 3 * L.g.nno;
+    hipLaunchKernelGGL(k_affine_apply<V>, dim3((n3 + 3) / 4), dim3(256), 0, s, L.Ma, L.Mb, din, F, out, n3);
+}
+template void affine_apply<double>(hipStream_t, const Level &, const double *, const double *, double *);
+template void affine_apply<float>(hipStream_t, const Level &, const float *, const float *, float *);
 
 // d = Mc F: sixteen lanes per row i, lane t summing j = t, t+16, ... in
 // order into S[t]; the S[t] meet in a pairwise tree (S[q] += S[q+w],

@@ -374,7 +374,19 @@ int build_mesh(chip_ctx *c) {
     }
     {
         Level &C = c->L[c->levmin];
-        if (gs_small_kind(C) && dev_alloc(c, &C.Mc, 9 * (size_t)C.g.nno * C.g.nno)) return -1;
+        if (affine_ok(C) && dev_alloc(c, &C.Mc, 9 * (size_t)C.g.nno * C.g.nno)) return -1;
+        // the level above: affine form when its two smoother calls in
+        // multi_grid use the same number of sweeps
+        if (C.Mc && c->levmin + 1 < c->levmax && p.down_heavy == p.up_heavy) {
+            Level &A = c->L[c->levmin + 1];
+            if (affine_ok(A)) {
+                const size_t n3 = 3 * (size_t)A.g.nno;
+                if (dev_alloc(c, &A.Ma, n3 * n3) || dev_alloc(c, &A.Mb, n3 * n3) || dev_alloc(c, &A.tmp, n3) ||
+                    dev_alloc(c, &A.tmpf, n3))
+                    return -1;
+                A.affine_sweeps = p.down_heavy;
+            }
+        }
     }
     const Geo &g = c->L[c->levmax].g;
     const size_t nno = g.nno, nel = g.nel, neq = 3 * nno;

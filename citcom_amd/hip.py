@@ -251,7 +251,8 @@ class Citcom:
     def apply(self, op: str, lev: int, x=None, x2=None, sweeps: int = 0):
         """Single operator on host arrays (see chip_apply in include/citcom_hip.h)."""
         n3, ne = 3 * self.nno(lev), self.nel(lev)
-        out_n = {"matvec": n3, "gs0": n3, "gs1": n3, "coarse": n3, "div": ne, "grad": n3, "strip": n3,
+        out_n = {"matvec": n3, "gs0": n3, "gs1": n3, "coarse": n3, "affine0": n3, "affine1": n3, "div": ne, "grad": n3,
+                 "strip": n3,
                  "project": 3 * self.nno(lev - 1) if lev > 0 else 0, "interp": n3}[op]
         out = np.zeros(out_n)
         out2 = np.zeros(n3)

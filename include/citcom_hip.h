@@ -115,6 +115,9 @@ int chip_construct_stiffness_B_matrix(chip_ctx *c);
  *         out = d0 after, out2 = A d0
  *     "coarse" (coarsest level) the multigrid's coarsest-level smoother call
  *         (v_steps_low sweeps from zero) applied as its matrix: out = Mc in2
+ *     "affine0"/"affine1" (the level above) its multigrid smoother call
+ *         (down_heavy sweeps) as an affine map: out = Mb in2 (zero guess) or
+ *         Ma in + Mb in2
  *     "project" project_vector lev -> lev-1 (Solver_multigrid.c:72-159)
  *     "interp"  interp_vector lev-1 -> lev (Solver_multigrid.c:173-298)
  *     "div" assemble_div_u (Element_calculations.c:691-720)

@@ -73,6 +73,7 @@ struct orc
 	/* work */
 	double *mg_vel[MAXL], *mg_res[MAXL], *mg_rhs[MAXL], *mg_fl[MAXL], *mg_del[MAXL], *mg_AU[MAXL];
 	double *Mc;	/* coarsest-level smoother as a matrix (smoother 1), column-major */
+	double *Ma, *Mb;	/* level levmin+1: its smoother call d = Ma din + Mb F (smoother 1), column-major */
 };
 
 static double lpoly(int p, double y)
@@ -472,6 +473,8 @@ void orc_free(orc *o)
 	for(d = 0; d < 3; d++)
 		free(o->TB[d]);
 	free(o->Mc);
+	free(o->Ma);
+	free(o->Mb);
 	free(o->mat);
 	free(o->expansivity);
 	free(o->diffusivity);
@@ -855,18 +858,83 @@ static void build_bpi(orc *o, int lev)
 }
 
 /* construct_stiffness_B_matrix, Construct_arrays.c:834-889 */
-/* The coarsest level's gauss_seidel call in multi_grid (v_steps_low sweeps
+/* Smoother calls applied as matrices (8-colour smoother only), on levels of
+ * at most MATRIX_MAX_DOF unknowns.
+ * The coarsest level's gauss_seidel call in multi_grid (v_steps_low sweeps
  * from a zero guess, General_matrix_functions.c:574, :613) is linear in its
- * right-hand side apart from the FP32 rounding of each increment.  With the
- * 8-colour smoother it is applied as a matrix: column j is the smoother's
- * result for the unit right-hand side e_j, built whenever the operators are
- * rebuilt.  Mc F then equals the sweeps on F up to that FP32 rounding. */
+ * right-hand side apart from the FP32 rounding of each increment: column j
+ * of Mc is the smoother's result for the unit right-hand side e_j.  The
+ * level above (down_heavy = up_heavy sweeps, with or without a guess) is
+ * affine: d = Ma din + Mb F, column j of Ma from din = e_j, F = 0, of Mb from
+ * din = 0, F = e_j.  Built whenever the operators are rebuilt; they equal
+ * the sweeps up to the FP32 rounding of the increments. */
+#define MATRIX_MAX_DOF 1500
+
+static void build_affine_operator(orc *o)
+{
+	const int lev = o->levmin + 1, n3 = o->neq[lev];
+	double *e = calloc(n3, sizeof(double)), *z = calloc(n3, sizeof(double)), *Ad = malloc(sizeof(double) * n3);
+	int j;
+	free(o->Ma);
+	free(o->Mb);
+	o->Ma = o->Mb = NULL;
+	if(!(o->Mc && lev < o->levmax && o->p.down_heavy == o->p.up_heavy && n3 <= MATRIX_MAX_DOF))
+		goto done;
+	o->Ma = malloc(sizeof(double) * (size_t)n3 * n3);
+	o->Mb = malloc(sizeof(double) * (size_t)n3 * n3);
+	for(j = 0; j < n3; j++)
+	{
+		double *ca = o->Ma + (size_t)j * n3, *cb = o->Mb + (size_t)j * n3;
+		memset(ca, 0, sizeof(double) * n3);
+		ca[j] = 1.0;
+		orc_gauss_seidel(o, lev, ca, z, Ad, o->p.down_heavy, 1);
+		e[j] = 1.0;
+		orc_gauss_seidel(o, lev, cb, e, Ad, o->p.down_heavy, 0);
+		e[j] = 0.0;
+	}
+done:
+	free(e);
+	free(z);
+	free(Ad);
+}
+
+/* out = Ma din + Mb F (din NULL: zero guess); row i sums the Ma terms
+ * j = t, t+64, ... in order, then the Mb terms likewise, into S[t] (t < 64),
+ * and the S[t] meet in a pairwise tree (S[q] += S[q+w], w = 1 .. 32) */
+void orc_affine_solve(orc *o, const double *din, const double *F, double *out)
+{
+	const int n3 = o->neq[o->levmin + 1];
+	int i, j, w, q;
+	for(i = 0; i < n3; i++)
+	{
+		double S[64];
+		for(q = 0; q < 64; q++)
+			S[q] = 0.0;
+		if(din)
+			for(j = 0; j < n3; j++)
+				S[j & 63] = S[j & 63] + o->Ma[(size_t)j * n3 + i] * din[j];
+		for(j = 0; j < n3; j++)
+			S[j & 63] = S[j & 63] + o->Mb[(size_t)j * n3 + i] * F[j];
+		for(w = 1; w < 64; w *= 2)
+			for(q = 0; q < 64; q += 2 * w)
+				S[q] = S[q] + S[q + w];
+		out[i] = S[0];
+	}
+}
+
 static void build_coarse_operator(orc *o)
 {
 	const int lev = o->levmin, n3 = o->neq[lev];
 	double *e = calloc(n3, sizeof(double)), *Ad = malloc(sizeof(double) * n3);
 	int j;
 	free(o->Mc);
+	o->Mc = NULL;
+	if(n3 > MATRIX_MAX_DOF)
+	{
+		free(e);
+		free(Ad);
+		return;
+	}
 	o->Mc = malloc(sizeof(double) * (size_t)n3 * n3);
 	for(j = 0; j < n3; j++)
 	{
@@ -913,7 +981,10 @@ void orc_build_operators(orc *o)
 	/* rebuild_BI_on_boundary (:892-952) changes only OFFSIDE nodes, which a
 	 * single subdomain does not have */
 	if(o->p.smoother == 1)
+	{
 		build_coarse_operator(o);
+		build_affine_operator(o);
+	}
 }
 
 void orc_set_viscosity(orc *o, const float *evi)
@@ -1221,10 +1292,27 @@ static double pdot(const orc *o, int lev, const double *A, const double *B)
  * afterwards) */
 static void coarse_smooth(orc *o, double *vel, const double *rhs, double *AU)
 {
-	if(o->p.smoother == 1)
+	if(o->p.smoother == 1 && o->Mc)
 		orc_coarse_solve(o, rhs, vel);
 	else
 		orc_gauss_seidel(o, o->levmin, vel, rhs, AU, o->p.v_steps_low, 0);
+}
+
+/* a smoother call of multi_grid above the coarsest level: the affine form on
+ * the level that has one (Ad = K d by a matvec, as gauss_seidel returns) */
+static void mg_smooth(orc *o, int lev, double *d, const double *F, double *Ad, int cycles, int guess)
+{
+	if(o->Ma && lev == o->levmin + 1 && cycles == o->p.down_heavy)
+	{
+		const int n3 = o->neq[lev];
+		double *out = malloc(sizeof(double) * n3);
+		orc_affine_solve(o, guess ? d : NULL, F, out);
+		memcpy(d, out, sizeof(double) * n3);
+		free(out);
+		orc_matvec(o, lev, d, Ad, 1);
+	}
+	else
+		orc_gauss_seidel(o, lev, d, F, Ad, cycles, guess);
 }
 
 /* multi_grid, General_matrix_functions.c:525-653 */
@@ -1254,7 +1342,7 @@ double orc_multi_grid(orc *o, double *d1, double *F, double *Au, double acc, int
 			for(dlev = lev; dlev >= levmin + 1; dlev--)
 			{
 				const int cycles = (dlev == levmax) ? o->p.v_steps_high : o->p.down_heavy;
-				orc_gauss_seidel(o, dlev, vel[dlev], rhs[dlev], AU[dlev], cycles, dlev == lev);
+				mg_smooth(o, dlev, vel[dlev], rhs[dlev], AU[dlev], cycles, dlev == lev);
 				for(i = 0; i < o->neq[dlev]; i++)
 					res[dlev][i] = rhs[dlev][i] - AU[dlev][i];
 				orc_project(o, dlev, res[dlev], rhs[dlev - 1]);
@@ -1267,7 +1355,7 @@ double orc_multi_grid(orc *o, double *d1, double *F, double *Au, double acc, int
 				double AudotAu, alpha;
 				orc_interp(o, ulev, vel[ulev - 1], del[ulev]);
 				orc_strip(o, ulev, del[ulev]);
-				orc_gauss_seidel(o, ulev, del[ulev], res[ulev], AU[ulev], cycles, 1);
+				mg_smooth(o, ulev, del[ulev], res[ulev], AU[ulev], cycles, 1);
 				AudotAu = vdot(o, ulev, AU[ulev], AU[ulev]);
 				alpha = vdot(o, ulev, AU[ulev], res[ulev]) / AudotAu;
 				for(i = 0; i < o->neq[ulev]; i++)

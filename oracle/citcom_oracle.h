@@ -80,6 +80,8 @@ void orc_matvec(orc *o, int lev, const double *u, double *Au, int strip);
 void orc_gauss_seidel(orc *o, int lev, double *d0, const double *F, double *Ad, int cycles, int guess);
 /* smoother 1: the coarsest-level smoother of multi_grid as a matrix, d = Mc F */
 void orc_coarse_solve(orc *o, const double *F, double *d);
+/* smoother 1: the level above the coarsest, out = Ma din + Mb F (din may be NULL) */
+void orc_affine_solve(orc *o, const double *din, const double *F, double *out);
 void orc_project(orc *o, int lev, const double *fine, double *coarse);
 void orc_interp(orc *o, int lev, const double *coarse, double *fine);
 void orc_div(orc *o, int lev, const double *U, double *divU);

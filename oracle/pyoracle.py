@@ -81,6 +81,7 @@ def lib():
         L.orc_matvec.argtypes = [P, C.c_int, dp, dp, C.c_int]
         L.orc_gauss_seidel.argtypes = [P, C.c_int, dp, dp, dp, C.c_int, C.c_int]
         L.orc_coarse_solve.argtypes = [P, dp, dp]
+        L.orc_affine_solve.argtypes = [P, P, dp, dp]
         L.orc_project.argtypes = [P, C.c_int, dp, dp]
         L.orc_interp.argtypes = [P, C.c_int, dp, dp]
         L.orc_div.argtypes = [P, C.c_int, dp, dp]
@@ -162,6 +163,14 @@ class Oracle:
         d = np.zeros(3 * self.nno(0))
         lib().orc_coarse_solve(self.h, np.ascontiguousarray(F, dtype=np.float64), d)
         return d
+
+    def affine_solve(self, din, F):
+        """smoother 1: the level above the coarsest, Ma din + Mb F"""
+        out = np.zeros(3 * self.nno(1))
+        a = None if din is None else np.ascontiguousarray(din, dtype=np.float64)
+        lib().orc_affine_solve(self.h, None if a is None else a.ctypes.data_as(C.c_void_p),
+                               np.ascontiguousarray(F, dtype=np.float64), out)
+        return out
 
     def project(self, lev, fine):
         out = np.zeros(3 * self.nno(lev - 1))

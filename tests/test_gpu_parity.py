@@ -235,5 +235,12 @@ def test_smoother_kernels_bitwise(name, small, pyoracle):
         assert np.array_equal(c.apply("coarse", 0, None, F), o.coarse_solve(F))
         d, _ = o.gauss_seidel(0, np.zeros_like(F), F, p.v_steps_low, 0)
         assert rel_l2(c.apply("coarse", 0, None, F), d) < 1e-6
+        if 3 * c.nno(1) <= 1500 and p.levels > 2:
+            # the level above, as an affine map (with and without a guess)
+            x = inp[1]
+            assert np.array_equal(c.apply("affine0", 1, None, x["F"]), o.affine_solve(None, x["F"]))
+            assert np.array_equal(c.apply("affine1", 1, x["d0"], x["F"]), o.affine_solve(x["d0"], x["F"]))
+            d, _ = o.gauss_seidel(1, x["d0"], x["F"], p.down_heavy, 1)
+            assert rel_l2(c.apply("affine1", 1, x["d0"], x["F"]), d) < 1e-6
     say(name, "small_levels", small, "smoother bitwise on", p.levels, "levels")
     c.close()
