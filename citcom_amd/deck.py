@@ -264,6 +264,8 @@ def workload(name: str) -> dict[str, object]:
     cfg3  128x128x64 elements in a 2x2x1 box, 6 levels, Ra=1e7,
           eta = exp(-ln(1e4) T) clipped to [1e-4, 1], CFL 0.5
     cfg5  256x256x128 elements, 7 levels, Ra=1e7, contrast 1e5, top layer x100
+    learn16  16^3 elements, 3 levels, Ra=1e5, eta = exp(-ln(1e3) T): the
+          training and test problems of the learned warm start
 
     ``name@acc`` is the same workload with ``accuracy=acc`` (e.g. cfg1@1e-8).
     """
@@ -294,6 +296,10 @@ def workload(name: str) -> dict[str, object]:
         d.update(_grid((2.0, 2.0, 1.0), (4, 4, 2), 6))
         d.update({"mgunitx": 4, "mgunity": 4, "mgunitz": 2, "levels": 6, "maxstep": 10, "finetunedt": 1.0})
         d.update(_tdep(1e7, 1e4, vmin=1e-4, vmax=1.0))
+    elif name == "learn16":
+        d.update(_grid((1.0, 1.0, 1.0), (4, 4, 4), 3))
+        d.update({"mgunitx": 4, "mgunity": 4, "mgunitz": 4, "levels": 3, "maxstep": 1, "finetunedt": 0.75})
+        d.update(_tdep(1e5, 1e3))
     elif name == "cfg5":
         d.update(_grid((2.0, 2.0, 1.0), (4, 4, 2), 7))
         d.update({"mgunitx": 4, "mgunity": 4, "mgunitz": 2, "levels": 7, "maxstep": 5, "finetunedt": 1.0})

@@ -1,0 +1,179 @@
+"""The learned warm start: its BF16 MFMA kernels against the CPU
+restatement (oracle/learn_oracle.py), then training on solutions of the
+plain solver and what the trained network gains as the Uzawa initial guess.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+
+
+def _bf(a):
+    import learn_oracle as lo
+    return lo.to_bf16(np.asarray(a, np.float32))
+
+
+def test_kernels_against_oracle():
+    import torch
+    import learn_oracle as lo
+    from citcom_amd import learn
+    rng = np.random.default_rng(4)
+    dims = (5, 6, 7)   # nx, ny, nz
+    nb, nv, CH = 2, 5 * 6 * 7, 16
+    x = _bf(rng.standard_normal((nb, nv, CH)))
+    w = _bf(rng.standard_normal((CH, 27, CH)) * 0.1)
+    bias = rng.standard_normal(CH).astype(np.float32) * 0.1
+    net = learn.ConvNet(layers=1)
+    net.w[0] = torch.tensor(w).cuda()
+    net.wbf[0] = net.w[0].to(torch.bfloat16)
+    net.b[0] = torch.tensor(bias).cuda()
+    L = learn._cl()
+    P = learn._p
+    xt = torch.tensor(x).to(torch.bfloat16).cuda()
+    for relu in (0, 1):
+        y = torch.empty_like(xt)
+        yf = torch.empty(xt.shape, dtype=torch.float32, device="cuda")
+        assert L.cl_conv3d_fwd(P(xt), nb, *dims, P(net.wbf[0]), P(net.b[0]), relu, P(y), P(yf), None) == 0
+        yo_bf, yo = lo.conv_fwd(x, w, bias, relu, dims)
+        assert np.abs(yf.cpu().numpy() - yo).max() <= 1e-5 * np.abs(yo).max()
+        assert rel_l2(y.float().cpu().numpy(), yo_bf) < 4e-3
+    dy = _bf(rng.standard_normal((nb, nv, CH)))
+    dyt = torch.tensor(dy).to(torch.bfloat16).cuda()
+    dx = torch.empty_like(dyt)
+    assert L.cl_conv3d_bwd_data(P(dyt), nb, *dims, P(net.wbf[0]), P(dx), None) == 0
+    assert rel_l2(dx.float().cpu().numpy(), lo.conv_bwd_data(dy, w, dims)) < 4e-3
+    ws = torch.empty(L.cl_conv3d_bwd_weight_workspace(nb, *dims) // 4, dtype=torch.float32, device="cuda")
+    g = torch.empty(CH * 27 * CH + CH, dtype=torch.float32, device="cuda")
+    assert L.cl_conv3d_bwd_weight(P(xt), P(dyt), nb, *dims, P(g), P(ws), None) == 0
+    dW, db = lo.conv_bwd_weight(x, dy, dims)
+    gn = g.cpu().numpy()
+    assert rel_l2(gn[:CH * 27 * CH], dW.reshape(-1)) < 1e-5
+    assert rel_l2(gn[CH * 27 * CH:], db) < 1e-5
+    # ReLU backward and the loss gradient
+    yb = torch.tensor(_bf(rng.standard_normal((nb, nv, CH)))).to(torch.bfloat16).cuda()
+    d2 = dyt.clone()
+    assert L.cl_relu_bwd(P(yb), P(d2), d2.numel(), None) == 0
+    assert np.array_equal(d2.float().cpu().numpy(), lo.relu_bwd(yb.float().cpu().numpy(), dy))
+    yf = rng.standard_normal((nb, nv, CH)).astype(np.float32)
+    tg = rng.standard_normal((nb, nv, CH)).astype(np.float32)
+    loss, dyl = net.loss_grad(torch.tensor(yf).cuda(), torch.tensor(tg).cuda())
+    lo_loss, lo_dy = lo.mse_grad(yf, tg, learn.NOUT)
+    assert loss == pytest.approx(lo_loss / (nb * nv * learn.NOUT), rel=1e-5)
+    assert rel_l2(dyl.float().cpu().numpy(), lo_dy) < 4e-3
+    say("learned-net kernels match the CPU restatement (fwd, bwd data, bwd weight, relu, mse)")
+
+
+def test_gradient_through_network():
+    """The full forward and backward pass of the 4-layer network on the GPU
+    against the same chain composed from the CPU restatement (gradients of
+    every layer), and the last layer's gradient against a finite difference
+    (the loss is quadratic in the last layer's weights)."""
+    import torch
+    import learn_oracle as lo
+    from citcom_amd import learn
+    rng = np.random.default_rng(8)
+    dims = (5, 5, 5)
+    nb, nv = 2, 125
+    net = learn.ConvNet(seed=3)
+    x = np.zeros((nb, nv, 16), np.float32)
+    x[..., :learn.NIN] = rng.standard_normal((nb, nv, learn.NIN))
+    x = _bf(x)
+    tg = np.zeros((nb, nv, 16), np.float32)
+    tg[..., :learn.NOUT] = rng.standard_normal((nb, nv, learn.NOUT))
+    yf = net.forward(torch.tensor(x).to(torch.bfloat16).cuda(), dims)
+    _, dy = net.loss_grad(yf, torch.tensor(tg).cuda())
+    grads = [g.cpu().numpy() for g in net.backward(dy)]
+    W = [_bf(w.cpu().numpy()) for w in net.w]
+    B = [b.cpu().numpy() for b in net.b]
+    acts = [x]
+    for li in range(4):
+        a_bf, a_f = lo.conv_fwd(acts[-1], W[li], B[li], li < 3, dims)
+        acts.append(a_bf if li < 3 else a_f)
+    _, d = lo.mse_grad(acts[4], tg, learn.NOUT)
+    for li in reversed(range(4)):
+        if li < 3:
+            d = lo.relu_bwd(acts[li + 1], d)
+        dW, db = lo.conv_bwd_weight(acts[li], d, dims)
+        e = rel_l2(grads[li][:16 * 27 * 16], dW.reshape(-1))
+        say(f"layer {li}: rel L2 of dW against the CPU chain {e:.2e}")
+        assert e < 2e-2
+        if li > 0:
+            d = lo.conv_bwd_data(d, W[li], dims)
+
+    def loss_last(w3):
+        y = lo.conv_fwd(acts[3], w3, B[3], 0, dims)[1].astype(np.float64)
+        e = (y - tg)[..., :learn.NOUT]
+        return (e * e).sum() / (nb * nv * learn.NOUT)
+    g3 = grads[3][:16 * 27 * 16].reshape(16, 27, 16).astype(np.float64)
+    h = 1e-3 / max(np.abs(g3).max(), 1e-30)
+    fd = (loss_last(W[3] + h * g3) - loss_last(W[3] - h * g3)) / (2 * h)
+    an = float((g3 ** 2).sum())
+    say(f"last layer: directional derivative analytic {an:.5e}, finite difference {fd:.5e}")
+    assert fd == pytest.approx(an, rel=1e-2)
+
+
+def test_warm_start_gain():
+    """Train on the plain solver's solutions (learn16: 16^3, Ra 1e5, random
+    temperatures and contrasts 1e2..1e4) and use the network as the Uzawa
+    initial guess on held-out problems: the solve reaches the same criterion
+    (solution within the solver tolerance of the cold start's) in fewer
+    iterations."""
+    import torch
+    from citcom_amd import learn
+    from citcom_amd.hip import Citcom
+    t0 = time.time()
+    train_set = learn.make_samples("learn16", 96, seed=1)
+    test_set = learn.make_samples("learn16", 8, seed=2)
+    say(f"samples: {len(train_set)} training + {len(test_set)} test from the plain solver, {time.time() - t0:.1f} s")
+    p0 = learn.problem_with_contrast("learn16", 100.0)
+    dims = p0.dims(learn.net_level(p0))
+    t0 = time.time()
+    net, losses = learn.train(train_set, dims, epochs=300, batch=16, lr=2e-3, log=lambda m: say(m))
+    say(f"training {time.time() - t0:.1f} s, loss {losses[0]:.3e} -> {losses[-1]:.3e}")
+    assert losses[-1] < 0.1 * losses[0]
+
+    # throughput of the forward convolutions (MFMA)
+    xb = torch.zeros((64, dims[0] * dims[1] * dims[2], 16), dtype=torch.bfloat16, device="cuda")
+    net.forward(xb, dims)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        net.forward(xb, dims)
+    e1.record()
+    torch.cuda.synchronize()
+    flops = 20 * 4 * 2 * xb.shape[0] * xb.shape[1] * 27 * 16 * 16
+    say(f"forward: {flops / (e0.elapsed_time(e1) * 1e-3) / 1e12:.2f} TFLOP/s bf16 "
+        f"({flops / (e0.elapsed_time(e1) * 1e-3) / 2.5e15 * 100:.3f} % of the 2.5 PFLOP/s dense MFMA peak; "
+        f"{xb.shape[0]} volumes of {xb.shape[1]} nodes)")
+
+    cold_its, warm_its = [], []
+    for s in test_set:
+        p = learn.problem_with_contrast("learn16", s.contrast)
+        c = Citcom(p)
+        c.initialize(s.T)
+        st = c.general_stokes_solver()
+        Uc, Pc = c.download("U"), c.download("P")
+        c.close()
+        c = Citcom(p)
+        c.initialize(s.T)
+        learn.warm_start(net, c, p, s.T)
+        st_w = c.general_stokes_solver()
+        Uw, Pw = c.download("U"), c.download("P")
+        c.close()
+        cold_its.append(st["iterations"])
+        warm_its.append(st_w["iterations"])
+        # free-slip walls all round: the pressure is fixed up to a constant,
+        # which depends on the initial guess (as in the reference)
+        assert rel_l2(Uw, Uc) < 1e-4 and rel_l2(Pw - Pw.mean(), Pc - Pc.mean()) < 1e-4
+    say(f"Uzawa iterations on {len(test_set)} held-out problems: cold start {np.mean(cold_its):.1f} "
+        f"({cold_its}), learned warm start {np.mean(warm_its):.1f} ({warm_its})")
+    assert np.mean(warm_its) < np.mean(cold_its)
