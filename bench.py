@@ -241,7 +241,9 @@ def main():
     if plain is not None:
         config["plain_fp64_s_per_step"] = plain["s_per_step"]
         config["speedup_vs_plain_fp64"] = plain["s_per_step"] / s_per_step
-        config["learned_warm_start"] = "not built"
+        config["learned_warm_start"] = ("citcom_amd.learn (BF16 MFMA 3-D CNN) is the Uzawa initial guess where no "
+                                        "previous solution exists; in this time loop the previous step's U, P are "
+                                        "the initial guess, as in the reference, so it is not used here")
         log(f"plain FP64 form: {plain['s_per_step']:.3f} s/step; mixed form {s_per_step:.3f} s/step")
     out = {
         "metric": METRIC,
