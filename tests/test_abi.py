@@ -1,4 +1,4 @@
-"""The C ABI: every entry point include/citcom_hip.h declares is exported by
+"""The C ABI: every entry point include/*.h declares is exported by
 libcitcom_hip.so, and the library refuses to run without a GPU (no CPU
 fallback)."""
 import os
@@ -10,20 +10,22 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared():
-    text = open(os.path.join(ROOT, "include", "citcom_hip.h")).read()
+def declared(header="citcom_hip.h", prefix="chip_"):
+    text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(chip_[a-z_A-Z0-9]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(" + prefix + r"[a-z_A-Z0-9]+)\s*\(", text)))
 
 
 def test_header_symbols_exported():
     from citcom_amd import hip
     assert os.path.exists(hip.LIB_PATH), "build the library first"
     out = subprocess.run(["nm", "-D", "--defined-only", hip.LIB_PATH], capture_output=True, text=True, check=True).stdout
-    exported = set(re.findall(r"\bT (chip_\w+)", out))
+    exported = set(re.findall(r"\bT (\w+)", out))
     missing = [s for s in declared() if s not in exported]
     assert not missing, missing
     assert len(declared()) >= 20
+    learn = declared("citcom_learn.h", "cl_")
+    assert len(learn) >= 8 and not [s for s in learn if s not in exported]
 
 
 def test_no_cpu_fallback_without_device():
