@@ -741,6 +741,13 @@ int chip_assemble_forces(chip_ctx *c) {
 
 int chip_heat_flux(chip_ctx *c, float *nu) { return heat_flux_nu(c, nu); }
 
+int chip_vrms(chip_ctx *c, double *vrms_out) {
+    vrms(c->stream, c, c->V, c->dots + D_TMP);
+    if (sync_dots(c)) return -1;
+    *vrms_out = std::sqrt(c->h_dots[D_TMP] / c->h_dots[D_TMP + 1]);
+    return 0;
+}
+
 int chip_general_stokes_solver(chip_ctx *c, chip_stokes_stats *st) {
     const int r = stokes_step(c, st);
     return r;

@@ -96,6 +96,8 @@ void pg_residual(hipStream_t s, const chip_ctx *c, const float *T, const float *
 void pg_gather_correct(hipStream_t s, const chip_ctx *c, const double *Eres, float *T, float *Tdot, float mult);
 void conform_temperature(hipStream_t s, const chip_ctx *c, float *T);
 void thermal_buoyancy(hipStream_t s, const chip_ctx *c, const float *T, float *buoy);
+// volume integrals of v.v and of 1 (finest level) into out2[0], out2[1]
+void vrms(hipStream_t s, chip_ctx *c, const float *V, double *out2);
 void assemble_forces(hipStream_t s, const chip_ctx *c, const float *buoy, double *F);
 void heat_flux(hipStream_t s, chip_ctx *c, const float *T, const float *V);
 void fcopy(hipStream_t s, float *dst, const float *src, long n);

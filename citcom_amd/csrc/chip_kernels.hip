@@ -1779,6 +1779,51 @@ __global__ void __launch_bounds__(TPB) k_hf_elem(Geo g, const float *__restrict_
     uTe[e] = uT / ECO[(size_t)e * ECO_SZ + 12];
 }
 
+// Volume-RMS velocity (no counterpart in the reference): per element the Q1
+// quadrature of v.v and of 1 with the reference's shape functions and
+// Jacobian weights (Shape_function_dA), in FP64.
+__global__ void __launch_bounds__(TPB) k_vrms_elem(Geo g, const float *__restrict__ GDA, const float *__restrict__ V,
+                                                   double *__restrict__ num, double *__restrict__ den,
+                                                   double *__restrict__ ones) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.nel) return;
+    const int ez = e % g.elz, ex = (e / g.elz) % g.elx, ey = e / (g.elz * g.elx);
+    const float *gda = GDA + (size_t)e * GDA_SZ;
+    double Vn[3][9];
+#pragma unroll
+    for (int a = 1; a <= 8; a++) {
+        const int n = nid(g, ez + cOffz[a], ex + cOffx[a], ey + cOffy[a]);
+#pragma unroll
+        for (int d = 0; d < 3; d++) Vn[d][a] = V[d * g.nno + n];
+    }
+    double vv = 0.0, vol = 0.0;
+#pragma unroll
+    for (int i = 1; i <= 8; i++) {
+        double s2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            double u = 0.0;
+#pragma unroll
+            for (int j = 1; j <= 8; j++) u += Vn[d][j] * cNv[8 * (j - 1) + i - 1];
+            s2 += u * u;
+        }
+        vv += s2 * gda[i];
+        vol += gda[i];
+    }
+    num[e] = vv;
+    den[e] = vol;
+    ones[e] = 1.0;
+}
+
+void vrms(hipStream_t s, chip_ctx *c, const float *V, double *out2) {
+    const Level &L = c->L[c->levmax];
+    const long nel = L.g.nel;
+    double *num = c->Eres, *den = c->Eres + nel, *one = c->Eres + 2 * nel;
+    hipLaunchKernelGGL(k_vrms_elem, dim3(nblk(nel)), dim3(TPB), 0, s, L.g, L.GDA, V, num, den, one);
+    dots(s, c, (int)nel, 2, (const double *)num, (const double *)one, (const double *)den, (const double *)one,
+         nullptr, nullptr, nullptr, nullptr, out2);
+}
+
 __global__ void __launch_bounds__(TPB) k_hf_gather(Geo g, const float *__restrict__ TWW, const float *__restrict__ MASS,
                                                    const double *__restrict__ uTe, float *__restrict__ hf) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;

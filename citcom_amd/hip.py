@@ -95,6 +95,7 @@ def lib():
         L.chip_thermal_buoyancy.argtypes = [P]
         L.chip_assemble_forces.argtypes = [P]
         L.chip_heat_flux.argtypes = [P, C.POINTER(C.c_float)]
+        L.chip_vrms.argtypes = [P, C.POINTER(C.c_double)]
         L.chip_timestep.argtypes = [P, C.POINTER(PGStats), C.POINTER(StokesStats), C.POINTER(C.c_float)]
         L.chip_general_stokes_solver.argtypes = [P, C.POINTER(StokesStats)]
         L.chip_profile_enable.argtypes = [P, C.c_char_p, C.c_int]
@@ -228,6 +229,12 @@ class Citcom:
         st = PGStats()
         _check(lib().chip_PG_timestep(self.h, C.byref(st)), "PG_timestep")
         return st.as_dict()
+
+    def vrms(self) -> float:
+        """volume-RMS velocity of the current V (no reference routine)"""
+        v = C.c_double()
+        _check(lib().chip_vrms(self.h, C.byref(v)), "vrms")
+        return v.value
 
     def heat_flux(self) -> tuple[float, float]:
         nu = (C.c_float * 2)()

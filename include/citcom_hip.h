@@ -145,6 +145,11 @@ int chip_thermal_buoyancy(chip_ctx *c);
 int chip_assemble_forces(chip_ctx *c);
 /* heat_flux (Process_buoyancy.c:63-203): nu[0] = Nu_top, nu[1] = Nu_bot */
 int chip_heat_flux(chip_ctx *c, float *nu);
+/* volume-RMS velocity of the device V: sqrt(int v.v dV / int dV) with the
+ * 2x2x2 Gauss quadrature of the reference's shape functions and Jacobians
+ * (Shape_functions.c:53-170, Size_does_matter.c:60-179); the reference has
+ * no routine for it */
+int chip_vrms(chip_ctx *c, double *vrms);
 
 /* One timestep of the reference's time loop (Citcom.c:111-161) with every
  * stage on the device: PG_timestep, heat_flux, then general_stokes_solver

@@ -1846,6 +1846,38 @@ float orc_pg_timestep(orc *o, float *T, float *Tdot, const float *V, float *stat
 }
 
 /* heat_flux, Process_buoyancy.c:63-203: nu[0] = Nu_top, nu[1] = Nu_bot */
+/* volume-RMS velocity, sqrt(int v.v dV / int dV), 2x2x2 Gauss quadrature
+ * with the shape functions and dA of the finest level (no routine in the
+ * reference; the definition both sides use) */
+double orc_vrms(orc *o, const float *V)
+{
+	const int lev = o->levmax, nno = o->nno[lev];
+	double vv = 0.0, vol = 0.0;
+	int e, i, j, d;
+	for(e = 0; e < o->nel[lev]; e++)
+	{
+		int ez, ex, ey, nodes[9];
+		const float *gda = o->GDA[lev] + (size_t)e * GDA_SZ;
+		edecomp(o, lev, e, &ez, &ex, &ey);
+		for(j = 1; j <= 8; j++)
+			nodes[j] = enode(o, lev, ez, ex, ey, j);
+		for(i = 1; i <= 8; i++)
+		{
+			double s2 = 0.0;
+			for(d = 0; d < 3; d++)
+			{
+				double u = 0.0;
+				for(j = 1; j <= 8; j++)
+					u += (double)V[d * nno + nodes[j]] * o->Nv[GNV(j, i)];
+				s2 += u * u;
+			}
+			vv += s2 * gda[i];
+			vol += gda[i];
+		}
+	}
+	return sqrt(vv / vol);
+}
+
 void orc_heat_flux(orc *o, const float *T, const float *V, float *nu)
 {
 	const int lev = o->levmax, nno = o->nno[lev], noz = o->noz[lev], nox = o->nox[lev], noy = o->noy[lev];

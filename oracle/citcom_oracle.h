@@ -100,6 +100,7 @@ void orc_assemble_forces(orc *o, const float *buoy, double *F);
 void orc_visc_from_T(orc *o, const float *T, float *evi);
 /* one PG_timestep; T, Tdot in/out; V in (float[3][nno]); returns dt */
 float orc_pg_timestep(orc *o, float *T, float *Tdot, const float *V, float *stats);
+double orc_vrms(orc *o, const float *V);	/* volume-RMS velocity (no reference routine) */
 void orc_heat_flux(orc *o, const float *T, const float *V, float *nu);
 
 #ifdef __cplusplus

@@ -99,6 +99,8 @@ def lib():
         L.orc_pg_timestep.restype = C.c_float
         L.orc_pg_timestep.argtypes = [P, fp, fp, fp, fp]
         L.orc_heat_flux.argtypes = [P, fp, fp, fp]
+        L.orc_vrms.argtypes = [P, fp]
+        L.orc_vrms.restype = C.c_double
         _lib = L
     return _lib
 
@@ -234,6 +236,9 @@ class Oracle:
         stats = np.zeros(4, dtype=np.float32)
         dt = lib().orc_pg_timestep(self.h, T, Tdot, np.ascontiguousarray(V, dtype=np.float32), stats)
         return T, Tdot, float(dt), stats
+
+    def vrms(self, V):
+        return float(lib().orc_vrms(self.h, np.ascontiguousarray(V, dtype=np.float32)))
 
     def heat_flux(self, T, V):
         nu = np.zeros(2, dtype=np.float32)

@@ -142,6 +142,16 @@ def test_stokes_converged_parity(name):
     c.close()
 
 
+def test_vrms(pair):
+    """volume-RMS velocity of the reference's step-0 velocity: the GPU
+    reduction against the restatement's sequential sum"""
+    name, p, c, o, T = pair
+    g = golden(f"{name}_run.npz")
+    V = np.concatenate([g["step0_V1"], g["step0_V2"], g["step0_V3"]])
+    c.upload("V", V)
+    assert c.vrms() == pytest.approx(o.vrms(V), rel=1e-12)
+
+
 def test_pg_and_nu_against_reference(pair):
     """PG_timestep and heat_flux from the reference's own step-0 state: the
     energy step reproduces the reference bit for bit."""

@@ -21,7 +21,7 @@ pytestmark = pytest.mark.gpu
     ("cfg1@1e-8", "cfg1_at_1e-8_run.npz", 1e-6, 1e-5),
     ("cfg1", "cfg1_run.npz", 1e-5, 1e-5),
 ])
-def test_cfg1_200_steps(name, fixture, field_tol, nu_tol):
+def test_cfg1_200_steps(name, fixture, field_tol, nu_tol, pyoracle):
     from citcom_amd import deck
     from citcom_amd.hip import Citcom
     g = golden(fixture)
@@ -46,8 +46,14 @@ def test_cfg1_200_steps(name, fixture, field_tol, nu_tol):
     eT = rel_l2(c.download("T"), g["final_T"])
     eU = rel_l2(c.download("U"), g["final_U"])
     eP = rel_l2(c.download("P"), g["final_P"])
+    # volume-RMS velocity of the final state against the reference's final
+    # velocity (same quadrature on both sides)
+    o = pyoracle.Oracle(p)
+    vr_ref = o.vrms(pyoracle.v_from_vector(g["final_U"], c.nno()).reshape(-1))
+    evr = abs(c.vrms() / vr_ref - 1)
     print(f"[gpu-test] {name} 200 steps: step-0 U {e0:.2e}, max rel dNu {worst_nu:.2e}, max rel ddt {worst_dt:.2e}, "
-          f"final T {eT:.2e} U {eU:.2e} P {eP:.2e}", flush=True)
+          f"final T {eT:.2e} U {eU:.2e} P {eP:.2e}, Vrms {c.vrms():.6f} (ref {vr_ref:.6f}, rel {evr:.2e})", flush=True)
     c.close()
+    assert evr < nu_tol
     assert worst_nu < nu_tol
     assert eT < field_tol and eU < field_tol and eP < field_tol

@@ -109,3 +109,19 @@ def test_colour_smoother_fixed_point(tiny):
     x[f] = np.linalg.solve(A[np.ix_(f, f)], F[f])
     d, _ = o.gauss_seidel(lev, np.zeros_like(F), F, 4000, 0)
     assert np.linalg.norm(d - x) <= 1e-5 * np.linalg.norm(x)
+
+
+def test_vrms_closed_forms(tiny):
+    """Volume-RMS velocity: a uniform field gives its magnitude (partition of
+    unity), u = x gives sqrt(<x^2>) = Lx/sqrt(3) (2-point Gauss is exact for
+    the quadratic integrand on each element)."""
+    p, o = tiny
+    nno = o.nno()
+    V = np.zeros(3 * nno, np.float32)
+    V[:nno], V[nno:2 * nno], V[2 * nno:] = 0.3, -0.4, 1.2
+    assert o.vrms(V) == pytest.approx(np.sqrt(0.09 + 0.16 + 1.44), rel=1e-6)
+    X = o.get("X").astype(np.float32)
+    V[:] = 0.0
+    V[:nno] = X
+    Lx = float(X.max())
+    assert o.vrms(V) == pytest.approx(Lx / np.sqrt(3.0), rel=1e-6)
