@@ -112,17 +112,20 @@ def cpu_baseline_port(name, seed, mg_calls_per_step):
                        f"{mg_calls_per_step:.1f} multigrid calls per step (as counted on the GPU)")}
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed counter summary
-    (profiles/k_gs_colour_traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE
-    runs, scaled by what the same counters read for a copy of known size), or
-    None."""
+def measured_traffic(kernel, config):
+    """HBM bytes per launch of `kernel` on workload `config` from the committed
+    counter summary (profiles/k_gs_colour_traffic.json: rocprofv3 FETCH_SIZE
+    and WRITE_SIZE runs, scaled by what the same counters read for a copy of
+    known size), or None where it was not measured."""
     path = os.path.join(ROOT, "profiles", "k_gs_colour_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
-        per_kernel = json.load(fh).get("hbm_bytes_per_launch")
-    return per_kernel.get(kernel) if isinstance(per_kernel, dict) else None
+        rec = json.load(fh)
+    per_kernel = rec.get("hbm_bytes_per_launch")
+    if rec.get("config") != config or not isinstance(per_kernel, dict):
+        return None
+    return per_kernel.get(kernel)
 
 
 def cpu_baseline(name, seed, mg_calls_per_step):
@@ -231,8 +234,10 @@ def main():
         f"matvec: {n_mv} launches, {ms_mv:.1f} ms, {(bytes_mv / max(ms_mv, 1e-9)) / 1e6:.0f} GB/s")
     s_per_step = main_run["s_per_step"]
     kernel = "k_gs_colour<float>" if mixed else "k_gs_colour<double>"
+    visc = (f"viscosity exp(-{p.viscE[0]:.4g} T)" + (f", {p.num_mat} layers (visc0 {p.visc0[:p.num_mat]})"
+                                                      if p.num_mat > 1 else "")) if p.tdepv else "isoviscous"
     config = {"workload": f"{args.config}: {nox - 1}x{noy - 1}x{noz - 1} elements, {p.levels} MG levels, "
-                          f"Ra={p.rayleigh:g}, viscosity contrast 1e4, accuracy={p.accuracy:g}",
+                          f"Ra={p.rayleigh:g}, {visc}, accuracy={p.accuracy:g}",
               "form": ("FP32 multigrid cycles inside FP64 iterative refinement (FP64 residual, Uzawa CG, "
                        "coarsest level); FP32 stiffness as in the reference") if mixed else
                       "plain: FP64 vectors, the reference's FP32 stiffness",
@@ -260,7 +265,7 @@ def main():
         "data": "synthetic",
         "config": config,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(kernel), "kernel": kernel,
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(kernel, args.config), "kernel": kernel,
                      "launches": n_gs, "avg_launch_ms": avg_ms},
     }
     if not args.no_cpu_baseline:

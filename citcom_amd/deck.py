@@ -259,6 +259,7 @@ def workload(name: str) -> dict[str, object]:
     """Deck keys of a named workload.
 
     tiny / tiny_tdep  8x8x4 elements, 3 levels (test size)
+    tiny_layer  the same grid with two viscosity layers, contrast 1e5, clips
     cfg1  16^3 elements, 3 levels, isoviscous Ra=1e4, 200 steps
     cfg2  64^3 elements, 5 levels, Ra=1e6, eta = exp(-ln(1e4) T), 500 steps
     cfg3  128x128x64 elements in a 2x2x1 box, 6 levels, Ra=1e7,
@@ -283,6 +284,13 @@ def workload(name: str) -> dict[str, object]:
         d.update(_grid((1.0, 1.0, 0.5), (2, 2, 1), 3))
         d.update({"mgunitx": 2, "mgunity": 2, "mgunitz": 1, "levels": 3, "maxstep": 5, "finetunedt": 0.75})
         d.update(_tdep(1e5, 100.0))
+        d.update({"num_perturbations": 1, "perturbmag": 0.01, "perturbk": 1.0, "perturbl": 1.0, "perturbm": 0.0})
+    elif name == "tiny_layer":
+        # the viscosity features of the 128x128x64 and 256x256x128 workloads at
+        # test size: two material layers (top 0.1 x100), contrast 1e5, both clips
+        d.update(_grid((1.0, 1.0, 0.5), (2, 2, 1), 3))
+        d.update({"mgunitx": 2, "mgunity": 2, "mgunitz": 1, "levels": 3, "maxstep": 5, "finetunedt": 1.0})
+        d.update(_tdep(1e6, 1e5, vmin=1e-3, vmax=10.0, num_mat=2, visc0="100,1", z_layer="0.4,0"))
         d.update({"num_perturbations": 1, "perturbmag": 0.01, "perturbk": 1.0, "perturbl": 1.0, "perturbm": 0.0})
     elif name == "cfg1":
         d.update(_grid((1.0, 1.0, 1.0), (4, 4, 4), 3))

@@ -15,7 +15,7 @@ import refdump
 
 pytestmark = pytest.mark.gpu
 
-DECKS = ["tiny", "tiny_tdep"]
+DECKS = ["tiny", "tiny_tdep", "tiny_layer"]
 
 
 def say(*a):

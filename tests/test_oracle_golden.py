@@ -16,7 +16,7 @@ import pytest
 from conftest import golden, problem, rel_l2
 import refdump
 
-DECKS = ["tiny", "tiny_tdep"]
+DECKS = ["tiny", "tiny_tdep", "tiny_layer"]
 
 
 def sha(a):
