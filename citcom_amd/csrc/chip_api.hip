@@ -117,34 +117,38 @@ void do_matvec(chip_ctx *c, int lev, const V *u, V *Au, const V *rhs = nullptr, 
 }
 
 // gauss_seidel (General_matrix_functions.c:1160-1361), 8-colour; returns Ad = K d
+// (with res: also res = rhs - Ad, formed by the final matvec)
 template <typename V>
-void do_gs(chip_ctx *c, int lev, V *d, const V *F, V *Ad, int sweeps, int guess);
+void do_gs(chip_ctx *c, int lev, V *d, const V *F, V *Ad, int sweeps, int guess, const V *rhs = nullptr,
+           V *res = nullptr);
 
 inline void copy_v(hipStream_t s, double *d, const double *a, long n) { copy(s, d, a, n); }
 inline void copy_v(hipStream_t s, float *d, const float *a, long n) { fcopy(s, d, a, n); }
 
 // a smoother call of multi_grid: the affine form where the level has one
 template <typename V>
-void mg_gs(chip_ctx *c, int lev, V *d, const V *F, V *Ad, int sweeps, int guess) {
+void mg_gs(chip_ctx *c, int lev, V *d, const V *F, V *Ad, int sweeps, int guess, const V *rhs = nullptr,
+           V *res = nullptr) {
     const Level &L = c->L[lev];
     if (L.Ma && c->small_levels && sweeps == L.affine_sweeps) {
         V *out;
         if constexpr (sizeof(V) == sizeof(double)) out = L.tmp;
         else out = L.tmpf;
         affine_apply(S(c), L, guess ? (const V *)d : nullptr, F, out);
-        do_matvec(c, lev, (const V *)out, Ad);
+        do_matvec(c, lev, (const V *)out, Ad, rhs, res);
         copy_v(S(c), d, (const V *)out, 3L * L.g.nno);
         return;
     }
-    do_gs(c, lev, d, F, Ad, sweeps, guess);
+    do_gs(c, lev, d, F, Ad, sweeps, guess, rhs, res);
 }
 
 template <typename V>
-void do_gs(chip_ctx *c, int lev, V *d, const V *F, V *Ad, int sweeps, int guess) {
+void do_gs(chip_ctx *c, int lev, V *d, const V *F, V *Ad, int sweeps, int guess, const V *rhs, V *res) {
     const Level &L = c->L[lev];
     if constexpr (sizeof(V) == sizeof(double)) {
         if (c->small_levels && gs_small_kind(L)) {
             gs_small(S(c), L, d, F, Ad, sweeps, guess);
+            if (res) sub(S(c), res, rhs, Ad, 3L * L.g.nno);
             return;
         }
     }
@@ -155,7 +159,7 @@ void do_gs(chip_ctx *c, int lev, V *d, const V *F, V *Ad, int sweeps, int guess)
             ProfGuard pg(c, 1, gs_bytes<V>() * L.g.cn[col]);
             gs_colour(S(c), L, BI, col, d, F, pg.a, pg.b);
         }
-    do_matvec(c, lev, (const V *)d, Ad);
+    do_matvec(c, lev, (const V *)d, Ad, rhs, res);
 }
 
 // multi_grid (General_matrix_functions.c:525-653).  F holds the right-hand
@@ -192,8 +196,8 @@ void enqueue_cycles(chip_ctx *c, V *F) {
                 const LV<V> v = vecs<V>(L[dlev]);
                 const int cycles = (dlev == levmax) ? p.v_steps_high : p.down_heavy;
                 V *rhs = (dlev == lev) ? fl(dlev) : v.fl;
-                mg_gs(c, dlev, v.vel, (const V *)rhs, v.AU, cycles, dlev == lev);
-                sub(S(c), v.res, rhs, v.AU, 3L * L[dlev].g.nno);
+                // res = rhs - AU formed by the smoother's final matvec
+                mg_gs(c, dlev, v.vel, (const V *)rhs, v.AU, cycles, dlev == lev, (const V *)rhs, v.res);
                 project(S(c), L[dlev], L[dlev - 1], (const V *)v.res, vecs<V>(L[dlev - 1]).fl);
             }
             coarse_smooth(c, vecs<V>(L[levmin]).vel, (const V *)vecs<V>(L[levmin]).fl);

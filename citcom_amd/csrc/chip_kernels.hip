@@ -856,7 +856,9 @@ void strip(hipStream_t s, const Level &L, double *v) {
 // sums the partials in block order (deterministic for a given size)
 
 constexpr int DOT_TPB = 256;
-constexpr int DOT_MAXB = 1024;
+// at most 256 workgroups: the last-workgroup handoff is one atomic per
+// workgroup on a single counter
+constexpr int DOT_MAXB = 256;
 
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
@@ -891,15 +893,28 @@ __global__ void __launch_bounds__(DOT_TPB) k_dots(int n, const V *__restrict__ a
                                                   double *partial, unsigned *counter, double *out) {
     __shared__ double sh[4 * 16];
     __shared__ bool last;
-    double v[ND];
+    // four independent partial sums per thread (u = 0..3 over consecutive
+    // grid strides), combined as (s0 + s1) + (s2 + s3)
+    double v[ND], w[ND][4];
 #pragma unroll
-    for (int m = 0; m < ND; m++) v[m] = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        v[0] += (double)a0[i] * (double)b0[i];
-        if (ND > 1) v[1] += (double)a1[i] * (double)b1[i];
-        if (ND > 2) v[2] += (double)a2[i] * (double)b2[i];
-        if (ND > 3) v[3] += (double)a3[i] * (double)b3[i];
+    for (int m = 0; m < ND; m++)
+#pragma unroll
+        for (int u = 0; u < 4; u++) w[m][u] = 0.0;
+    const int stride = gridDim.x * blockDim.x;
+    for (int i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += 4 * stride) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int i = i0 + u * stride;
+            if (i < n) {
+                w[0][u] += (double)a0[i] * (double)b0[i];
+                if (ND > 1) w[1][u] += (double)a1[i] * (double)b1[i];
+                if (ND > 2) w[2][u] += (double)a2[i] * (double)b2[i];
+                if (ND > 3) w[3][u] += (double)a3[i] * (double)b3[i];
+            }
+        }
     }
+#pragma unroll
+    for (int m = 0; m < ND; m++) v[m] = (w[m][0] + w[m][1]) + (w[m][2] + w[m][3]);
     block_sum<ND>(v, sh);
     if (threadIdx.x == 0) {
 #pragma unroll
