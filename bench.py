@@ -187,7 +187,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="cfg3")
+    ap.add_argument("--config", default="tdep128")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--form", choices=("mixed", "plain"), default="mixed",
                     help="mixed: FP32 multigrid inside FP64 refinement; plain: the reference's FP64 arithmetic")

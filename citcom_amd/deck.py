@@ -260,15 +260,15 @@ def workload(name: str) -> dict[str, object]:
 
     tiny / tiny_tdep  8x8x4 elements, 3 levels (test size)
     tiny_layer  the same grid with two viscosity layers, contrast 1e5, clips
-    cfg1  16^3 elements, 3 levels, isoviscous Ra=1e4, 200 steps
-    cfg2  64^3 elements, 5 levels, Ra=1e6, eta = exp(-ln(1e4) T), 500 steps
-    cfg3  128x128x64 elements in a 2x2x1 box, 6 levels, Ra=1e7,
+    iso16  16^3 elements, 3 levels, isoviscous Ra=1e4, 200 steps
+    tdep64  64^3 elements, 5 levels, Ra=1e6, eta = exp(-ln(1e4) T), 500 steps
+    tdep128  128x128x64 elements in a 2x2x1 box, 6 levels, Ra=1e7,
           eta = exp(-ln(1e4) T) clipped to [1e-4, 1], CFL 0.5
-    cfg5  256x256x128 elements, 7 levels, Ra=1e7, contrast 1e5, top layer x100
+    layer256  256x256x128 elements, 7 levels, Ra=1e7, contrast 1e5, top layer x100
     learn16  16^3 elements, 3 levels, Ra=1e5, eta = exp(-ln(1e3) T): the
           training and test problems of the learned warm start
 
-    ``name@acc`` is the same workload with ``accuracy=acc`` (e.g. cfg1@1e-8).
+    ``name@acc`` is the same workload with ``accuracy=acc`` (e.g. iso16@1e-8).
     """
     if "@" in name:
         base, acc = name.split("@", 1)
@@ -292,15 +292,15 @@ def workload(name: str) -> dict[str, object]:
         d.update({"mgunitx": 2, "mgunity": 2, "mgunitz": 1, "levels": 3, "maxstep": 5, "finetunedt": 1.0})
         d.update(_tdep(1e6, 1e5, vmin=1e-3, vmax=10.0, num_mat=2, visc0="100,1", z_layer="0.4,0"))
         d.update({"num_perturbations": 1, "perturbmag": 0.01, "perturbk": 1.0, "perturbl": 1.0, "perturbm": 0.0})
-    elif name == "cfg1":
+    elif name == "iso16":
         d.update(_grid((1.0, 1.0, 1.0), (4, 4, 4), 3))
         d.update({"mgunitx": 4, "mgunity": 4, "mgunitz": 4, "levels": 3, "maxstep": 200, "finetunedt": 0.75})
         d.update(_iso(1e4))
-    elif name == "cfg2":
+    elif name == "tdep64":
         d.update(_grid((1.0, 1.0, 1.0), (4, 4, 4), 5))
         d.update({"mgunitx": 4, "mgunity": 4, "mgunitz": 4, "levels": 5, "maxstep": 500, "finetunedt": 0.75})
         d.update(_tdep(1e6, 1e4))
-    elif name == "cfg3":
+    elif name == "tdep128":
         d.update(_grid((2.0, 2.0, 1.0), (4, 4, 2), 6))
         d.update({"mgunitx": 4, "mgunity": 4, "mgunitz": 2, "levels": 6, "maxstep": 10, "finetunedt": 1.0})
         d.update(_tdep(1e7, 1e4, vmin=1e-4, vmax=1.0))
@@ -308,7 +308,7 @@ def workload(name: str) -> dict[str, object]:
         d.update(_grid((1.0, 1.0, 1.0), (4, 4, 4), 3))
         d.update({"mgunitx": 4, "mgunity": 4, "mgunitz": 4, "levels": 3, "maxstep": 1, "finetunedt": 0.75})
         d.update(_tdep(1e5, 1e3))
-    elif name == "cfg5":
+    elif name == "layer256":
         d.update(_grid((2.0, 2.0, 1.0), (4, 4, 2), 7))
         d.update({"mgunitx": 4, "mgunity": 4, "mgunitz": 2, "levels": 7, "maxstep": 5, "finetunedt": 1.0})
         d.update(_tdep(1e7, 1e5, num_mat=2, visc0="100,1", z_layer="0.9,0"))

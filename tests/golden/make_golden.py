@@ -13,7 +13,7 @@ citcom_amd.deck and stores what the tests compare against:
   harness's seeded inputs (regenerated in Python by ``xorshift``).
 * ``<deck>_run.npz``    the first Stokes solve, the first PG step and the
   per-step (time, dt, Nu_top, Nu_bot) of a short run.
-* ``cfg1_run.npz``      the 16^3 isoviscous Ra=1e4 workload: 200 steps.
+* ``iso16_run.npz``      the 16^3 isoviscous Ra=1e4 workload: 200 steps.
 * ``<deck>_at_<acc>_run.npz``  the same runs with accuracy = acc: two solvers
   that stop on relative-increment tests at 1e-6 can differ by O(1e-6), so
   field parity at 1e-6 is shown where both are converged well below it.
@@ -107,7 +107,7 @@ def main(names):
         raise SystemExit(f"{HARNESS} missing: make -C oracle ref")
     with tempfile.TemporaryDirectory() as tmp:
         for name in names:
-            if name.startswith("cfg1"):
+            if name.startswith("iso16"):
                 d = run_harness(name, tmp, "run", "200")
                 tag = name.replace("@", "_at_")
                 np.savez_compressed(os.path.join(HERE, f"{tag}_run.npz"), **run_fixture(d, ("U.f64", "P.f64", "T.f32")))
@@ -127,4 +127,4 @@ def main(names):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:] or ["tiny", "tiny_tdep", "tiny@1e-9", "tiny_tdep@1e-9", "cfg1", "cfg1@1e-8"])
+    main(sys.argv[1:] or ["tiny", "tiny_tdep", "tiny@1e-9", "tiny_tdep@1e-9", "iso16", "iso16@1e-8"])

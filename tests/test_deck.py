@@ -10,7 +10,7 @@ from conftest import golden, problem
 
 def test_parse_roundtrip(tmp_path):
     from citcom_amd import deck
-    keys = deck.workload("cfg3")
+    keys = deck.workload("tdep128")
     path = tmp_path / "d.input"
     deck.write_deck(str(path), keys)
     back = deck.read_deck(str(path))
@@ -41,7 +41,7 @@ def test_coordinates_match_reference():
 
 def test_seeded_temperature():
     from citcom_amd import deck
-    p = problem("cfg2")
+    p = problem("tdep64")
     T = deck.seeded_temperature(p, seed=3)
     nox, noy, noz = p.dims()
     Tv = T.reshape(noy, nox, noz)

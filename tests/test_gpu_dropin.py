@@ -41,7 +41,7 @@ def run_wrapped(name: str, steps: int, tmp: str) -> str:
 @pytest.mark.skipif(not os.path.exists(HARNESS_HIP), reason="oracle/_ref/ref_harness_hip not built")
 @pytest.mark.parametrize("name,fixture,steps,field_tol,nu_tol", [
     ("tiny_tdep@1e-9", "tiny_tdep_at_1e-9_run.npz", 3, 1e-6, 1e-6),
-    ("cfg1@1e-8", "cfg1_at_1e-8_run.npz", 200, 1e-6, 1e-5),
+    ("iso16@1e-8", "iso16_at_1e-8_run.npz", 200, 1e-6, 1e-5),
 ])
 def test_wrapped_reference_run(name, fixture, steps, field_tol, nu_tol):
     g = golden(fixture)
@@ -87,9 +87,9 @@ def test_wrapped_program_outputs():
     from citcom_amd import outputs
     with tempfile.TemporaryDirectory() as ta, tempfile.TemporaryDirectory() as tb:
         print("[gpu-test] pristine reference program, 20 steps", flush=True)
-        da = run_program(PROGRAM_REF, "cfg1@1e-8", 20, ta)
+        da = run_program(PROGRAM_REF, "iso16@1e-8", 20, ta)
         print("[gpu-test] reference program with the GPU drop-in, 20 steps", flush=True)
-        db = run_program(PROGRAM_HIP, "cfg1@1e-8", 20, tb)
+        db = run_program(PROGRAM_HIP, "iso16@1e-8", 20, tb)
         va = outputs.read_velo(os.path.join(da, "run.velo.0.20"))
         vb = outputs.read_velo(os.path.join(db, "run.velo.0.20"))
         ta_ = outputs.read_temp(os.path.join(da, "run.temp.0.20"))

@@ -46,13 +46,13 @@ def test_mixed_stokes_against_reference(name, fixture, tol):
     assert eu < tol and ep < tol
 
 
-def test_mixed_cfg1_trajectory():
+def test_mixed_iso16_trajectory():
     """16^3, Ra 1e4, accuracy 1e-8: 200 steps in the mixed form against the
     reference's run (T, U, P to 1e-6, Nu to 1e-5)."""
     from citcom_amd import deck
     from citcom_amd.hip import Citcom
-    p = problem("cfg1@1e-8")
-    g = golden("cfg1_at_1e-8_run.npz")
+    p = problem("iso16@1e-8")
+    g = golden("iso16_at_1e-8_run.npz")
     c = Citcom(p, mixed=True)
     c.initialize(deck.perturbed_temperature(p))
     c.general_stokes_solver()
@@ -66,6 +66,6 @@ def test_mixed_cfg1_trajectory():
     eU = rel_l2(c.download("U"), g["final_U"])
     eP = rel_l2(c.download("P"), g["final_P"])
     c.close()
-    say(f"cfg1@1e-8 mixed 200 steps: max rel dNu {worst_nu:.2e}, final T {eT:.2e} U {eU:.2e} P {eP:.2e}")
+    say(f"iso16@1e-8 mixed 200 steps: max rel dNu {worst_nu:.2e}, final T {eT:.2e} U {eU:.2e} P {eP:.2e}")
     assert worst_nu < 1e-5
     assert eT < 1e-6 and eU < 1e-6 and eP < 1e-6

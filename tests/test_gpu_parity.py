@@ -5,7 +5,7 @@ Operators that the reference computes deterministically are compared bit for
 bit: geometry, viscosity on every level, stiffness rows, BI, BPI, elt_del,
 forces, grid transfers, div/grad and the 8-colour smoother (against the
 restatement's 8-colour variant).  Solutions are compared to the solver
-tolerance (relative L2 < 1e-6).
+tolerance (relative L2 < 1e-6 on the fields).
 """
 import numpy as np
 import pytest
@@ -212,11 +212,11 @@ def test_graph_replay_matches_eager():
         assert np.array_equal(out[0][k], out[1][k]) and np.array_equal(out[2][k], out[1][k])
 
 
-@pytest.mark.parametrize("name", ["tiny_tdep", "cfg1"])
+@pytest.mark.parametrize("name", ["tiny_tdep", "iso16"])
 @pytest.mark.parametrize("small", [1, 0])
 def test_smoother_kernels_bitwise(name, small, pyoracle):
     """The three smoother implementations (whole level in LDS, one workgroup
-    with the stiffness in L2, one launch per colour pass; cfg1's levels use
+    with the stiffness in L2, one launch per colour pass; iso16's levels use
     one each) give the restatement's 8-colour iterate bit for bit, and so do
     per-colour launches on every level (small_levels = 0)."""
     from citcom_amd import deck

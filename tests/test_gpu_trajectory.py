@@ -1,5 +1,5 @@
 """The 16^3 isoviscous Ra=1e4 workload (200 timesteps) on the GPU against the
-reference's own runs (tests/golden/cfg1*_run.npz): per-step Nusselt number and
+reference's own runs (tests/golden/iso16*_run.npz): per-step Nusselt number and
 timestep, and the final T, U, P fields.
 
 At accuracy 1e-8 both codes are converged far below the 1e-6 bar and the
@@ -18,10 +18,10 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("name,fixture,field_tol,nu_tol", [
-    ("cfg1@1e-8", "cfg1_at_1e-8_run.npz", 1e-6, 1e-5),
-    ("cfg1", "cfg1_run.npz", 1e-5, 1e-5),
+    ("iso16@1e-8", "iso16_at_1e-8_run.npz", 1e-6, 1e-5),
+    ("iso16", "iso16_run.npz", 1e-5, 1e-5),
 ])
-def test_cfg1_200_steps(name, fixture, field_tol, nu_tol, pyoracle):
+def test_iso16_200_steps(name, fixture, field_tol, nu_tol, pyoracle):
     from citcom_amd import deck
     from citcom_amd.hip import Citcom
     g = golden(fixture)

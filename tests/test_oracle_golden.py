@@ -94,13 +94,13 @@ def test_stokes_pg_nu_against_reference(pyoracle, name):
     assert np.array_equal(Td1, g["pg1_Tdot"])
 
 
-def test_cfg1_first_steps(pyoracle):
+def test_iso16_first_steps(pyoracle):
     """16^3 isoviscous Ra=1e4: the first five steps of the time loop."""
     from citcom_amd import deck
-    p = problem("cfg1")
+    p = problem("iso16")
     o = pyoracle.Oracle(p)
     o.set_viscosity(o.visc_from_T(deck.perturbed_temperature(p)))
-    g = golden("cfg1_run.npz")
+    g = golden("iso16_run.npz")
     T = deck.perturbed_temperature(p)
     assert np.array_equal(T, g["step0_T"])
     nno, nel = o.nno(), o.nel()
