@@ -15,8 +15,10 @@ timed beside it for the ratio.
 
 Prints one JSON line: s/timestep (lower is better), the roofline of the
 dominant kernel (the 8-colour Gauss-Seidel pass) from HIP events on the
-library's stream, and a CPU baseline: the unmodified reference (or, where its
-build is absent, the C restatement) timed on the host on a bounded sample.
+library's stream, and a CPU baseline: the unmodified reference program under
+MPI on up to 16 host cores (or, where MPI or that build is absent, the
+reference's objects on one core, else the C restatement) timed on a bounded
+sample.
 """
 from __future__ import annotations
 
@@ -47,6 +49,7 @@ def make_problem(name):
 
 
 REF_HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+REF_PROGRAM = os.path.join(ROOT, "oracle", "_ref", "citcom.mpi")
 
 
 def cpu_baseline_reference(name, seed, mg_calls_per_step):
@@ -128,12 +131,106 @@ def measured_traffic(kernel, config):
     return per_kernel.get(kernel)
 
 
+def mpi_decomposition(p, cap):
+    """Largest rank count <= cap that the reference accepts for this grid
+    (mgunit{x,y,z} % nproc{x,y,z} == 0), least subdomain surface first."""
+    best = None
+    mx, my, mz = p.mgunit
+    for px in range(1, mx + 1):
+        for py in range(1, my + 1):
+            for pz in range(1, mz + 1):
+                if mx % px or my % py or mz % pz or px * py * pz > cap:
+                    continue
+                nox, noy, noz = p.dims()
+                ex, ey, ez = (nox - 1) // px, (noy - 1) // py, (noz - 1) // pz
+                key = (px * py * pz, -(ex * ey + ey * ez + ex * ez))
+                if best is None or key > best[0]:
+                    best = (key, (px, py, pz))
+    return best[1]
+
+
+def cpu_baseline_mpi(name, seed, mg_calls_per_step, cap=16, iterations=(4, 14)):
+    """Time the pristine reference program (oracle/_ref/citcom.mpi, built
+    from the unmodified sources) under MPI on the host cores, on the same
+    grid, deck and seeded temperature (one restart file per rank).
+
+    Two short runs (maxstep 2, piterations = iterations[0], iterations[1])
+    give step 2's wall time t and multigrid calls m from the reference's own
+    log (Citcom.c:151-158; General_matrix_functions.c:494-498); the fit
+    t = a + b m separates the per-step fixed work (PG_timestep, forces,
+    viscosity, stiffness rebuild, heat flux) from the cost of one multigrid
+    call, and the estimate is a + b x the multigrid calls per step counted on
+    the GPU."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+
+    mpiexec = shutil.which("mpiexec") or "/opt/conda/bin/mpiexec"
+    if not (os.path.exists(REF_PROGRAM) and os.path.exists(mpiexec)):
+        raise RuntimeError("no reference program or mpiexec")
+    p = make_problem(name)
+    try:
+        cap = min(cap, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        cap = min(cap, os.cpu_count() or 1)
+    nproc = mpi_decomposition(p, cap)
+    ranks = nproc[0] * nproc[1] * nproc[2]
+    T = deck.seeded_temperature(p, seed=seed)
+    samples = []
+    with tempfile.TemporaryDirectory() as tmp:
+        deck.write_restart_temperature(os.path.join(tmp, "seed"), T, p.dims(), nproc)
+        for its in iterations:
+            keys = deck.workload(name)
+            out = os.path.join(tmp, f"run{its}")
+            os.makedirs(out)
+            keys.update({"datafile": os.path.join(out, "run"), "restart": 1, "restart_timesteps": 0,
+                         "oldfile": os.path.join(tmp, "seed"), "maxstep": 2, "piterations": its,
+                         "nprocx": nproc[0], "nprocy": nproc[1], "nprocz": nproc[2]})
+            path = os.path.join(out, "deck.input")
+            deck.write_deck(path, keys)
+            t0 = time.perf_counter()
+            r = subprocess.run([mpiexec, "-launcher", "fork", "-n", str(ranks), REF_PROGRAM, path],
+                               capture_output=True, text=True, timeout=900, cwd=out)
+            text = open(os.path.join(out, "run.log0")).read()
+            if "for step 2" not in text:
+                raise RuntimeError(f"reference run failed (rc {r.returncode}): {r.stderr[-400:]}")
+            step2 = text[text.index("for step 1"):text.index("for step 2")]
+            t = float(re.findall(r"& CPU = ([0-9.eE+-]+) for step 2", text)[0])
+            m = sum(int(a) for a in re.findall(r"residual \((\d+)\)\(\d+\)", step2))
+            samples.append((its, t, m))
+            log(f"reference under MPI ({ranks} ranks {nproc}): piterations {its}: step 2 {t:.2f} s, {m} multigrid "
+                f"calls (run {time.perf_counter() - t0:.0f} s)")
+    (_, t1, m1), (_, t2, m2) = samples
+    b = (t2 - t1) / (m2 - m1)
+    a = t1 - b * m1
+    est = a + b * mg_calls_per_step
+    return {"value": est, "unit": "s/timestep", "cores": ranks, "kind": "reference",
+            "sample": (f"pristine reference program (oracle/_ref/citcom.mpi, gcc -O2), {ranks} MPI ranks "
+                       f"(nprocx,y,z = {nproc[0]},{nproc[1]},{nproc[2]}) at {name} size on the seeded temperature: "
+                       f"step 2 of two runs with piterations {iterations[0]} and {iterations[1]} "
+                       f"({t1:.2f} s / {m1} multigrid calls, {t2:.2f} s / {m2}); fit: {a:.2f} s per step + "
+                       f"{b:.3f} s per multigrid call x {mg_calls_per_step:.1f} multigrid calls per step "
+                       f"(as counted on the GPU)")}
+
+
 def cpu_baseline(name, seed, mg_calls_per_step):
+    one_rank = None
     if os.path.exists(REF_HARNESS):
         try:
-            return cpu_baseline_reference(name, seed, mg_calls_per_step)
+            one_rank = cpu_baseline_reference(name, seed, mg_calls_per_step)
         except Exception as e:  # noqa: BLE001 - report and fall back to the restatement
-            log(f"reference timing sample unavailable ({e}); timing the C restatement")
+            log(f"reference timing sample unavailable ({e})")
+    try:
+        out = cpu_baseline_mpi(name, seed, mg_calls_per_step)
+        if one_rank:
+            out["sample"] += f"; one rank: {one_rank['value']:.1f} s/timestep ({one_rank['sample']})"
+        return out
+    except Exception as e:  # noqa: BLE001 - report and fall back
+        log(f"reference MPI timing unavailable ({e})")
+    if one_rank:
+        return one_rank
+    log("timing the C restatement")
     return cpu_baseline_port(name, seed, mg_calls_per_step)
 
 

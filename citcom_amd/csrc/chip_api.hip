@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <string>
 
@@ -285,6 +286,8 @@ int solve_del2_u(chip_ctx *c, double *d0, const double *F, double acc, int *vali
             return -1;
         }
     }
+    if (c->print_convergence)
+        std::fprintf(stderr, " *  residual (%03d) = %.3e from %.3e to %.3e\n", cnt, residual, r0, acc);
     *count = cnt;
     return 0;
 }
@@ -350,6 +353,9 @@ int uzawa(chip_ctx *c, chip_stokes_stats *st) {
         s.vdotv = std::sqrt(vdotv / gneq);
         s.pdotp = std::sqrt(pdotp / gnpno);
         count++;
+        if (c->print_convergence)   // generate_log_message, Stokes_flow_Incomp.c:501-526
+            std::fprintf(stderr, "AhatP (%03d) with v %12.6e dv/v %9.3e div/v %9.3e  p %12.6e dp/p %9.3e\n", count,
+                         s.vdotv, dvelocity, s.incompressibility, s.pdotp, dpressure);
         sh = s1; s1 = s2; s2 = sh;
         sh = r0; r0 = r1; r1 = r2; r2 = sh;
         sh = z0; z0 = z1; z1 = sh;
@@ -795,6 +801,10 @@ int chip_set_option(chip_ctx *c, const char *name, int value) {
     const std::string n(name ? name : "");
     if (n == "graphs") {
         c->use_graphs = value != 0;
+        return 0;
+    }
+    if (n == "print_convergence") {
+        c->print_convergence = value != 0;
         return 0;
     }
     if (n == "small_levels") {

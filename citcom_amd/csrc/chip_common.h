@@ -120,6 +120,7 @@ struct chip_ctx {
     bool built = false;      // operators constructed at least once
     bool use_graphs = true;  // replay multi_grid as a captured HIP graph
     bool small_levels = true;  // single-workgroup smoother on small levels
+    bool print_convergence = false;  // per-iteration lines on stderr, as E->control.print_convergence
     hipGraphExec_t mg_graph = nullptr;
     chip::Prof prof;
     long dev_bytes = 0;

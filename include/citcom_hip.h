@@ -171,7 +171,10 @@ long chip_device_bytes(chip_ctx *c);
  * graph, 0 launches its kernels one by one; results are identical.
  * "small_levels": 1 (default) smooths levels whose stiffness fits in LDS in
  * one single-workgroup kernel per gauss_seidel call, 0 launches one kernel
- * per colour pass on every level; results are identical. */
+ * per colour pass on every level; results are identical.
+ * "print_convergence": 1 prints one line per Uzawa iteration and per inner
+ * solve on stderr (generate_log_message, Stokes_flow_Incomp.c:501-526;
+ * General_matrix_functions.c:494-498). */
 int chip_set_option(chip_ctx *c, const char *name, int value);
 
 #ifdef __cplusplus

@@ -212,6 +212,33 @@ def test_graph_replay_matches_eager():
         assert np.array_equal(out[0][k], out[1][k]) and np.array_equal(out[2][k], out[1][k])
 
 
+def test_print_convergence(capfd):
+    """print_convergence writes one AhatP line per Uzawa iteration and one
+    residual line per inner solve (Stokes_flow_Incomp.c:501-526,
+    General_matrix_functions.c:494-498), and changes no result."""
+    from citcom_amd import deck
+    from citcom_amd.hip import Citcom
+    p = problem("tiny_tdep")
+    T = deck.perturbed_temperature(p)
+    res = []
+    for flag in (0, 1):
+        c = Citcom(p)
+        c.set_option("print_convergence", flag)
+        c.initialize(T)
+        capfd.readouterr()
+        st = c.general_stokes_solver()
+        err = capfd.readouterr().err
+        res.append((c.download("U"), st))
+        c.close()
+    lines = err.splitlines()
+    st = res[1][1]
+    assert sum(ln.startswith("AhatP (") for ln in lines) == st["iterations"]
+    assert sum("residual (" in ln for ln in lines) == st["inner_solves"]
+    last = [ln for ln in lines if ln.startswith("AhatP (")][-1]
+    assert f"dp/p {st['dpressure']:9.3e}" in last
+    assert np.array_equal(res[0][0], res[1][0])
+
+
 @pytest.mark.parametrize("name", ["tiny_tdep", "iso16"])
 @pytest.mark.parametrize("small", [1, 0])
 def test_smoother_kernels_bitwise(name, small, pyoracle):
