@@ -1100,7 +1100,7 @@ void conform_velocity(hipStream_t s, const Level &L, double *U) { strip(s, L, U)
 // ---------------------------------------------------------------------------
 // stiffness rebuild
 
-// visc_from_T (Viscosity_structures.c:491-737, rheol 0 and 3) or
+// visc_from_T (Viscosity_structures.c:491-737, rheol 0, 3 and 11) or
 // visc_from_mat (:477-489), then the MAX / MIN clip of get_system_viscosity
 struct ViscLaw {
     int rheol, tdepv, vmin_on, vmax_on;
@@ -1126,10 +1126,18 @@ __global__ void __launch_bounds__(TPB) k_visc_from_T(Geo g, ViscLaw law, const i
             float temp = 1.0e-32f;
 #pragma unroll
             for (int kk = 1; kk <= 8; kk++) temp = (float)(temp + fmaxf(0.0f, TT[kk]) * cNv[8 * (kk - 1) + jj - 1]);
-            float arg;
-            if (law.rheol == 3) arg = law.viscE[l] * (law.viscT[l] - temp);
-            else arg = law.viscE[l] * (1.0f - temp);
-            v = (float)(tempa * exp((double)arg));
+            if (law.rheol == 11) {
+                // eta0 exp(E/(T+T0) - E/(0.5+T0)): the first quotient in
+                // float, the second in double (the literal 0.5), :697-719
+                const double arg = (double)(law.viscE[l] / (temp + law.viscT[l])) -
+                                   (double)law.viscE[l] / (0.5 + (double)law.viscT[l]);
+                v = (float)(tempa * exp(arg));
+            } else {
+                float arg;
+                if (law.rheol == 3) arg = law.viscE[l] * (law.viscT[l] - temp);
+                else arg = law.viscE[l] * (1.0f - temp);
+                v = (float)(tempa * exp((double)arg));
+            }
         }
         if (law.vmax_on && v > law.visc_max) v = law.visc_max;
         if (law.vmin_on && v < law.visc_min) v = law.visc_min;
@@ -1373,7 +1381,11 @@ __global__ void __launch_bounds__(128) k_node_ks(Geo g, const float *__restrict_
                             }
                         }
                     }
-                    if (!nbig) {
+                    // m > n: the lower node's stored block, transposed.  m == n:
+                    // the reference's matvec and smoother read a node's own block
+                    // through the column loop (Element_calculations.c:604-605,
+                    // General_matrix_functions.c:1254-1255), i.e. transposed too
+                    if (!nbig || s == 13) {
                         float t[3][3];
 #pragma unroll
                         for (int r = 0; r < 3; r++)

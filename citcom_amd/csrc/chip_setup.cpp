@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "chip_common.h"
@@ -256,6 +257,15 @@ int build_mesh(chip_ctx *c) {
     c->levmax = p.levels - 1;
     if (p.levels < 2 || p.levels > MAXL) {
         set_error("levels must be in [2, 12]");
+        return -1;
+    }
+    // the viscosity laws visc_from_T implements here (Viscosity_structures.c:491-737)
+    if (p.tdepv && p.rheol != 0 && p.rheol != 3 && p.rheol != 11) {
+        set_error("rheol " + std::to_string(p.rheol) + " is not supported (0, 3 and 11 are)");
+        return -1;
+    }
+    if (p.num_mat < 1 || p.num_mat > 4) {
+        set_error("num_mat must be in [1, 4]");
         return -1;
     }
     for (int lev = c->levmin; lev <= c->levmax; lev++) {

@@ -168,6 +168,14 @@ def problem_from_deck(d: dict[str, str]) -> Problem:
         coords.append(axis_coordinates(layer[ax], nn[ax], sx, sn))
     num_mat = _i(d, "num_mat", 1)
     rheol = _i(d, "rheol", 3)
+    if "z_layer" in d:
+        z_layer = _fv(d, "z_layer", 4, 0.0)
+    else:
+        # Viscosity_structures.c:129-141: without z_layer the four layer bases
+        # are z_lith, z_410, z_lmantle and the bottom
+        if num_mat != 4:
+            raise ValueError("either use z_layer for non dim layer depths, or set num_mat to four")
+        z_layer = [_f(d, "z_lith", 0.0), _f(d, "z_410", 1.0), _f(d, "z_lmantle", 1.0), 0.0]
     p = Problem(
         levels=levels, mgunit=mg, layer=layer, coords=tuple(coords),
         topvbc=_i(d, "topvbc", 0), botvbc=_i(d, "botvbc", 0),
@@ -180,7 +188,7 @@ def problem_from_deck(d: dict[str, str]) -> Problem:
         aug_number=_f(d, "aug_number", 0.0), precond=_b(d, "precond", "off"),
         rheol=rheol, tdepv=_b(d, "TDEPV", "on"), visc_update=_b(d, "VISC_UPDATE", "on"),
         num_mat=num_mat, visc0=_fv(d, "visc0", 4, 1.0), viscE=_fv(d, "viscE", 4, 0.0),
-        viscT=_fv(d, "viscT", 4, 0.0), z_layer=_fv(d, "z_layer", 4, 0.0),
+        viscT=_fv(d, "viscT", 4, 0.0), z_layer=z_layer,
         vmin_on=_b(d, "VMIN", "off"), vmax_on=_b(d, "VMAX", "off"),
         visc_min=_f(d, "visc_min", 1e-20), visc_max=_f(d, "visc_max", 1e20),
         rayleigh=_f(d, "rayleigh", 1.0), finetunedt=_f(d, "finetunedt", 0.9),
@@ -267,6 +275,11 @@ def workload(name: str) -> dict[str, object]:
     layer256  256x256x128 elements, 7 levels, Ra=1e7, contrast 1e5, top layer x100
     learn16  16^3 elements, 3 levels, Ra=1e5, eta = exp(-ln(1e3) T): the
           training and test problems of the learned warm start
+    busse1a / busse2  the reference's own example decks
+          (examples/Busse1993/case1a.input, case2.input): 32x16x32 and 32^3
+          elements, 5 levels, rigid top and bottom; case 2 with the
+          Arrhenius law (rheol 11) on a z grid refined at the bottom, started
+          from the example's case2-tic.temp.0.0 (restart = 1)
 
     ``name@acc`` is the same workload with ``accuracy=acc`` (e.g. iso16@1e-8).
     """
@@ -312,9 +325,54 @@ def workload(name: str) -> dict[str, object]:
         d.update(_grid((2.0, 2.0, 1.0), (4, 4, 2), 7))
         d.update({"mgunitx": 4, "mgunity": 4, "mgunitz": 2, "levels": 7, "maxstep": 5, "finetunedt": 1.0})
         d.update(_tdep(1e7, 1e5, num_mat=2, visc0="100,1", z_layer="0.9,0"))
+    elif name in ("busse1a", "busse2"):
+        d.update(_busse(name))
     else:
         raise KeyError(name)
     return d
+
+
+def _busse(name: str) -> dict[str, object]:
+    """Keys of examples/Busse1993/case1a.input and case2.input (the parts the
+    hot path reads; output, sphere and phase-change keys left out)."""
+    d = {"rayleigh": 3.0e4 if name == "busse1a" else 2.0e4,
+         "maxstep": 20000 if name == "busse1a" else 10000,
+         "nprocx": 1, "nprocy": 1, "nprocz": 1, "mgunitx": 2, "mgunitz": 2, "mgunity": 1 if name == "busse1a" else 2,
+         "levels": 5, "num_mat": 4, "visc0": "1,1,1,1", "viscZ": "5e-6,5e-6,5e-6,5e-6",
+         "z_lmantle": 0.76655052, "z_410": 0.857143, "z_lith": 0.9651568,
+         "VMIN": "off", "visc_min": 5.0e-2, "VMAX": "off", "visc_max": 2.0e4, "visc_smooth_cycles": 1,
+         "topvbc": 1, "topvbxval": 0.0, "topvbyval": 0.0, "botvbc": 1, "botvbxval": 0.0, "botvbyval": 0.0,
+         "toptbc": 1, "bottbc": 1, "toptbcval": 0.0, "bottbcval": 1.0,
+         "num_perturbations": 1, "perturbmag": 0.001, "perturbk": 1.0, "perturbl": 6.0, "perturbm": 0.0,
+         "aug_lagr": "on", "aug_number": 1.0e3, "precond": "on", "mg_cycle": 1, "down_heavy": 3, "up_heavy": 3,
+         "vlowstep": 20, "vhighstep": 3, "piterations": 375, "accuracy": 1.0e-3, "adv_sub_iterations": 2,
+         "finetunedt": 0.75}
+    if name == "busse1a":
+        d.update({"restart": 0, "dimenx": 1.0079, "dimeny": 0.6283, "dimenz": 1.0,
+                  "x_grid_layers": 2, "xx": "0,1.0079", "nx": "1,33", "y_grid_layers": 2, "yy": "0,0.6283",
+                  "ny": "1,17", "z_grid_layers": 2, "zz": "0.0,1.0", "nz": "1,33",
+                  "rheol": 0, "TDEPV": "off", "VISC_UPDATE": "off", "update_every_steps": 1,
+                  "viscE": "0,0,0,0", "viscT": "273,273,273,273"})
+    else:
+        d.update({"restart": 1, "restart_timesteps": 0, "dimenx": 1.0, "dimeny": 1.0, "dimenz": 1.0,
+                  "x_grid_layers": 2, "xx": "0,1.0", "nx": "1,33", "y_grid_layers": 2, "yy": "0,1.0", "ny": "1,33",
+                  "z_grid_layers": 4, "zz": "0.0,0.04,0.12,1.0", "nz": "1,3,6,33",
+                  "rheol": 11, "TDEPV": "on", "VISC_UPDATE": "on", "update_every_steps": 1,
+                  "viscE": "74.357912,74.357912,74.357912,74.357912", "viscT": "4.507123,4.507123,4.507123,4.507123"})
+    return d
+
+
+def read_restart_temperature(path: str) -> np.ndarray:
+    """T of a restart file as Convection.c:473-490 reads it: the first value
+    of each node line (%g into float), clamped to T >= 0."""
+    with open(path) as fh:
+        head = fh.readline().split()
+        vals = [line.split()[0] for line in fh if line.strip()]
+    nno = int(head[0])
+    if len(vals) != nno:
+        raise ValueError(f"{path}: {len(vals)} node lines for {nno} nodes")
+    T = np.array(vals, dtype=np.float32)
+    return np.maximum(T, np.float32(0.0))
 
 
 def seeded_temperature(p: Problem, seed: int, amplitude: float = 0.01) -> np.ndarray:

@@ -76,6 +76,15 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ChipError(f"{LIB_PATH} is missing: run citcom_amd/build.py (or __graft_entry__.build())")
+        # torch bundles a HIP runtime with the same soname (libamdhip64.so.7)
+        # as /opt/rocm's.  Whichever loads first serves the whole process; when
+        # it is /opt/rocm's, torch later finds no device.  Loading torch first
+        # keeps torch tensors (the learned network) and this library usable
+        # together in one process.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         P = C.c_void_p
         L.chip_last_error.restype = C.c_char_p

@@ -515,7 +515,7 @@ void *orc_get(orc *o, const char *name, int lev, long *count)
 /* ------------------------------------------------------------------------ */
 /* viscosity                                                                 */
 
-/* visc_from_T, Viscosity_structures.c:491-737 (rheol 0 and 3), followed by
+/* visc_from_T, Viscosity_structures.c:491-737 (rheol 0, 3 and 11), followed by
  * the MAX/MIN clip of get_system_viscosity :409-425 */
 void orc_visc_from_T(orc *o, const float *T, float *evi)
 {
@@ -539,6 +539,8 @@ void orc_visc_from_T(orc *o, const float *T, float *evi)
 				temp = (float)(temp + fmaxf_(zero, TT[kk]) * o->Nv[GNV(kk, jj)]);
 			if(!o->p.tdepv)
 				v = tempa;
+			else if(o->p.rheol == 11)	/* :697-719 */
+				v = tempa * exp(o->p.viscE[l] / (temp + o->p.viscT[l]) - o->p.viscE[l] / (0.5 + o->p.viscT[l]));
 			else
 			{
 				if(o->p.rheol == 3)
@@ -819,7 +821,10 @@ static void node_ks(orc *o, int lev)
 						for(c = 0; c < 3; c++)
 						{
 							float v;
-							if(m <= n)
+							/* a node's own block (m == n) is read transposed by the
+							 * reference's matvec and smoother (Element_calculations.c:604-605,
+							 * General_matrix_functions.c:1254-1255) */
+							if(m < n)
 								v = half[(size_t)n * KROW + s * 9 + r * 3 + c];
 							else
 								v = half[(size_t)m * KROW + nslot(-dz, -dx, -dy) * 9 + c * 3 + r];

@@ -29,6 +29,16 @@ def problem(name):
     return deck.problem_from_deck({k: str(v) for k, v in deck.workload(name).items()})
 
 
+def initial_temperature(name, p):
+    """The starting temperature of a deck: the reference's shipped restart
+    field for busse2 (examples/Busse1993/case2-tic.temp.0.0, as its restart
+    reader leaves it), else the perturbed conductive profile."""
+    from citcom_amd import deck
+    if name.split("@")[0] == "busse2":
+        return golden("busse2_tic.npz")["T"]
+    return deck.perturbed_temperature(p)
+
+
 def golden(name):
     return np.load(os.path.join(GOLDEN, name))
 

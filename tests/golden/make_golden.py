@@ -17,6 +17,14 @@ citcom_amd.deck and stores what the tests compare against:
 * ``<deck>_at_<acc>_run.npz``  the same runs with accuracy = acc: two solvers
   that stop on relative-increment tests at 1e-6 can differ by O(1e-6), so
   field parity at 1e-6 is shown where both are converged well below it.
+* ``busse2_tic.npz``     the initial temperature the reference ships with its
+  example deck (examples/Busse1993/case2-tic.temp.0.0), as read by the
+  restart reader.
+* ``busse*_setup.npz``   the setup hashes of the reference's example decks
+  (no raw fields: 33^3 nodes); ``busse*_run.npz`` per-step (time, dt,
+  Nu_top, Nu_bot) of 20 steps at the decks' own accuracy (1e-3);
+  ``busse*_at_1e-8_run.npz`` the first Stokes solve and the state after 3
+  steps (U, P, T in float32) at accuracy 1e-8.
 
 Usage: python tests/golden/make_golden.py [deck ...]
 """
@@ -39,6 +47,7 @@ from citcom_amd import deck  # noqa: E402
 import refdump  # noqa: E402
 
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+REF_EXAMPLE_TIC = "/root/reference/examples/Busse1993/case2-tic.temp.0.0"
 
 
 def sha(a: np.ndarray) -> str:
@@ -50,6 +59,9 @@ def run_harness(deck_name: str, workdir: str, mode: str, arg: str | None = None)
     keys["datafile"] = os.path.join(workdir, "out", deck_name)
     keys["oldfile"] = keys["datafile"]
     os.makedirs(os.path.join(workdir, "out"), exist_ok=True)
+    if str(keys.get("restart", 0)) == "1":
+        T = np.load(os.path.join(HERE, "busse2_tic.npz"))["T"]
+        deck.write_restart_temperature(keys["oldfile"], T)
     path = os.path.join(workdir, f"{deck_name}.input")
     deck.write_deck(path, keys)
     out = os.path.join(workdir, f"{deck_name}_{mode}")
@@ -58,15 +70,17 @@ def run_harness(deck_name: str, workdir: str, mode: str, arg: str | None = None)
     return out
 
 
-def setup_fixture(name: str, d: str) -> dict:
+def setup_fixture(name: str, d: str, raw: bool = True) -> dict:
     p = deck.problem_from_deck({k: str(v) for k, v in deck.workload(name).items()})
     out = {}
     for lev in range(p.levels):
         nox, noy, noz = p.dims(lev)
-        for key, fn in (("X", "X1.f32"), ("Y", "X2.f32"), ("Z", "X3.f32"), ("GNX", "GNX.f32"), ("GDA", "GDA.f32"),
-                        ("ECO", "ECO.f32"), ("MASS", "MASS.f32"), ("EVI", "EVI.f32"), ("EG", "EG.f32"),
+        for key, fn in (("X", "X1.f32"), ("Y", "X2.f32"), ("Z", "X3.f32"), ("GNX", "GNX.f32"),
+                        ("MASS", "MASS.f32"), ("EVI", "EVI.f32"), ("EG", "EG.f32"),
                         ("BI", "BI.f64"), ("BPI", "BPI.f64")):
             out[f"sha_{key}_{lev}"] = sha(refdump.load(d, fn, lev))
+        out[f"sha_GDA_{lev}"] = sha(refdump.load(d, "GDA.f32", lev).reshape(-1, 11)[:, refdump.GDA_USED])
+        out[f"sha_ECO_{lev}"] = sha(refdump.load(d, "ECO.f32", lev).reshape(-1, 13)[:, refdump.ECO_USED])
         out[f"sha_TWW_{lev}"] = sha(refdump.load(d, "TWW.f32", lev).reshape(-1, 9)[:, 1:])
         out[f"sha_NODEV_{lev}"] = sha(refdump.load(d, "NODE.u32", lev) & np.uint32(0xE))
         full = refdump.full_rows_from_half(*(refdump.load(d, f"K{i}.f32", lev) for i in (1, 2, 3)),
@@ -76,9 +90,10 @@ def setup_fixture(name: str, d: str) -> dict:
         out[f"sha_{key}"] = sha(refdump.load(d, fn))
     # velocity and temperature boundary bits (VBX|VBZ|VBY|TBX|TBZ|TBY)
     out["sha_nodeflags"] = sha(refdump.load(d, "node.u32") & np.uint32(0x7E))
-    out["T"] = refdump.load(d, "T.f32")
-    out["F"] = refdump.load(d, "F.f64")
-    out["EVI"] = refdump.load(d, "EVI.f32", p.levmax)
+    if raw:
+        out["T"] = refdump.load(d, "T.f32")
+        out["F"] = refdump.load(d, "F.f64")
+        out["EVI"] = refdump.load(d, "EVI.f32", p.levmax)
     return out
 
 
@@ -107,6 +122,26 @@ def main(names):
         raise SystemExit(f"{HARNESS} missing: make -C oracle ref")
     with tempfile.TemporaryDirectory() as tmp:
         for name in names:
+            if name == "busse2_tic":
+                T = deck.read_restart_temperature(REF_EXAMPLE_TIC)
+                np.savez_compressed(os.path.join(HERE, "busse2_tic.npz"), T=T)
+                print("wrote", name)
+                continue
+            if name.startswith("busse"):
+                tag = name.replace("@", "_at_")
+                if "@" in name:
+                    d = run_harness(name, tmp, "run", "3")
+                    fx = {k: (v.astype(np.float32) if k != "steps" else v)
+                          for k, v in run_fixture(d, ("U.f64", "P.f64", "T.f32")).items()
+                          if not k.startswith("pg1") and k != "step0_T"}
+                    np.savez_compressed(os.path.join(HERE, f"{tag}_run.npz"), **fx)
+                else:
+                    d = run_harness(name, tmp, "setup")
+                    np.savez_compressed(os.path.join(HERE, f"{name}_setup.npz"), **setup_fixture(name, d, raw=False))
+                    d = run_harness(name, tmp, "run", "20")
+                    np.savez_compressed(os.path.join(HERE, f"{name}_run.npz"), steps=run_fixture(d, ())["steps"])
+                print("wrote", name)
+                continue
             if name.startswith("iso16"):
                 d = run_harness(name, tmp, "run", "200")
                 tag = name.replace("@", "_at_")

@@ -38,7 +38,23 @@ def full_rows_from_half(K1, K2, K3, MAP, nox, noy, noz):
             # transpose into the row of the lower node m
             tslot = 26 - slot
             full[m, c, tslot, r] = Kd[ok, r, s]
+    # a node's own block as n_assemble_del2_u and gauss_seidel read it: through
+    # the column loop (Element_calculations.c:604-605), i.e. transposed
+    for s in range(42):
+        eq = M[:, s]
+        self_ = (eq < neq) & (eq // 3 == n)
+        a = eq[self_] % 3
+        nn = n[self_]
+        for b in range(3):
+            full[nn, a, 13, b] = Kd[self_, b, s]
     return full.reshape(nno, 243)
+
+
+# slots of the reference's per-element structs that it writes (the others
+# hold whatever the allocation held): Shape_function_dA vpt[1..8], ppt[1];
+# COORD centre[1..3], size[1..3], area (global_defs.h:240-244, 316-322)
+GDA_USED = [1, 2, 3, 4, 5, 6, 7, 8, 10]
+ECO_USED = [1, 2, 3, 5, 6, 7, 12]
 
 
 class XorShift:

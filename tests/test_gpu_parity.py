@@ -281,3 +281,13 @@ def test_smoother_kernels_bitwise(name, small, pyoracle):
             assert rel_l2(c.apply("affine1", 1, x["d0"], x["F"]), d) < 1e-6
     say(name, "small_levels", small, "smoother bitwise on", p.levels, "levels")
     c.close()
+
+
+def test_unsupported_viscosity_law_is_refused():
+    """Viscosity laws other than rheol 0, 3 and 11 are refused at setup, not
+    computed as another law."""
+    import dataclasses
+    from citcom_amd.hip import ChipError, Citcom
+    p = dataclasses.replace(problem("tiny_tdep"), rheol=1)
+    with pytest.raises(ChipError, match="rheol 1 is not supported"):
+        Citcom(p)

@@ -13,10 +13,13 @@ import hashlib
 import numpy as np
 import pytest
 
-from conftest import golden, problem, rel_l2
+from conftest import golden, initial_temperature, problem, rel_l2
 import refdump
 
 DECKS = ["tiny", "tiny_tdep", "tiny_layer"]
+# the reference's own example decks (examples/Busse1993): rigid lids, a
+# non-cubic box, a graded z grid, the Arrhenius law, the shipped restart field
+EXAMPLES = ["busse1a", "busse2"]
 
 
 def sha(a):
@@ -26,19 +29,20 @@ def sha(a):
 def make(pyoracle, name, smoother=0):
     p = problem(name)
     o = pyoracle.Oracle(p, smoother=smoother)
-    from citcom_amd import deck
-    T = deck.perturbed_temperature(p)
+    T = initial_temperature(name, p)
     o.set_viscosity(o.visc_from_T(T))
     return p, o, T
 
 
-@pytest.mark.parametrize("name", DECKS)
+@pytest.mark.parametrize("name", DECKS + EXAMPLES)
 def test_setup_bitwise(pyoracle, name):
     p, o, T = make(pyoracle, name)
     g = golden(f"{name}_setup.npz")
     for lev in range(p.levels):
-        for key in ("X", "Y", "Z", "GNX", "GDA", "ECO", "MASS", "EVI", "EG", "BI", "BPI", "K"):
+        for key in ("X", "Y", "Z", "GNX", "MASS", "EVI", "EG", "BI", "BPI", "K"):
             assert sha(o.get(key, lev)) == str(g[f"sha_{key}_{lev}"]), (key, lev)
+        assert sha(o.get("GDA", lev).reshape(-1, 11)[:, refdump.GDA_USED]) == str(g[f"sha_GDA_{lev}"]), lev
+        assert sha(o.get("ECO", lev).reshape(-1, 13)[:, refdump.ECO_USED]) == str(g[f"sha_ECO_{lev}"]), lev
         assert sha(o.get("TWW", lev).reshape(-1, 9)[:, 1:]) == str(g[f"sha_TWW_{lev}"]), lev
         assert sha(o.get("NODE", lev) & np.uint32(0xE)) == str(g[f"sha_NODEV_{lev}"]), lev
     assert sha(o.get("NODE") & np.uint32(0x7E)) == str(g["sha_nodeflags"])
@@ -115,3 +119,13 @@ def test_iso16_first_steps(pyoracle):
         # the harness records Nu after the step's Stokes solve
         nu = o.heat_flux(T, pyoracle.v_from_vector(U, nno).reshape(-1))
         assert nu[0] == pytest.approx(g["steps"][step, 3], rel=1e-5)
+
+
+@pytest.mark.parametrize("name", EXAMPLES)
+def test_example_decks_tight_solve(pyoracle, name):
+    """The first Stokes solve of the reference's example decks at accuracy
+    1e-8 (lexicographic smoother, as the reference)."""
+    p, o, T = make(pyoracle, name + "@1e-8")
+    g = golden(f"{name}_at_1e-8_run.npz")
+    U, P, st = o.solve_stokes(np.zeros(3 * o.nno()), np.zeros(o.nel()), o.assemble_forces(o.thermal_buoyancy(T)))
+    assert rel_l2(U, g["step0_U"]) < 1e-6 and rel_l2(P, g["step0_P"]) < 1e-6
