@@ -467,6 +467,15 @@ extern "C" {
 
 const char *chip_last_error(void) { return chip::g_err.c_str(); }
 
+// every entry point taking a context refuses a null one
+#define CHIP_NEED_CTX(c)                                 \
+    do {                                                 \
+        if (!(c)) {                                      \
+            chip::set_error(std::string(__func__) + ": null context"); \
+            return -1;                                   \
+        }                                                \
+    } while (0)
+
 int chip_create(const chip_problem *p, chip_ctx **out) {
     *out = nullptr;
     if (!p || !p->xc || !p->yc || !p->zc) {
@@ -523,6 +532,7 @@ void chip_destroy(chip_ctx *c) {
 }
 
 int chip_set_stream(chip_ctx *c, void *hip_stream) {
+    CHIP_NEED_CTX(c);
     if (!hip_stream) return 0;
     if (c->own_stream) {
         CHIP_CHECK(hipStreamSynchronize(c->stream));
@@ -536,6 +546,10 @@ int chip_set_stream(chip_ctx *c, void *hip_stream) {
 void *chip_get_stream(chip_ctx *c) { return (void *)c->stream; }
 
 static int resolve(chip_ctx *c, const char *which, int lev, void **ptr, size_t *elem, long *count) {
+    if (!c || !which) {
+        set_error("null context or field name");
+        return -1;
+    }
     if (lev < 0) lev = c->levmax;
     if (lev < c->levmin || lev > c->levmax) {
         set_error("level out of range");
@@ -569,6 +583,10 @@ static int resolve(chip_ctx *c, const char *which, int lev, void **ptr, size_t *
 }
 
 int chip_upload(chip_ctx *c, const char *which, int lev, const void *host, long count) {
+    if (!host) {
+        set_error("chip_upload: null host pointer");
+        return -1;
+    }
     void *ptr;
     size_t elem;
     long n;
@@ -583,10 +601,19 @@ int chip_upload(chip_ctx *c, const char *which, int lev, const void *host, long 
 }
 
 int chip_download(chip_ctx *c, const char *which, int lev, void *host, long count) {
+    if (!c || !which || !host) {
+        set_error("chip_download: null argument");
+        return -1;
+    }
     const std::string w(which);
     if (w == "K") {
         // device tiles [q/64][slot*9 + r*3 + c][q%64], colour order -> host [n][r*81 + slot*3 + c]
-        const Level &L = c->L[lev < 0 ? c->levmax : lev];
+        if (lev < 0) lev = c->levmax;
+        if (lev < c->levmin || lev > c->levmax) {
+            set_error("level out of range");
+            return -1;
+        }
+        const Level &L = c->L[lev];
         const Geo &g = L.g;
         if (count != (long)g.nno * KROW) {
             set_error("chip_download K: size mismatch");
@@ -624,12 +651,14 @@ int chip_download(chip_ctx *c, const char *which, int lev, void *host, long coun
 }
 
 int chip_get_system_viscosity(chip_ctx *c) {
+    CHIP_NEED_CTX(c);
     visc_from_T(c->stream, c, c->T, c->L[c->levmax].EVI);
     CHIP_CHECK(hipGetLastError());
     return 0;
 }
 
 int chip_construct_stiffness_B_matrix(chip_ctx *c) {
+    CHIP_NEED_CTX(c);
     if (rebuild(c)) return -1;
     CHIP_CHECK(hipStreamSynchronize(c->stream));
     return 0;
@@ -637,6 +666,11 @@ int chip_construct_stiffness_B_matrix(chip_ctx *c) {
 
 int chip_apply(chip_ctx *c, const char *op, int lev, const double *in, const double *in2, double *out, double *out2,
                int arg) {
+    CHIP_NEED_CTX(c);
+    if (!op) {
+        set_error("chip_apply: null op");
+        return -1;
+    }
     if (lev < c->levmin || lev > c->levmax) {
         set_error("chip_apply: level out of range");
         return -1;
@@ -722,6 +756,7 @@ int chip_apply(chip_ctx *c, const char *op, int lev, const double *in, const dou
 }
 
 int chip_solve_del2_u(chip_ctx *c, const double *F, double *d0, double acc, int *valid, int *cycles) {
+    CHIP_NEED_CTX(c);
     const long neq = 3L * c->L[c->levmax].g.nno;
     CHIP_CHECK(hipMemcpyAsync(c->Ah, F, neq * 8, hipMemcpyHostToDevice, c->stream));
     int v = 0, n = 0;
@@ -733,25 +768,37 @@ int chip_solve_del2_u(chip_ctx *c, const double *F, double *d0, double acc, int 
     return prof_flush(c);
 }
 
-int chip_solve_constrained_flow_iterative(chip_ctx *c, chip_stokes_stats *st) { return uzawa(c, st); }
+int chip_solve_constrained_flow_iterative(chip_ctx *c, chip_stokes_stats *st) {
+    CHIP_NEED_CTX(c);
+    return uzawa(c, st);
+}
 
-int chip_PG_timestep(chip_ctx *c, chip_pg_stats *st) { return pg_timestep(c, st); }
+int chip_PG_timestep(chip_ctx *c, chip_pg_stats *st) {
+    CHIP_NEED_CTX(c);
+    return pg_timestep(c, st);
+}
 
 int chip_thermal_buoyancy(chip_ctx *c) {
+    CHIP_NEED_CTX(c);
     thermal_buoyancy(c->stream, c, c->T, c->buoy);
     CHIP_CHECK(hipGetLastError());
     return 0;
 }
 
 int chip_assemble_forces(chip_ctx *c) {
+    CHIP_NEED_CTX(c);
     assemble_forces(c->stream, c, c->buoy, c->F);
     CHIP_CHECK(hipGetLastError());
     return 0;
 }
 
-int chip_heat_flux(chip_ctx *c, float *nu) { return heat_flux_nu(c, nu); }
+int chip_heat_flux(chip_ctx *c, float *nu) {
+    CHIP_NEED_CTX(c);
+    return heat_flux_nu(c, nu);
+}
 
 int chip_vrms(chip_ctx *c, double *vrms_out) {
+    CHIP_NEED_CTX(c);
     vrms(c->stream, c, c->V, c->dots + D_TMP);
     if (sync_dots(c)) return -1;
     *vrms_out = std::sqrt(c->h_dots[D_TMP] / c->h_dots[D_TMP + 1]);
@@ -759,11 +806,13 @@ int chip_vrms(chip_ctx *c, double *vrms_out) {
 }
 
 int chip_general_stokes_solver(chip_ctx *c, chip_stokes_stats *st) {
+    CHIP_NEED_CTX(c);
     const int r = stokes_step(c, st);
     return r;
 }
 
 int chip_timestep(chip_ctx *c, chip_pg_stats *pg, chip_stokes_stats *st, float *nu) {
+    CHIP_NEED_CTX(c);
     c->solution_cycles++;
     if (pg_timestep(c, pg)) return -1;
     if (nu && heat_flux_nu(c, nu)) return -1;
@@ -771,6 +820,7 @@ int chip_timestep(chip_ctx *c, chip_pg_stats *pg, chip_stokes_stats *st, float *
 }
 
 int chip_profile_enable(chip_ctx *c, const char *kernel, int on) {
+    CHIP_NEED_CTX(c);
     const std::string k(kernel ? kernel : "");
     if (prof_flush(c)) return -1;
     if (!on) {
@@ -788,6 +838,7 @@ int chip_profile_enable(chip_ctx *c, const char *kernel, int on) {
 }
 
 int chip_profile_read(chip_ctx *c, long *launches, double *total_ms, double *bytes) {
+    CHIP_NEED_CTX(c);
     if (prof_flush(c)) return -1;
     if (launches) *launches = c->prof.launches;
     if (total_ms) *total_ms = c->prof.total_ms;
@@ -795,9 +846,10 @@ int chip_profile_read(chip_ctx *c, long *launches, double *total_ms, double *byt
     return 0;
 }
 
-long chip_device_bytes(chip_ctx *c) { return c->dev_bytes; }
+long chip_device_bytes(chip_ctx *c) { return c ? c->dev_bytes : -1; }
 
 int chip_set_option(chip_ctx *c, const char *name, int value) {
+    CHIP_NEED_CTX(c);
     const std::string n(name ? name : "");
     if (n == "graphs") {
         c->use_graphs = value != 0;

@@ -37,3 +37,26 @@ def test_no_cpu_fallback_without_device():
     p = deck.problem_from_deck({k: str(v) for k, v in deck.workload("tiny").items()})
     with pytest.raises(ChipError, match="no HIP device"):
         Citcom(p)
+
+
+def test_null_context_refused():
+    """Every entry point that takes a context refuses a null one with an
+    error code and a message (no device work is started)."""
+    import ctypes as C
+    from citcom_amd import hip
+    L = hip.lib()
+    calls = {
+        "chip_solve_constrained_flow_iterative": (None, None),
+        "chip_PG_timestep": (None, None),
+        "chip_general_stokes_solver": (None, None),
+        "chip_construct_stiffness_B_matrix": (None,),
+        "chip_get_system_viscosity": (None,),
+        "chip_set_option": (None, b"graphs", 1),
+        "chip_heat_flux": (None, None),
+    }
+    for name, args in calls.items():
+        assert getattr(L, name)(*args) == -1, name
+        assert b"null context" in L.chip_last_error(), name
+    buf = (C.c_double * 4)()
+    assert L.chip_upload(None, b"U", 0, buf, 4) == -1
+    assert L.chip_download(None, b"U", 0, buf, 4) == -1
