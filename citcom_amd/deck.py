@@ -391,28 +391,35 @@ def seeded_temperature(p: Problem, seed: int, amplitude: float = 0.01) -> np.nda
     return np.maximum(T.reshape(-1), 0.0).astype(np.float32)
 
 
+def rank_blocks(dims: tuple[int, int, int], nproc: tuple[int, int, int]):
+    """(rank, y slice, x slice, z slice) of each MPI rank's local nodes in the
+    global (noy, nox, noz) grid, as parallel_domain_decomp0 assigns them
+    (Parallel_related.c:108-121): z fastest, then x, then y; neighbouring
+    subdomains share their face nodes."""
+    px, py, pz = nproc
+    nox, noy, noz = dims
+    ex, ey, ez = (nox - 1) // px, (noy - 1) // py, (noz - 1) // pz
+    if ex * px != nox - 1 or ey * py != noy - 1 or ez * pz != noz - 1:
+        raise ValueError(f"grid {dims} does not split over {nproc} ranks")
+    for me in range(px * py * pz):
+        kz, jy = me % pz, me // (px * pz)
+        ix = (me - jy * px * pz) // pz
+        yield (me, slice(jy * ey, jy * ey + ey + 1), slice(ix * ex, ix * ex + ex + 1),
+               slice(kz * ez, kz * ez + ez + 1))
+
+
 def write_restart_temperature(path_prefix: str, T: np.ndarray, dims: tuple[int, int, int] | None = None,
                               nproc: tuple[int, int, int] = (1, 1, 1)) -> list[str]:
     """Write ``<prefix>.temp.<rank>.0`` in the reference's restart format
     (read by Convection.c:473-490), one file per MPI rank over that rank's
-    local nodes.  Ranks follow parallel_domain_decomp0
-    (Parallel_related.c:108-121): z fastest, then x, then y.  ``dims`` =
-    (nox, noy, noz) of the global grid; needed when nproc is not (1, 1, 1)."""
-    px, py, pz = nproc
-    if (px, py, pz) == (1, 1, 1):
+    local nodes (rank_blocks).  ``dims`` = (nox, noy, noz) of the global grid;
+    needed when nproc is not (1, 1, 1)."""
+    if tuple(nproc) == (1, 1, 1):
         blocks = [(0, T.reshape(-1))]
     else:
         nox, noy, noz = dims
-        ex, ey, ez = (nox - 1) // px, (noy - 1) // py, (noz - 1) // pz
-        if ex * px != nox - 1 or ey * py != noy - 1 or ez * pz != noz - 1:
-            raise ValueError(f"grid {dims} does not split over {nproc} ranks")
         Tg = T.reshape(noy, nox, noz)
-        blocks = []
-        for me in range(px * py * pz):
-            kz, jy = me % pz, me // (px * pz)
-            ix = (me - jy * px * pz) // pz
-            blocks.append((me, Tg[jy * ey:jy * ey + ey + 1, ix * ex:ix * ex + ex + 1,
-                              kz * ez:kz * ez + ez + 1].reshape(-1)))
+        blocks = [(me, Tg[sy, sx, sz].reshape(-1)) for me, sy, sx, sz in rank_blocks(dims, nproc)]
     paths = []
     for me, vals in blocks:
         path = f"{path_prefix}.temp.{me}.0"

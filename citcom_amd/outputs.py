@@ -9,7 +9,9 @@ compared field by field.
   <datafile>.velo.<rank>.<step>      nno timesteps time, then V1 V2 V3
 
 Node order is the reference's (node = k + noz*(i + nox*j), 1-based in the
-files' line order), the same as this package's 0-based order.
+files' line order), the same as this package's 0-based order; a run on
+several MPI ranks writes one file per rank over its local nodes, which
+``assemble`` puts back on the global grid.
 """
 from __future__ import annotations
 
@@ -72,3 +74,21 @@ def read_coord(path: str) -> np.ndarray:
     if rows.shape[0] != nno:
         raise ValueError(f"{path}: {rows.shape[0]} rows for {nno} nodes")
     return rows
+
+
+def assemble(fields: list[np.ndarray], dims: tuple[int, int, int], nproc: tuple[int, int, int]) -> np.ndarray:
+    """Per-rank node rows (rank order) -> global node rows [nno][ncol];
+    shared face nodes must agree between ranks."""
+    from .deck import rank_blocks
+    nox, noy, noz = dims
+    ncol = fields[0].shape[1]
+    out = np.full((noy, nox, noz, ncol), np.nan)
+    for (me, sy, sx, sz), vals in zip(rank_blocks(dims, nproc), fields):
+        blk = out[sy, sx, sz]
+        v = vals.reshape(blk.shape)
+        if not np.all(np.isnan(blk) | (np.abs(blk - v) <= 1e-6 * np.maximum(np.abs(v), 1e-30))):
+            raise ValueError(f"rank {me}: shared nodes disagree")
+        out[sy, sx, sz] = v
+    if np.isnan(out).any():
+        raise ValueError("ranks do not cover the grid")
+    return out.reshape(-1, ncol)

@@ -17,6 +17,10 @@ citcom_amd.deck and stores what the tests compare against:
 * ``<deck>_at_<acc>_run.npz``  the same runs with accuracy = acc: two solvers
   that stop on relative-increment tests at 1e-6 can differ by O(1e-6), so
   field parity at 1e-6 is shown where both are converged well below it.
+* ``busse1a_steady.npz``  the reference program (oracle/_ref/citcom.mpi) on
+  the case-1a deck under MPI (4 ranks), 20000 steps to its steady state:
+  final T and V (assembled from the per-rank .temp/.velo files), Nu_top,
+  Nu_bot and time from the top and bottom ranks' .ave files.
 * ``busse2_tic.npz``     the initial temperature the reference ships with its
   example deck (examples/Busse1993/case2-tic.temp.0.0), as read by the
   restart reader.
@@ -47,6 +51,8 @@ from citcom_amd import deck  # noqa: E402
 import refdump  # noqa: E402
 
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+PROGRAM = os.path.join(ROOT, "oracle", "_ref", "citcom.mpi")
+MPIEXEC = "/opt/conda/bin/mpiexec"
 REF_EXAMPLE_TIC = "/root/reference/examples/Busse1993/case2-tic.temp.0.0"
 
 
@@ -117,11 +123,47 @@ def run_fixture(d: str, fields=("U.f64", "P.f64", "T.f32", "Tdot.f32", "V1.f32",
     return out
 
 
+def steady_fixture(tmp: str, nsteps: int = 20000, nproc=(2, 1, 2)) -> dict:
+    from citcom_amd import outputs
+    keys = deck.workload("busse1a")
+    out = os.path.join(tmp, "steady")
+    os.makedirs(out, exist_ok=True)
+    keys.update({"datafile": os.path.join(out, "run"), "maxstep": nsteps, "storage_spacing": nsteps // 10,
+                 "nprocx": nproc[0], "nprocy": nproc[1], "nprocz": nproc[2]})
+    path = os.path.join(out, "deck.input")
+    deck.write_deck(path, keys)
+    ranks = nproc[0] * nproc[1] * nproc[2]
+    subprocess.run([MPIEXEC, "-launcher", "fork", "-n", str(ranks), PROGRAM, path], cwd=out,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    return read_steady(out, nsteps, nproc)
+
+
+def read_steady(out: str, nsteps: int, nproc) -> dict:
+    from citcom_amd import outputs
+    ranks = nproc[0] * nproc[1] * nproc[2]
+    p = deck.problem_from_deck({k: str(v) for k, v in deck.workload("busse1a").items()})
+    T = outputs.assemble([outputs.read_temp(os.path.join(out, f"run.temp.{r}.{nsteps}")).values[:, :1]
+                          for r in range(ranks)], p.dims(), nproc)[:, 0]
+    V = outputs.assemble([outputs.read_velo(os.path.join(out, f"run.velo.{r}.{nsteps}")).values
+                          for r in range(ranks)], p.dims(), nproc)
+    # ranks 0 .. nprocz-1 write .ave files, each with the Nusselt numbers of
+    # its own subdomain's top and bottom (Process_buoyancy.c:166-193): the
+    # surface values come from the top and the bottom rank
+    top = outputs.read_ave(os.path.join(out, f"run.ave.{nproc[2] - 1}.{nsteps}"))
+    bot = outputs.read_ave(os.path.join(out, f"run.ave.0.{nsteps}"))
+    return {"T": T.astype(np.float32), "V": V.T.astype(np.float32).reshape(-1), "steps": np.int64(top.timesteps),
+            "time": np.float64(top.time), "nu": np.array([top.nu_top, bot.nu_bot])}
+
+
 def main(names):
     if not os.path.exists(HARNESS):
         raise SystemExit(f"{HARNESS} missing: make -C oracle ref")
     with tempfile.TemporaryDirectory() as tmp:
         for name in names:
+            if name == "busse1a_steady":
+                np.savez_compressed(os.path.join(HERE, "busse1a_steady.npz"), **steady_fixture(tmp))
+                print("wrote", name)
+                continue
             if name == "busse2_tic":
                 T = deck.read_restart_temperature(REF_EXAMPLE_TIC)
                 np.savez_compressed(os.path.join(HERE, "busse2_tic.npz"), T=T)

@@ -11,7 +11,9 @@ restart field the reference ships (2).
   Stokes solve and the state after 3 steps agree to 1e-6 relative L2 and the
   Nusselt numbers to 1e-5, in the plain and in the mixed form;
 - at the decks' own accuracy 1e-3 the two smoothers stop at different
-  iterates within that tolerance: 20 steps of dt and Nu_top agree to 1e-3.
+  iterates within that tolerance: 20 steps of dt and Nu_top agree to 1e-3;
+- case 1a run to its steady state (20000 steps) lands on the reference
+  program's steady state (fields, Nu, Vrms to 1e-3).
 """
 import hashlib
 
@@ -104,3 +106,32 @@ def test_example_own_accuracy_20_steps(name):
         f"max rel ddt {worst_dt:.2e}")
     c.close()
     assert worst_nu < 1e-3 and worst_dt < 1e-3
+
+
+def test_busse1a_steady_state(pyoracle):
+    """Case 1a run to its steady state (20000 steps, the deck's own accuracy)
+    against the reference program's run of the same deck on 4 MPI ranks: the
+    steady temperature and velocity fields, Nu and the volume-RMS velocity
+    agree to 1e-3, the bound the deck's accuracy sets."""
+    g = golden("busse1a_steady.npz")
+    say("deck busse1a, 20000 steps")
+    p, c, T = start("busse1a")
+    c.general_stokes_solver()
+    for step in range(1, int(g["steps"]) + 1):
+        r = c.timestep(with_nu=False)
+        if step % 5000 == 0:
+            say(f"busse1a step {step}: Nu_top {c.heat_flux()[0]:.5f}")
+    nu = c.heat_flux()
+    V = c.download("V")
+    eT, eV = rel_l2(c.download("T"), g["T"]), rel_l2(V, g["V"])
+    o = pyoracle.Oracle(p)
+    vr, vr_ref = c.vrms(), o.vrms(g["V"])
+    say(f"busse1a steady: time {r['pg']['elapsed_time']:.5f} (reference {float(g['time']):.5f}), Nu_top {nu[0]:.5f} "
+        f"(reference {g['nu'][0]:.5f}), Vrms {vr:.4f} (reference {vr_ref:.4f}), T {eT:.2e}, V {eV:.2e}")
+    c.close()
+    # the model time after 20000 steps sums the transient's CFL steps, whose
+    # velocities differ within the solver tolerance: a looser bound
+    assert abs(r["pg"]["elapsed_time"] / float(g["time"]) - 1) < 5e-3
+    assert abs(nu[0] / g["nu"][0] - 1) < 1e-3 and abs(nu[1] / g["nu"][1] - 1) < 1e-3
+    assert abs(vr / vr_ref - 1) < 1e-3
+    assert eT < 1e-3 and eV < 1e-3
