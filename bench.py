@@ -242,7 +242,7 @@ def run_form(p, T, mixed, steps, warmup, graphs, profile):
 
     from citcom_amd.hip import Citcom
 
-    form = "mixed" if mixed else "plain"
+    form = {0: "plain", 1: "mixed", 2: "mixed16"}[int(mixed)]
     t0 = time.perf_counter()
     c = Citcom(p, mixed=mixed)
     if not graphs:
@@ -286,8 +286,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="tdep128")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--form", choices=("mixed", "plain"), default="mixed",
-                    help="mixed: FP32 multigrid inside FP64 refinement; plain: the reference's FP64 arithmetic")
+    ap.add_argument("--form", choices=("mixed", "plain", "mixed16"), default="mixed",
+                    help="mixed: FP32 multigrid inside FP64 refinement; plain: the reference's FP64 arithmetic; "
+                         "mixed16: mixed with the fine levels' stiffness stored in FP16, scaled by BI")
     ap.add_argument("--compare-steps", type=int, default=2,
                     help="timed steps of the plain FP64 form run beside the mixed form (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -308,11 +309,13 @@ def main():
     nox, noy, noz = p.dims()
     log(f"config {args.config}: {nox - 1}x{noy - 1}x{noz - 1} elements, {p.levels} levels")
     T = deck.seeded_temperature(p, seed=args.seed) if p.tdepv else deck.perturbed_temperature(p)
-    mixed = args.form == "mixed"
+    mixed = {"plain": 0, "mixed": 1, "mixed16": 2}[args.form]
     main_run = run_form(p, T, mixed, args.steps, args.warmup, not args.no_graphs, profile=True)
-    plain = None
+    plain = fp16 = None
     if mixed and args.compare_steps > 0:
-        plain = run_form(p, T, False, args.compare_steps, 1, not args.no_graphs, profile=False)
+        plain = run_form(p, T, 0, args.compare_steps, 1, not args.no_graphs, profile=False)
+        if mixed == 1:
+            fp16 = run_form(p, T, 2, args.compare_steps, 1, not args.no_graphs, profile=False)
 
     # known-size copy (256 MiB read + 256 MiB written): the reference point that
     # scales hardware byte counters (FETCH_SIZE / WRITE_SIZE) into HBM bytes
@@ -330,14 +333,17 @@ def main():
     log(f"gs_colour: {n_gs} launches, {ms_gs:.1f} ms total, {achieved:.0f} GB/s; "
         f"matvec: {n_mv} launches, {ms_mv:.1f} ms, {(bytes_mv / max(ms_mv, 1e-9)) / 1e6:.0f} GB/s")
     s_per_step = main_run["s_per_step"]
-    kernel = "k_gs_colour<float>" if mixed else "k_gs_colour<double>"
+    kernel = {0: "k_gs_colour<double>", 1: "k_gs_colour<float>", 2: "k_gs_colour<float, _Float16>"}[mixed]
     visc = (f"viscosity exp(-{p.viscE[0]:.4g} T)" + (f", {p.num_mat} layers (visc0 {p.visc0[:p.num_mat]})"
                                                       if p.num_mat > 1 else "")) if p.tdepv else "isoviscous"
     config = {"workload": f"{args.config}: {nox - 1}x{noy - 1}x{noz - 1} elements, {p.levels} MG levels, "
                           f"Ra={p.rayleigh:g}, {visc}, accuracy={p.accuracy:g}",
-              "form": ("FP32 multigrid cycles inside FP64 iterative refinement (FP64 residual, Uzawa CG, "
-                       "coarsest level); FP32 stiffness as in the reference") if mixed else
-                      "plain: FP64 vectors, the reference's FP32 stiffness",
+              "form": {1: "FP32 multigrid cycles inside FP64 iterative refinement (FP64 residual, Uzawa CG, "
+                          "coarsest level); FP32 stiffness as in the reference",
+                       2: "FP32 multigrid cycles inside FP64 iterative refinement; the two finest levels' "
+                          "stiffness rows scaled by BI and stored in FP16 for the cycles (the FP64 residual uses "
+                          "the reference's FP32 stiffness)",
+                       0: "plain: FP64 vectors, the reference's FP32 stiffness"}[mixed],
               "uzawa_iterations_per_step": main_run["uzawa_iterations_per_step"],
               "mg_calls_per_step": main_run["mg_calls_per_step"]}
     if plain is not None:
@@ -346,7 +352,14 @@ def main():
         config["learned_warm_start"] = ("citcom_amd.learn (BF16 MFMA 3-D CNN) is the Uzawa initial guess where no "
                                         "previous solution exists; in this time loop the previous step's U, P are "
                                         "the initial guess, as in the reference, so it is not used here")
-        log(f"plain FP64 form: {plain['s_per_step']:.3f} s/step; mixed form {s_per_step:.3f} s/step")
+        log(f"plain FP64 form: {plain['s_per_step']:.3f} s/step; {args.form} form {s_per_step:.3f} s/step")
+    if fp16 is not None:
+        config["mixed16_s_per_step"] = fp16["s_per_step"]
+        config["mixed16_speedup_vs_plain_fp64"] = plain["s_per_step"] / fp16["s_per_step"]
+        config["mixed16_form"] = ("the same cycles with the two finest levels' stiffness rows scaled by BI and "
+                                  "stored in FP16 (FP32 arithmetic; the FP64 residual keeps the reference's FP32 "
+                                  "stiffness); same Uzawa iterations and multigrid calls")
+        log(f"mixed16 form: {fp16['s_per_step']:.3f} s/step")
     out = {
         "metric": METRIC,
         "value": s_per_step,

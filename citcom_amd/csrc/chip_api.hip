@@ -90,12 +90,15 @@ int prof_flush(chip_ctx *c) {
     return 0;
 }
 
-// algorithmic bytes: 243 FP32 coefficients per node plus the vectors
-// (matvec: read u, write Au; colour pass: read/write d, read F and BI)
+// algorithmic bytes: 243 coefficients per node (FP32, or FP16 where the
+// level has the scaled FP16 stiffness and the cycle runs in FP32) plus the
+// vectors (matvec: read u, write Au; colour pass: read/write d, read F and BI)
 template <typename V>
-constexpr double mv_bytes() { return 243.0 * 4 + 6 * sizeof(V); }
+double coef_bytes(const Level &L) { return (sizeof(V) == 4 && L.Kh) ? 243.0 * 2 : 243.0 * 4; }
 template <typename V>
-constexpr double gs_bytes() { return 243.0 * 4 + 12 * sizeof(V); }
+double mv_bytes(const Level &L) { return coef_bytes<V>(L) + 6 * sizeof(V); }
+template <typename V>
+double gs_bytes(const Level &L) { return coef_bytes<V>(L) + 12 * sizeof(V); }
 
 // the multigrid work vectors of a level in one precision
 template <typename V>
@@ -113,7 +116,7 @@ LV<float> vecs<float>(const Level &L) { return {L.velf, L.resf, L.flf, L.delf, L
 template <typename V>
 void do_matvec(chip_ctx *c, int lev, const V *u, V *Au, const V *rhs = nullptr, V *res = nullptr) {
     const Level &L = c->L[lev];
-    ProfGuard pg(c, 2, mv_bytes<V>() * L.g.nno);
+    ProfGuard pg(c, 2, mv_bytes<V>(L) * L.g.nno);
     matvec(S(c), L, u, Au, rhs, res, pg.a, pg.b);
 }
 
@@ -157,7 +160,7 @@ void do_gs(chip_ctx *c, int lev, V *d, const V *F, V *Ad, int sweeps, int guess,
     const V *BI = vecs<V>(L).BI;
     for (int cyc = 0; cyc < sweeps; cyc++)
         for (int col = 0; col < 8; col++) {
-            ProfGuard pg(c, 1, gs_bytes<V>() * L.g.cn[col]);
+            ProfGuard pg(c, 1, gs_bytes<V>(L) * L.g.cn[col]);
             gs_colour(S(c), L, BI, col, d, F, pg.a, pg.b);
         }
     do_matvec(c, lev, (const V *)d, Ad, rhs, res);
@@ -381,6 +384,8 @@ int rebuild(chip_ctx *c) {
         if (c->L[lev].Ma) affine_build(q, c->L[lev], c->L[lev].affine_sweeps);
     if (c->p.mixed)
         for (int lev = c->levmin; lev <= c->levmax; lev++) to_f32(q, c->L[lev].BIf, c->L[lev].BI, 3L * c->L[lev].g.nno);
+    for (int lev = c->levmin; lev <= c->levmax; lev++)
+        if (c->L[lev].Kh) scale_half(q, c->L[lev]);
     CHIP_CHECK(hipGetLastError());
     return 0;
 }

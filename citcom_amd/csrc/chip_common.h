@@ -45,6 +45,7 @@ struct Level {
     Geo g;
     // operator
     float *K = nullptr;      // [pitch/64][243][64]: 64-node tiles, colour order
+    uint16_t *Kh = nullptr;  // FP16 bits of BI_row * K, same tiling (mixed = 2, fine levels only)
     double *BI = nullptr;    // [3 nno]
     double *BPI = nullptr;   // [nel]
     float *EG = nullptr;     // [nel][24]

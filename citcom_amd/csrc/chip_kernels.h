@@ -68,6 +68,7 @@ void dots(hipStream_t s, chip_ctx *c, int n, int nd, const V *a0, const V *b0, t
 void fill(hipStream_t s, double *v, double val, long n);
 void fill(hipStream_t s, float *v, float val, long n);
 void to_f32(hipStream_t s, float *dst, const double *src, long n);
+void scale_half(hipStream_t s, const Level &L);
 void add_f32(hipStream_t s, double *y, const float *x, long n);   // y += x
 void copy(hipStream_t s, double *dst, const double *src, long n);
 void axpy_scalar(hipStream_t s, double *y, const double *x, double a, long n);   // y += a x

@@ -163,6 +163,14 @@ __device__ __forceinline__ void row_product(const Geo &g, const float *__restric
         g, [kp](int ci) { return kp[ci * 64]; }, k, i, j, [u](int idx) { return u[idx]; }, out);
 }
 
+// the same with the FP16 (row-scaled) coefficients, widened to FP32
+__device__ __forceinline__ void row_product(const Geo &g, const _Float16 *__restrict__ K, int q, int k, int i, int j,
+                                            const float *__restrict__ u, float out[3]) {
+    const _Float16 *kp = K + ktile(q);
+    row_product_t<float>(
+        g, [kp](int ci) { return (float)kp[ci * 64]; }, k, i, j, [u](int idx) { return u[idx]; }, out);
+}
+
 template <typename V>
 __device__ __forceinline__ void strip3(unsigned f, V v[3]) {
     if (f & VBX) v[0] = V(0);
@@ -227,8 +235,8 @@ __device__ __forceinline__ double lane_value(double v, int lane) {
 
 // Plane P = dy+1 of the three rows of node (k, i, j), coefficients kp (its
 // tile column), into P[r]; arithmetic in the vector type V
-template <int P, typename V>
-__device__ __forceinline__ void plane_rows(const Geo &g, const float *__restrict__ kp, int k, int i, int j, int self,
+template <int P, typename V, typename KT>
+__device__ __forceinline__ void plane_rows(const Geo &g, const KT *__restrict__ kp, int k, int i, int j, int self,
                                            const V *__restrict__ u, V out[3]) {
     constexpr int DY = P - 1;
     const bool oy = (unsigned)(j + DY) < (unsigned)g.noy;
@@ -250,7 +258,7 @@ __device__ __forceinline__ void plane_rows(const Geo &g, const float *__restrict
         }
     }
 #pragma unroll
-    for (int c = 0; c < 81; c++) kv[c] = kp[(81 * P + c) * 64];
+    for (int c = 0; c < 81; c++) kv[c] = (float)kp[(81 * P + c) * 64];
     PairSum16<V> ps[3];
 #define CHIP_TERM(B, R)                                                                                            \
     (okb[B] ? kv[9 * (B) + 3 * (R)] * uv[3 * (B)] + kv[9 * (B) + 3 * (R) + 1] * uv[3 * (B) + 1] +                  \
@@ -280,10 +288,11 @@ __device__ __forceinline__ void plane_rows(const Geo &g, const float *__restrict
 // matvec: n_assemble_del2_u (Element_calculations.c:552-621) with strip;
 // optionally res = rhs - Au.  One thread per node: the three rows share the
 // 27 vector gathers.
-template <typename V>
-__global__ void __launch_bounds__(256) k_matvec(Geo g, const float *__restrict__ K, const unsigned *__restrict__ NODE,
+template <typename V, typename KT>
+__global__ void __launch_bounds__(256) k_matvec(Geo g, const KT *__restrict__ K, const unsigned *__restrict__ NODE,
                                                 const V *__restrict__ u, V *__restrict__ Au,
-                                                const V *__restrict__ rhs, V *__restrict__ res) {
+                                                const V *__restrict__ rhs, V *__restrict__ res,
+                                                const V *__restrict__ BIs) {
     const int q = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int col = colour_valid(g, q);
     if (col < 0) return;
@@ -292,6 +301,11 @@ __global__ void __launch_bounds__(256) k_matvec(Geo g, const float *__restrict__
     V out[3];
     row_product(g, K, q, k, i, j, u, out);
     const int n = nid(g, k, i, j);
+    if (BIs) {  // rows stored scaled by BI: K d = (BI K) d / BI
+        out[0] = out[0] / BIs[3 * n];
+        out[1] = out[1] / BIs[3 * n + 1];
+        out[2] = out[2] / BIs[3 * n + 2];
+    }
     strip3(NODE[n], out);
     Au[3 * n] = out[0];
     Au[3 * n + 1] = out[1];
@@ -305,11 +319,27 @@ __global__ void __launch_bounds__(256) k_matvec(Geo g, const float *__restrict__
 
 template <typename V>
 void matvec(hipStream_t s, const Level &L, const V *u, V *Au, const V *rhs, V *res, hipEvent_t e0, hipEvent_t e1) {
+    if constexpr (sizeof(V) == 4) {
+        if (L.Kh) {
+            const _Float16 *Kh = (const _Float16 *)L.Kh;
+            const V *BIs = (const V *)L.BIf;
+            auto kern = k_matvec<V, _Float16>;
+            if (e0)
+                hipExtLaunchKernelGGL(kern, dim3(nblk(L.g.pitch)), dim3(256), 0, s, e0, e1, 0, L.g, Kh, L.NODE, u, Au,
+                                      rhs, res, BIs);
+            else
+                hipLaunchKernelGGL(kern, dim3(nblk(L.g.pitch)), dim3(256), 0, s, L.g, Kh, L.NODE, u, Au, rhs, res, BIs);
+            return;
+        }
+    }
+    const V *none = nullptr;
+    const float *K = L.K;
+    auto kern = k_matvec<V, float>;
     if (e0)
-        hipExtLaunchKernelGGL(k_matvec<V>, dim3(nblk(L.g.pitch)), dim3(256), 0, s, e0, e1, 0, L.g, L.K, L.NODE, u, Au,
-                              rhs, res);
+        hipExtLaunchKernelGGL(kern, dim3(nblk(L.g.pitch)), dim3(256), 0, s, e0, e1, 0, L.g, K, L.NODE, u, Au, rhs, res,
+                              none);
     else
-        hipLaunchKernelGGL(k_matvec<V>, dim3(nblk(L.g.pitch)), dim3(256), 0, s, L.g, L.K, L.NODE, u, Au, rhs, res);
+        hipLaunchKernelGGL(kern, dim3(nblk(L.g.pitch)), dim3(256), 0, s, L.g, K, L.NODE, u, Au, rhs, res, none);
 }
 template void matvec<double>(hipStream_t, const Level &, const double *, double *, const double *, double *,
                              hipEvent_t, hipEvent_t);
@@ -324,8 +354,8 @@ template void matvec<float>(hipStream_t, const Level &, const float *, float *, 
 // together), the partial sums meet in LDS, and wavefront r updates row r.
 // Every row is formed before any unknown is written (workgroup barrier), as
 // the reference updates all three unknowns of a node from the same Ad.
-template <typename V>
-__global__ void __launch_bounds__(192) k_gs_colour(Geo g, int c, const float *__restrict__ K,
+template <typename V, typename KT>
+__global__ void __launch_bounds__(192) k_gs_colour(Geo g, int c, const KT *__restrict__ K,
                                                    const V *__restrict__ BI, V *d, const V *__restrict__ F) {
     __shared__ V Ps[3][3][64];   // [plane][row][node]
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -344,7 +374,7 @@ __global__ void __launch_bounds__(192) k_gs_colour(Geo g, int c, const float *__
     }
     if (act) {
         V out[3];
-        const float *kp = K + ktile(q);
+        const KT *kp = K + ktile(q);
         if (w == 0)
             plane_rows<0>(g, kp, k, i, j, self, (const V *)d, out);
         else if (w == 1)
@@ -359,7 +389,8 @@ __global__ void __launch_bounds__(192) k_gs_colour(Geo g, int c, const float *__
     if (act) {
         const int r = w;
         const V o = (Ps[0][r][lane] + Ps[1][r][lane]) + Ps[2][r][lane];
-        const float t = (float)((Fr - o) * Br);
+        // FP16 rows are stored scaled by BI: t = BI F - (BI K) d
+        const float t = sizeof(KT) == 2 ? (float)(Fr * Br - o) : (float)((Fr - o) * Br);
         d[3 * self + r] = dr + t;
     }
 }
@@ -367,15 +398,53 @@ __global__ void __launch_bounds__(192) k_gs_colour(Geo g, int c, const float *__
 template <typename V>
 void gs_colour(hipStream_t s, const Level &L, const V *BI, int colour, V *d, const V *F, hipEvent_t e0, hipEvent_t e1) {
     const int tiles = (L.g.cn[colour] + 63) / 64;
+    if constexpr (sizeof(V) == 4) {
+        if (L.Kh) {
+            const _Float16 *Kh = (const _Float16 *)L.Kh;
+            auto kern = k_gs_colour<V, _Float16>;
+            if (e0)
+                hipExtLaunchKernelGGL(kern, dim3(tiles), dim3(192), 0, s, e0, e1, 0, L.g, colour, Kh, BI, d, F);
+            else
+                hipLaunchKernelGGL(kern, dim3(tiles), dim3(192), 0, s, L.g, colour, Kh, BI, d, F);
+            return;
+        }
+    }
+    const float *K = L.K;
+    auto kern = k_gs_colour<V, float>;
     if (e0)
-        hipExtLaunchKernelGGL(k_gs_colour<V>, dim3(tiles), dim3(192), 0, s, e0, e1, 0, L.g, colour, L.K, BI, d, F);
+        hipExtLaunchKernelGGL(kern, dim3(tiles), dim3(192), 0, s, e0, e1, 0, L.g, colour, K, BI, d, F);
     else
-        hipLaunchKernelGGL(k_gs_colour<V>, dim3(tiles), dim3(192), 0, s, L.g, colour, L.K, BI, d, F);
+        hipLaunchKernelGGL(kern, dim3(tiles), dim3(192), 0, s, L.g, colour, K, BI, d, F);
 }
 template void gs_colour<double>(hipStream_t, const Level &, const double *, int, double *, const double *, hipEvent_t,
                                 hipEvent_t);
 template void gs_colour<float>(hipStream_t, const Level &, const float *, int, float *, const float *, hipEvent_t,
                                hipEvent_t);
+
+// the FP16 stiffness of the mixed = 2 form: each row scaled by its BI, so
+// the coefficients are O(1) (the diagonal block's diagonal is 1) whatever the
+// viscosity, then rounded to FP16.  Same tiling as K.
+__global__ void __launch_bounds__(TPB) k_scale_half(Geo g, const float *__restrict__ K, const double *__restrict__ BI,
+                                                    _Float16 *__restrict__ Kh) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.pitch) return;
+    const int col = colour_valid(g, q);
+    const size_t base = ktile(q);
+    if (col < 0) {
+        for (int ci = 0; ci < KROW; ci++) Kh[base + (size_t)ci * 64] = (_Float16)0.0f;
+        return;
+    }
+    int k, i, j;
+    colour_kij(g, col, q, k, i, j);
+    const int n = nid(g, k, i, j);
+    const float b[3] = {(float)BI[3 * n], (float)BI[3 * n + 1], (float)BI[3 * n + 2]};
+    for (int ci = 0; ci < KROW; ci++) Kh[base + (size_t)ci * 64] = (_Float16)(K[base + (size_t)ci * 64] * b[(ci / 3) % 3]);
+}
+
+void scale_half(hipStream_t s, const Level &L) {
+    hipLaunchKernelGGL(k_scale_half, dim3(nblk(L.g.pitch)), dim3(TPB), 0, s, L.g, (const float *)L.K,
+                       (const double *)L.BI, (_Float16 *)L.Kh);
+}
 
 // ---------------------------------------------------------------------------
 // A level whose stiffness fits in LDS (the coarsest, 4x4x2 elements at the

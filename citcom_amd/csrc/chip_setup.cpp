@@ -264,6 +264,10 @@ int build_mesh(chip_ctx *c) {
         set_error("rheol " + std::to_string(p.rheol) + " is not supported (0, 3 and 11 are)");
         return -1;
     }
+    if (p.mixed < 0 || p.mixed > 2) {
+        set_error("mixed must be 0, 1 or 2");
+        return -1;
+    }
     if (p.num_mat < 1 || p.num_mat > 4) {
         set_error("num_mat must be in [1, 4]");
         return -1;
@@ -376,6 +380,11 @@ int build_mesh(chip_ctx *c) {
                 dev_alloc(c, &L.BIf, 3 * (size_t)g.nno) ||
                 dev_alloc(c, &L.flf, 3 * (size_t)g.nno) || dev_alloc(c, &L.delf, 3 * (size_t)g.nno) ||
                 dev_alloc(c, &L.AUf, 3 * (size_t)g.nno))
+                return -1;
+            // the FP16 stiffness of the two finest levels, whose colour
+            // passes and matvecs are bandwidth-bound
+            if (p.mixed == 2 && lev >= c->levmax - 1 && lev >= c->levmin + 2 &&
+                dev_alloc(c, &L.Kh, (size_t)KROW * g.pitch))
                 return -1;
         }
         // isoviscous until a viscosity is set

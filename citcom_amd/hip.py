@@ -121,7 +121,7 @@ def _check(rc: int, what: str) -> None:
         raise ChipError(f"{what}: {lib().chip_last_error().decode()}")
 
 
-def problem_struct(p: _deck.Problem, mixed: bool = False):
+def problem_struct(p: _deck.Problem, mixed: int = 0):
     s = ChipProblem()
     keep = [np.ascontiguousarray(c, dtype=np.float32) for c in p.coords]
     s.levels = p.levels
@@ -152,7 +152,9 @@ class Shapes:
 class Citcom:
     """Device-resident hot-path state of one CitcomCU problem."""
 
-    def __init__(self, problem: _deck.Problem, mixed: bool = False):
+    def __init__(self, problem: _deck.Problem, mixed: int = 0):
+        """mixed: 0 the reference's arithmetic, 1 FP32 multigrid inside FP64
+        refinement, 2 the same with the fine levels' stiffness in FP16"""
         self.p = problem
         self.levmax = problem.levels - 1
         s, keep = problem_struct(problem, mixed)

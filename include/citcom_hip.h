@@ -48,7 +48,9 @@ typedef struct chip_problem
 	double Q0;
 	float fixed_timestep;
 	int mixed;	/* 0: reference arithmetic (FP64 vectors, FP32 stored K);
-			   1: FP32 multigrid inside FP64 iterative refinement */
+			   1: FP32 multigrid inside FP64 iterative refinement;
+			   2: as 1, with the two finest levels' stiffness rows scaled
+			      by BI and stored in FP16 for the multigrid cycles */
 } chip_problem;
 
 typedef struct chip_ctx chip_ctx;

@@ -1,6 +1,7 @@
-"""The mixed-precision form (FP32 multigrid cycles inside FP64 iterative
+"""The mixed-precision forms (FP32 multigrid cycles inside FP64 iterative
 refinement: the residual, the solution update, the Uzawa CG and the coarsest
-level stay FP64) against the reference's FP64 runs.
+level stay FP64; mixed = 2 also stores the fine levels' stiffness rows,
+scaled by BI, in FP16 for the cycles) against the reference's FP64 runs.
 
 Refinement recomputes the residual in FP64 after every multigrid call, so
 the mixed form stops on the same criterion as the plain one and converges
@@ -32,28 +33,31 @@ def test_mixed_stokes_against_reference(name, fixture, tol):
     p = problem(name)
     g = golden(fixture)
     res = {}
-    for mixed in (False, True):
+    for mixed in (0, 1, 2):
         c = Citcom(p, mixed=mixed)
         c.initialize(deck.perturbed_temperature(p))
         st = c.general_stokes_solver()
         res[mixed] = (c.download("U"), c.download("P"), st)
         c.close()
-    U, P, st = res[True]
-    eu, ep = rel_l2(U, g["step0_U"]), rel_l2(P, g["step0_P"])
-    eu0, ep0 = rel_l2(res[False][0], g["step0_U"]), rel_l2(res[False][1], g["step0_P"])
-    say(f"{name} mixed: {st['iterations']} Uzawa its, {st['mg_cycles']} MG calls; rel L2 vs reference U {eu:.2e} "
-        f"P {ep:.2e} (plain FP64: {res[False][2]['iterations']} its, U {eu0:.2e} P {ep0:.2e})")
-    assert eu < tol and ep < tol
+    eu0, ep0 = rel_l2(res[0][0], g["step0_U"]), rel_l2(res[0][1], g["step0_P"])
+    for mixed in (1, 2):
+        U, P, st = res[mixed]
+        eu, ep = rel_l2(U, g["step0_U"]), rel_l2(P, g["step0_P"])
+        say(f"{name} mixed={mixed}: {st['iterations']} Uzawa its, {st['mg_cycles']} MG calls; rel L2 vs reference "
+            f"U {eu:.2e} P {ep:.2e} (plain FP64: {res[0][2]['iterations']} its, U {eu0:.2e} P {ep0:.2e})")
+        assert eu < tol and ep < tol
 
 
-def test_mixed_iso16_trajectory():
-    """16^3, Ra 1e4, accuracy 1e-8: 200 steps in the mixed form against the
-    reference's run (T, U, P to 1e-6, Nu to 1e-5)."""
+@pytest.mark.parametrize("mixed", [1, 2])
+def test_mixed_iso16_trajectory(mixed):
+    """16^3, Ra 1e4, accuracy 1e-8: 200 steps in the mixed form (2: with the
+    FP16 fine-level stiffness) against the reference's run (T, U, P to 1e-6,
+    Nu to 1e-5)."""
     from citcom_amd import deck
     from citcom_amd.hip import Citcom
     p = problem("iso16@1e-8")
     g = golden("iso16_at_1e-8_run.npz")
-    c = Citcom(p, mixed=True)
+    c = Citcom(p, mixed=mixed)
     c.initialize(deck.perturbed_temperature(p))
     c.general_stokes_solver()
     steps = g["steps"]
@@ -66,6 +70,6 @@ def test_mixed_iso16_trajectory():
     eU = rel_l2(c.download("U"), g["final_U"])
     eP = rel_l2(c.download("P"), g["final_P"])
     c.close()
-    say(f"iso16@1e-8 mixed 200 steps: max rel dNu {worst_nu:.2e}, final T {eT:.2e} U {eU:.2e} P {eP:.2e}")
+    say(f"iso16@1e-8 mixed={mixed} 200 steps: max rel dNu {worst_nu:.2e}, final T {eT:.2e} U {eU:.2e} P {eP:.2e}")
     assert worst_nu < 1e-5
     assert eT < 1e-6 and eU < 1e-6 and eP < 1e-6
