@@ -283,11 +283,13 @@ def test_smoother_kernels_bitwise(name, small, pyoracle):
     c.close()
 
 
-def test_unsupported_viscosity_law_is_refused():
-    """Viscosity laws other than rheol 0, 3 and 11 are refused at setup, not
-    computed as another law."""
+def test_unsupported_problem_is_refused():
+    """Viscosity laws other than rheol 0, 3 and 11, and unknown precision
+    forms, are refused at setup, not computed as something else."""
     import dataclasses
     from citcom_amd.hip import ChipError, Citcom
     p = dataclasses.replace(problem("tiny_tdep"), rheol=1)
     with pytest.raises(ChipError, match="rheol 1 is not supported"):
         Citcom(p)
+    with pytest.raises(ChipError, match="mixed must be 0, 1 or 2"):
+        Citcom(problem("tiny_tdep"), mixed=3)
