@@ -333,7 +333,9 @@ def main():
     log(f"gs_colour: {n_gs} launches, {ms_gs:.1f} ms total, {achieved:.0f} GB/s; "
         f"matvec: {n_mv} launches, {ms_mv:.1f} ms, {(bytes_mv / max(ms_mv, 1e-9)) / 1e6:.0f} GB/s")
     s_per_step = main_run["s_per_step"]
-    kernel = {0: "k_gs_colour<double>", 1: "k_gs_colour<float>", 2: "k_gs_colour<float, _Float16>"}[mixed]
+    # <vector type, stiffness type>; mixed16 runs the FP16 variant on the two
+    # finest levels and the FP32 one below, both averaged here
+    kernel = {0: "k_gs_colour<double, float>", 1: "k_gs_colour<float, float>", 2: "k_gs_colour<float, _Float16>"}[mixed]
     visc = (f"viscosity exp(-{p.viscE[0]:.4g} T)" + (f", {p.num_mat} layers (visc0 {p.visc0[:p.num_mat]})"
                                                       if p.num_mat > 1 else "")) if p.tdepv else "isoviscous"
     config = {"workload": f"{args.config}: {nox - 1}x{noy - 1}x{noz - 1} elements, {p.levels} MG levels, "
