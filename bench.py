@@ -115,17 +115,19 @@ def cpu_baseline_port(name, seed, mg_calls_per_step):
                        f"{mg_calls_per_step:.1f} multigrid calls per step (as counted on the GPU)")}
 
 
-def measured_traffic(kernel, config):
-    """HBM bytes per launch of `kernel` on workload `config` from the committed
-    counter summary (profiles/k_gs_colour_traffic.json: rocprofv3 FETCH_SIZE
-    and WRITE_SIZE runs, scaled by what the same counters read for a copy of
-    known size), or None where it was not measured."""
+def measured_traffic(kernel, config, field="hbm_bytes_per_launch"):
+    """HBM bytes per launch of `kernel` on workload `config` (all its colour
+    passes, or with field "finest_level_hbm_bytes_per_launch" the finest
+    level's) from the committed counter summary
+    (profiles/k_gs_colour_traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE
+    runs, scaled by what the same counters read for a copy of known size), or
+    None where it was not measured."""
     path = os.path.join(ROOT, "profiles", "k_gs_colour_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         rec = json.load(fh)
-    per_kernel = rec.get("hbm_bytes_per_launch")
+    per_kernel = rec.get(field)
     if rec.get("config") != config or not isinstance(per_kernel, dict):
         return None
     return per_kernel.get(kernel)
@@ -274,6 +276,12 @@ def run_form(p, T, mixed, steps, warmup, graphs, profile):
         c.profile("matvec")
         c.timestep()
         out["mv"] = c.profile_read()
+        # the finest level alone: its colour passes are the bandwidth-bound ones
+        c.set_option("profile_level", p.levmax)
+        c.profile("gs_colour")
+        c.timestep()
+        out["gs_finest"] = c.profile_read()
+        c.set_option("profile_level", -1)
         c.profile(None)
     c.close()
     return out
@@ -328,6 +336,9 @@ def main():
 
     n_gs, ms_gs, bytes_gs = main_run["gs"]
     n_mv, ms_mv, bytes_mv = main_run["mv"]
+    n_f, ms_f, bytes_f = main_run["gs_finest"]
+    ach_f = (bytes_f / max(n_f, 1)) / ((ms_f / max(n_f, 1)) * 1e-3) / 1e9
+    log(f"finest-level colour passes: {n_f} launches, {ms_f / max(n_f, 1) * 1e3:.1f} us each, {ach_f:.0f} GB/s")
     avg_ms = ms_gs / max(n_gs, 1)
     achieved = (bytes_gs / max(n_gs, 1)) / (avg_ms * 1e-3) / 1e9
     log(f"gs_colour: {n_gs} launches, {ms_gs:.1f} ms total, {achieved:.0f} GB/s; "
@@ -362,6 +373,9 @@ def main():
                                   "stored in FP16 (FP32 arithmetic; the FP64 residual keeps the reference's FP32 "
                                   "stiffness); same Uzawa iterations and multigrid calls")
         log(f"mixed16 form: {fp16['s_per_step']:.3f} s/step")
+    finest = {"achieved": ach_f, "frac": ach_f / PEAK_HBM_GBS, "launches": n_f,
+              "avg_launch_ms": ms_f / max(n_f, 1), "algorithmic_bytes_per_launch": bytes_f / max(n_f, 1),
+              "traffic": measured_traffic(kernel, args.config, "finest_level_hbm_bytes_per_launch")}
     out = {
         "metric": METRIC,
         "value": s_per_step,
@@ -378,7 +392,7 @@ def main():
         "config": config,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(kernel, args.config), "kernel": kernel,
-                     "launches": n_gs, "avg_launch_ms": avg_ms},
+                     "launches": n_gs, "avg_launch_ms": avg_ms, "finest_level": finest},
     }
     if not args.no_cpu_baseline:
         log("cpu baseline ...")

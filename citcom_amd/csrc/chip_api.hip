@@ -61,8 +61,8 @@ int sync_minmax(chip_ctx *c, float *mn, float *mx) {
 // stamps them with the kernel's own start and end (hipExtLaunchKernelGGL).
 struct ProfGuard {
     hipEvent_t a = nullptr, b = nullptr;
-    ProfGuard(chip_ctx *c, int kernel, double bytes) {
-        if (c->prof.kernel != kernel) return;
+    ProfGuard(chip_ctx *c, int kernel, double bytes, int lev) {
+        if (c->prof.kernel != kernel || (c->prof.level >= 0 && c->prof.level != lev)) return;
         if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
             a = b = nullptr;
             return;
@@ -116,7 +116,7 @@ LV<float> vecs<float>(const Level &L) { return {L.velf, L.resf, L.flf, L.delf, L
 template <typename V>
 void do_matvec(chip_ctx *c, int lev, const V *u, V *Au, const V *rhs = nullptr, V *res = nullptr) {
     const Level &L = c->L[lev];
-    ProfGuard pg(c, 2, mv_bytes<V>(L) * L.g.nno);
+    ProfGuard pg(c, 2, mv_bytes<V>(L) * L.g.nno, lev);
     matvec(S(c), L, u, Au, rhs, res, pg.a, pg.b);
 }
 
@@ -160,7 +160,7 @@ void do_gs(chip_ctx *c, int lev, V *d, const V *F, V *Ad, int sweeps, int guess,
     const V *BI = vecs<V>(L).BI;
     for (int cyc = 0; cyc < sweeps; cyc++)
         for (int col = 0; col < 8; col++) {
-            ProfGuard pg(c, 1, gs_bytes<V>(L) * L.g.cn[col]);
+            ProfGuard pg(c, 1, gs_bytes<V>(L) * L.g.cn[col], lev);
             gs_colour(S(c), L, BI, col, d, F, pg.a, pg.b);
         }
     do_matvec(c, lev, (const V *)d, Ad, rhs, res);
@@ -858,6 +858,10 @@ int chip_set_option(chip_ctx *c, const char *name, int value) {
     const std::string n(name ? name : "");
     if (n == "graphs") {
         c->use_graphs = value != 0;
+        return 0;
+    }
+    if (n == "profile_level") {  // restrict chip_profile_enable to one level (-1: all)
+        c->prof.level = value;
         return 0;
     }
     if (n == "print_convergence") {

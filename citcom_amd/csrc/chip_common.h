@@ -77,6 +77,7 @@ struct Level {
 
 struct Prof {
     int kernel = 0;          // 0 off, 1 gs_colour, 2 matvec
+    int level = -1;          // >= 0: only launches on this level
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
     std::vector<double> bytes;
     long launches = 0;

@@ -174,6 +174,8 @@ long chip_device_bytes(chip_ctx *c);
  * "small_levels": 1 (default) smooths levels whose stiffness fits in LDS in
  * one single-workgroup kernel per gauss_seidel call, 0 launches one kernel
  * per colour pass on every level; results are identical.
+ * "profile_level": >= 0 restricts chip_profile_enable to that level's
+ * launches, -1 (default) profiles every level.
  * "print_convergence": 1 prints one line per Uzawa iteration and per inner
  * solve on stderr (generate_log_message, Stokes_flow_Incomp.c:501-526;
  * General_matrix_functions.c:494-498). */
