@@ -331,7 +331,23 @@ def main():
     dst = torch.empty_like(src)
     for _ in range(3):
         dst.copy_(src)
+    src.sum()
     torch.cuda.synchronize()
+    # the practical bandwidth the same device reaches on plain streams (the
+    # second denominator beside the 8 TB/s data-sheet peak): a copy (read +
+    # write) and a read-only reduction over 256 MiB, timed on torch's stream
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    e0.record()
+    for _ in range(5):
+        dst.copy_(src)
+    e1.record()
+    for _ in range(5):
+        src.sum()
+    e2.record()
+    torch.cuda.synchronize()
+    stream = {"copy_gbs": 5 * 2 * COPY_PROBE_BYTES / (e0.elapsed_time(e1) * 1e-3) / 1e9,
+              "read_gbs": 5 * COPY_PROBE_BYTES / (e1.elapsed_time(e2) * 1e-3) / 1e9}
+    log(f"device streams: copy {stream['copy_gbs']:.0f} GB/s, read {stream['read_gbs']:.0f} GB/s")
     del src, dst
 
     n_gs, ms_gs, bytes_gs = main_run["gs"]
@@ -392,7 +408,7 @@ def main():
         "config": config,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(kernel, args.config), "kernel": kernel,
-                     "launches": n_gs, "avg_launch_ms": avg_ms, "finest_level": finest},
+                     "launches": n_gs, "avg_launch_ms": avg_ms, "finest_level": finest, "device_streams": stream},
     }
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
