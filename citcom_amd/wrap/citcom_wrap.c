@@ -4,11 +4,19 @@
  *
  * Linked with -Wl,--wrap=solve_constrained_flow_iterative
  *             -Wl,--wrap=PG_timestep
+ *             -Wl,--wrap=construct_stiffness_B_matrix
  * the driver's call of solve_constrained_flow_iterative (Drive_solvers.c:130)
  * and the PG_timestep pointer installed as E->next_buoyancy_field
  * (Convection.c:93, called at Citcom.c:125) reach the functions below, which
  * hand E's arrays to the GPU through the C ABI of citcom_hip.h and write the
- * results back where the reference writes them.  __real_* stay linked.
+ * results back where the reference writes them.  The driver's host rebuild of
+ * the operators (Drive_solvers.c:126), which only the replaced solver reads
+ * after the first step, runs once; the device rebuilds them from the
+ * uploaded viscosity.  __real_* stay linked.
+ *
+ * CITCOM_HIP_FORM=mixed (or mixed16) in the environment selects the mixed
+ * precision form (chip_problem.mixed = 1, 2); the default is the reference's
+ * arithmetic.
  *
  * Compiled against the reference's own headers with the reference's -D flags
  * (struct All_variables must have the driver's layout, global_defs.h).
@@ -125,6 +133,18 @@ static void setup(struct All_variables *E)
 	p.Q0 = E->control.Q0;
 	p.fixed_timestep = E->advection.fixed_timestep;
 	p.mixed = 0;
+	{
+		const char *form = getenv("CITCOM_HIP_FORM");
+		if(form && strcmp(form, "mixed") == 0)
+			p.mixed = 1;
+		else if(form && strcmp(form, "mixed16") == 0)
+			p.mixed = 2;
+		else if(form && *form && strcmp(form, "plain") != 0)
+		{
+			fprintf(stderr, "citcom_hip: CITCOM_HIP_FORM=%s (plain, mixed or mixed16)\n", form);
+			parallel_process_termination();
+		}
+	}
 
 	CHECK(chip_create(&p, &ctx));
 	free(xc);
@@ -132,6 +152,20 @@ static void setup(struct All_variables *E)
 	free(zc);
 	vpack = malloc(sizeof(float) * 3 * (size_t)E->lmesh.nno);
 	atexit(release);
+}
+
+void __real_construct_stiffness_B_matrix(struct All_variables *E);
+
+/* Construct_arrays.c:834-889.  The host operators are built once: besides
+ * the solver (replaced here), the reference reads only the geometry-only
+ * elt_del they include (topography output, Topo_gravity.c:157).  Every later
+ * rebuild happens on the device in the solve below. */
+void __wrap_construct_stiffness_B_matrix(struct All_variables *E)
+{
+	static int built;
+	if(!built)
+		__real_construct_stiffness_B_matrix(E);
+	built = 1;
 }
 
 /* Stokes_flow_Incomp.c:50-90 */
