@@ -267,6 +267,7 @@ def workload(name: str) -> dict[str, object]:
     """Deck keys of a named workload.
 
     tiny / tiny_tdep  8x8x4 elements, 3 levels (test size)
+    tiny2  4x4x2 elements, 2 levels: the shortest multigrid hierarchy
     tiny_layer  the same grid with two viscosity layers, contrast 1e5, clips
     iso16  16^3 elements, 3 levels, isoviscous Ra=1e4, 200 steps
     tdep64  64^3 elements, 5 levels, Ra=1e6, eta = exp(-ln(1e4) T), 500 steps
@@ -293,6 +294,11 @@ def workload(name: str) -> dict[str, object]:
         d.update(_grid((1.0, 1.0, 0.5), (2, 2, 1), 3))
         d.update({"mgunitx": 2, "mgunity": 2, "mgunitz": 1, "levels": 3, "maxstep": 5, "finetunedt": 0.75})
         d.update(_iso(1e5))
+    elif name == "tiny2":
+        d.update(_grid((1.0, 1.0, 0.5), (2, 2, 1), 2))
+        d.update({"mgunitx": 2, "mgunity": 2, "mgunitz": 1, "levels": 2, "maxstep": 5, "finetunedt": 0.75})
+        d.update(_tdep(1e5, 100.0))
+        d.update({"num_perturbations": 1, "perturbmag": 0.01, "perturbk": 1.0, "perturbl": 1.0, "perturbm": 0.0})
     elif name == "tiny_tdep":
         d.update(_grid((1.0, 1.0, 0.5), (2, 2, 1), 3))
         d.update({"mgunitx": 2, "mgunity": 2, "mgunitz": 1, "levels": 3, "maxstep": 5, "finetunedt": 0.75})

@@ -15,7 +15,7 @@ import refdump
 
 pytestmark = pytest.mark.gpu
 
-DECKS = ["tiny", "tiny_tdep", "tiny_layer"]
+DECKS = ["tiny", "tiny_tdep", "tiny_layer", "tiny2"]
 
 
 def say(*a):

@@ -16,7 +16,7 @@ import pytest
 from conftest import golden, initial_temperature, problem, rel_l2
 import refdump
 
-DECKS = ["tiny", "tiny_tdep", "tiny_layer"]
+DECKS = ["tiny", "tiny_tdep", "tiny_layer", "tiny2"]
 # the reference's own example decks (examples/Busse1993): rigid lids, a
 # non-cubic box, a graded z grid, the Arrhenius law, the shipped restart field
 EXAMPLES = ["busse1a", "busse2"]
