@@ -879,23 +879,30 @@ __global__ void __launch_bounds__(TPB) k_grad(Geo g, const float *__restrict__ E
     if (n >= g.nno) return;
     int k, i, j;
     node_kij(g, n, k, i, j);
+    // all eight elements' P and g entries are loaded first (absent elements
+    // read element 0 and are masked), then summed in ascending element order
+    double pe[8];
+    float eg[8][3];
+    bool ok[8];
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        const int ez = k - 1 + (m & 1), ex = i - 1 + ((m >> 1) & 1), ey = j - 1 + (m >> 2);
+        ok[m] = (unsigned)ez < (unsigned)g.elz && (unsigned)ex < (unsigned)g.elx && (unsigned)ey < (unsigned)g.ely;
+        const int e = ok[m] ? ez + g.elz * (ex + g.elx * ey) : 0;
+        const int a = cLoc[(k - ez) + 2 * (i - ex) + 4 * (j - ey)];
+        const float *ge = EG + (size_t)e * EG_SZ + 3 * ((ok[m] ? a : 1) - 1);
+        pe[m] = P[e];
+        eg[m][0] = ge[0];
+        eg[m][1] = ge[1];
+        eg[m][2] = ge[2];
+    }
     double acc[3] = {0.0, 0.0, 0.0};
-    for (int ey = j - 1; ey <= j; ey++) {
-        if ((unsigned)ey >= (unsigned)g.ely) continue;
-        for (int ex = i - 1; ex <= i; ex++) {
-            if ((unsigned)ex >= (unsigned)g.elx) continue;
-            for (int ez = k - 1; ez <= k; ez++) {
-                if ((unsigned)ez >= (unsigned)g.elz) continue;
-                const int e = ez + g.elz * (ex + g.elx * ey);
-                const double pe = P[e];
-                if (0.0 == pe) continue;
-                const int a = cLoc[(k - ez) + 2 * (i - ex) + 4 * (j - ey)];
-                const float *eg = EG + (size_t)e * EG_SZ + 3 * (a - 1);
-                acc[0] += eg[0] * pe;
-                acc[1] += eg[1] * pe;
-                acc[2] += eg[2] * pe;
-            }
-        }
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        if (!ok[m] || 0.0 == pe[m]) continue;   // assemble_grad_p skips P[e] == 0 (:746)
+        acc[0] += eg[m][0] * pe[m];
+        acc[1] += eg[m][1] * pe[m];
+        acc[2] += eg[m][2] * pe[m];
     }
     strip3(NODE[n], acc);
     gradP[3 * n] = acc[0];
