@@ -739,30 +739,53 @@ __global__ void __launch_bounds__(TPB) k_project(Geo gf, Geo gc, const float *__
     if (n >= gc.nno) return;
     int k, i, j;
     node_kij(gc, n, k, i, j);
-    V acc[3] = {V(0), V(0), V(0)};
-    for (int ey = j - 1; ey <= j; ey++) {
-        if ((unsigned)ey >= (unsigned)gc.ely) continue;
-        for (int ex = i - 1; ex <= i; ex++) {
-            if ((unsigned)ex >= (unsigned)gc.elx) continue;
-            for (int ez = k - 1; ez <= k; ez++) {
-                if ((unsigned)ez >= (unsigned)gc.elz) continue;
-                const int a = cLoc[(k - ez) + 2 * (i - ex) + 4 * (j - ey)];
-                const int el = ez + gc.elz * (ex + gc.elx * ey);
-                const int fz = 2 * ez + cOffz[a], fx = 2 * ex + cOffx[a], fy = 2 * ey + cOffy[a];
-                V av0 = V(0), av1 = V(0), av2 = V(0);
+    // every load first: the 27 fine nodes 2k-1..2k+1 (x, y likewise) that the
+    // coarse node's elements reach, and the element weights; then the sums
+    // in the reference's order (elements ascending, their 8 fine nodes in
+    // local order)
+    V fv[27][3];
 #pragma unroll
-                for (int b = 1; b <= 8; b++) {
-                    const int fn = nid(gf, fz + cOffz[b], fx + cOffx[b], fy + cOffy[b]);
-                    av0 += AU[3 * fn];
-                    av1 += AU[3 * fn + 1];
-                    av2 += AU[3 * fn + 2];
-                }
-                const float w = TWW[(size_t)el * TWW_SZ + a];
-                acc[0] += w * av0;
-                acc[1] += w * av1;
-                acc[2] += w * av2;
-            }
+    for (int t = 0; t < 27; t++) {
+        const int fz = 2 * k - 1 + t % 3, fx = 2 * i - 1 + (t / 3) % 3, fy = 2 * j - 1 + t / 9;
+        const bool in = (unsigned)fz < (unsigned)gf.noz && (unsigned)fx < (unsigned)gf.nox &&
+                        (unsigned)fy < (unsigned)gf.noy;
+        const int fn = in ? nid(gf, fz, fx, fy) : nid(gf, 2 * k, 2 * i, 2 * j);
+        fv[t][0] = AU[3 * fn];
+        fv[t][1] = AU[3 * fn + 1];
+        fv[t][2] = AU[3 * fn + 2];
+    }
+    float w[8];
+    bool ok[8];
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        const int ez = k - 1 + (m & 1), ex = i - 1 + ((m >> 1) & 1), ey = j - 1 + (m >> 2);
+        ok[m] = (unsigned)ez < (unsigned)gc.elz && (unsigned)ex < (unsigned)gc.elx && (unsigned)ey < (unsigned)gc.ely;
+        const int a = cLoc[(k - ez) + 2 * (i - ex) + 4 * (j - ey)];
+        w[m] = ok[m] ? TWW[(size_t)(ez + gc.elz * (ex + gc.elx * ey)) * TWW_SZ + a] : 0.0f;
+    }
+    // offsets of local node b (the cOff tables, as compile-time constants)
+    constexpr int OZ[9] = {0, 0, 0, 0, 0, 1, 1, 1, 1};
+    constexpr int OX[9] = {0, 0, 1, 1, 0, 0, 1, 1, 0};
+    constexpr int OY[9] = {0, 0, 0, 1, 1, 0, 0, 1, 1};
+    V acc[3] = {V(0), V(0), V(0)};
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        if (!ok[m]) continue;
+        // element m = (k-1+mz, i-1+mx, j-1+my) holds the coarse node at local
+        // offset (1-mz, 1-mx, 1-my); its fine node b is 2 e + that offset +
+        // off(b), i.e. (mz + OZ[b], ...) from 2k - 1
+        const int oz = m & 1, ox = (m >> 1) & 1, oy = m >> 2;
+        V av0 = V(0), av1 = V(0), av2 = V(0);
+#pragma unroll
+        for (int b = 1; b <= 8; b++) {
+            const int t = (oz + OZ[b]) + 3 * (ox + OX[b]) + 9 * (oy + OY[b]);
+            av0 += fv[t][0];
+            av1 += fv[t][1];
+            av2 += fv[t][2];
         }
+        acc[0] += w[m] * av0;
+        acc[1] += w[m] * av1;
+        acc[2] += w[m] * av2;
     }
     const float m = MASS[n];
     acc[0] = acc[0] * m;
