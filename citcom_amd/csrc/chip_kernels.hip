@@ -809,37 +809,6 @@ template void project<float>(hipStream_t, const Level &, const Level &, const fl
 // injection, then linear fills along x, z, y in the reference's order,
 // optionally followed by multi_grid's strip_bcs_from_residual
 template <typename V>
-__device__ __forceinline__ void line_x(const Geo &gc, const float *wx, const V *AD, int k, int i, int j, V v[3]) {
-    // k, j even fine indices; i fine x index
-    if (!(i & 1)) {
-        const int cn = nid(gc, k >> 1, i >> 1, j >> 1);
-        v[0] = AD[3 * cn];
-        v[1] = AD[3 * cn + 1];
-        v[2] = AD[3 * cn + 2];
-    } else {
-        const float n1 = wx[2 * i], n2 = wx[2 * i + 1];
-        const int c1 = nid(gc, k >> 1, (i - 1) >> 1, j >> 1), c2 = nid(gc, k >> 1, (i + 1) >> 1, j >> 1);
-#pragma unroll
-        for (int d = 0; d < 3; d++) v[d] = n1 * AD[3 * c1 + d] + n2 * AD[3 * c2 + d];
-    }
-}
-
-template <typename V>
-__device__ __forceinline__ void plane_xz(const Geo &gc, const float *wx, const float *wz, const V *AD, int k, int i,
-                                         int j, V v[3]) {
-    if (!(k & 1)) {
-        line_x(gc, wx, AD, k, i, j, v);
-    } else {
-        V a[3], b[3];
-        line_x(gc, wx, AD, k - 1, i, j, a);
-        line_x(gc, wx, AD, k + 1, i, j, b);
-        const float n1 = wz[2 * k], n2 = wz[2 * k + 1];
-#pragma unroll
-        for (int d = 0; d < 3; d++) v[d] = n1 * a[d] + n2 * b[d];
-    }
-}
-
-template <typename V>
 __global__ void __launch_bounds__(TPB) k_interp(Geo gc, Geo gf, const float *__restrict__ iw,
                                                 const unsigned *__restrict__ NODE, const V *__restrict__ AD,
                                                 V *__restrict__ AU, int do_strip) {
@@ -848,16 +817,37 @@ __global__ void __launch_bounds__(TPB) k_interp(Geo gc, Geo gf, const float *__r
     int k, i, j;
     node_kij(gf, n, k, i, j);
     const float *wx = iw, *wz = iw + 2 * gf.nox, *wy = iw + 2 * (gf.nox + gf.noz);
-    V v[3];
-    if (!(j & 1)) {
-        plane_xz(gc, wx, wz, AD, k, i, j, v);
-    } else {
-        V a[3], b[3];
-        plane_xz(gc, wx, wz, AD, k, i, j - 1, a);
-        plane_xz(gc, wx, wz, AD, k, i, j + 1, b);
-        const float n1 = wy[2 * j], n2 = wy[2 * j + 1];
+    // the (up to) 8 coarse nodes around the fine node are loaded together,
+    // then the linear fills along x, then z, then y: an odd fine index
+    // interpolates its two coarse neighbours n1 a + n2 b, an even one takes
+    // its single coarse value
+    const int kc[2] = {k >> 1, (k + 1) >> 1}, ic[2] = {i >> 1, (i + 1) >> 1}, jc[2] = {j >> 1, (j + 1) >> 1};
+    V c[2][2][2][3];   // [y][z][x][component]
 #pragma unroll
-        for (int d = 0; d < 3; d++) v[d] = n1 * a[d] + n2 * b[d];
+    for (int yy = 0; yy < 2; yy++)
+#pragma unroll
+        for (int zz = 0; zz < 2; zz++)
+#pragma unroll
+            for (int xx = 0; xx < 2; xx++) {
+                const int cn = nid(gc, kc[zz], ic[xx], jc[yy]);
+#pragma unroll
+                for (int d = 0; d < 3; d++) c[yy][zz][xx][d] = AD[3 * cn + d];
+            }
+    const float nx1 = wx[2 * i], nx2 = wx[2 * i + 1], nz1 = wz[2 * k], nz2 = wz[2 * k + 1];
+    const float ny1 = wy[2 * j], ny2 = wy[2 * j + 1];
+    V v[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        V lx[2][2];
+#pragma unroll
+        for (int yy = 0; yy < 2; yy++)
+#pragma unroll
+            for (int zz = 0; zz < 2; zz++)
+                lx[yy][zz] = (i & 1) ? nx1 * c[yy][zz][0][d] + nx2 * c[yy][zz][1][d] : c[yy][zz][0][d];
+        V pz[2];
+#pragma unroll
+        for (int yy = 0; yy < 2; yy++) pz[yy] = (k & 1) ? nz1 * lx[yy][0] + nz2 * lx[yy][1] : lx[yy][0];
+        v[d] = (j & 1) ? ny1 * pz[0] + ny2 * pz[1] : pz[0];
     }
     if (do_strip) strip3(NODE[n], v);
     AU[3 * n] = v[0];
