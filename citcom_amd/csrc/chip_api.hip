@@ -156,8 +156,13 @@ void do_gs(chip_ctx *c, int lev, V *d, const V *F, V *Ad, int sweeps, int guess,
             return;
         }
     }
-    if (!guess) fill(S(c), d, V(0), 3L * L.g.nno);
     const V *BI = vecs<V>(L).BI;
+    if (c->small_levels && c->gs_wg && gs_wg_ok<V>(L)) {
+        ProfGuard pg(c, 3, gs_bytes<V>(L) * L.g.nno * sweeps + mv_bytes<V>(L) * L.g.nno, lev);
+        gs_wg(S(c), L, BI, d, F, Ad, rhs, res, sweeps, guess, pg.a, pg.b);
+        return;
+    }
+    if (!guess) fill(S(c), d, V(0), 3L * L.g.nno);
     for (int cyc = 0; cyc < sweeps; cyc++)
         for (int col = 0; col < 8; col++) {
             ProfGuard pg(c, 1, gs_bytes<V>(L) * L.g.cn[col], lev);
@@ -832,7 +837,7 @@ int chip_profile_enable(chip_ctx *c, const char *kernel, int on) {
         c->prof.kernel = 0;
         return 0;
     }
-    c->prof.kernel = (k == "gs_colour") ? 1 : (k == "matvec") ? 2 : 0;
+    c->prof.kernel = (k == "gs_colour") ? 1 : (k == "matvec") ? 2 : (k == "gs_wg") ? 3 : 0;
     c->prof.launches = 0;
     c->prof.total_ms = c->prof.total_bytes = 0.0;
     if (!c->prof.kernel) {
@@ -868,8 +873,8 @@ int chip_set_option(chip_ctx *c, const char *name, int value) {
         c->print_convergence = value != 0;
         return 0;
     }
-    if (n == "small_levels") {
-        c->small_levels = value != 0;
+    if (n == "small_levels" || n == "gs_wg") {
+        (n == "small_levels" ? c->small_levels : c->gs_wg) = value != 0;
         if (c->mg_graph) {   // the captured kernel sequence changes
             CHIP_CHECK(hipStreamSynchronize(c->stream));
             CHIP_CHECK(hipGraphExecDestroy(c->mg_graph));

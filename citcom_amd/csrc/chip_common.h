@@ -122,6 +122,7 @@ struct chip_ctx {
     bool built = false;      // operators constructed at least once
     bool use_graphs = true;  // replay multi_grid as a captured HIP graph
     bool small_levels = true;  // single-workgroup smoother on small levels
+    bool gs_wg = true;         // ... including levels whose stiffness is read through L2
     bool print_convergence = false;  // per-iteration lines on stderr, as E->control.print_convergence
     hipGraphExec_t mg_graph = nullptr;
     chip::Prof prof;

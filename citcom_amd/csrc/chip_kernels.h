@@ -36,6 +36,14 @@ void gs_small(hipStream_t s, const Level &L, double *d, const double *F, double 
 // 1: the level's stiffness and vectors fit in LDS (gs_small), 0: one launch
 // per colour pass
 int gs_small_kind(const Level &L);
+// a whole gauss_seidel call in one workgroup with d in LDS and the FP32
+// stiffness read through L2, then Ad = K d (and res = rhs - Ad if res):
+// for levels gs_wg_ok accepts (small enough, no FP16 stiffness)
+template <typename V>
+bool gs_wg_ok(const Level &L);
+template <typename V>
+void gs_wg(hipStream_t s, const Level &L, const V *BI, V *d, const V *F, V *Ad, const V *rhs, V *res, int sweeps,
+           int guess, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // level above the coarsest: its smoother call as d = Ma din + Mb F
 bool affine_ok(const Level &L);
 void affine_build(hipStream_t s, const Level &L, int sweeps);

@@ -566,6 +566,20 @@ constexpr long AFFINE_MAX_DOF = 1500;
 
 bool affine_ok(const Level &L) { return 3L * L.g.nno <= AFFINE_MAX_DOF; }
 
+// one-workgroup smoother call (k_gs_wg, below): tiles in flight, threads,
+// and the largest level it takes.  One CU reads the stiffness through L2 at
+// 50-70 GB/s, so at 2601 nodes a call takes 156 us against 111 us as
+// per-colour launches of 4.3 us each; below about 1500 nodes it is faster.
+constexpr int WG_SLOTS = 4;
+constexpr int WG_TPB = WG_SLOTS * 3 * 64;
+constexpr int WG_MAX_NNO = 1024;
+template <typename V>
+__global__ void __launch_bounds__(WG_TPB) k_gs_wg(Geo g, const float *__restrict__ K,
+                                                  const unsigned *__restrict__ NODE, const V *__restrict__ BI, V *d,
+                                                  const V *__restrict__ F, V *__restrict__ Ad,
+                                                  const V *__restrict__ rhs, V *__restrict__ res, int sweeps,
+                                                  int guess);
+
 size_t lds_limit() {
     static size_t lim = 0;
     if (!lim) {
@@ -575,8 +589,14 @@ size_t lds_limit() {
             lim = (size_t)v;
         else
             lim = 65536;
-        if (lim > 65536)
+        if (lim > 65536) {
             (void)hipFuncSetAttribute((const void *)k_gs_lds, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim);
+            const size_t ps = (size_t)WG_SLOTS * 9 * 64;   // k_gs_wg's static partial sums
+            (void)hipFuncSetAttribute((const void *)k_gs_wg<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)(lim - ps * sizeof(double)));
+            (void)hipFuncSetAttribute((const void *)k_gs_wg<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)(lim - ps * sizeof(float)));
+        }
     }
     return lim;
 }
@@ -591,6 +611,119 @@ void gs_small(hipStream_t s, const Level &L, double *d, const double *F, double 
     hipLaunchKernelGGL(k_gs_lds, dim3(1), dim3(SMALL_TPB), kstage_bytes(nno) + vstage_bytes(nno), s, L.g, L.K, L.BI,
                        L.NODE, d, F, Ad, sweeps, guess, 0);
 }
+
+// ---------------------------------------------------------------------------
+// A small level whose stiffness does not fit in LDS (8x8x8 elements, 729
+// nodes, 708 KB of stiffness) runs a whole gauss_seidel call in one
+// workgroup: d in LDS, the stiffness read through L2 (resident in the one
+// XCD's L2 after the first sweep).  Each colour pass is
+// k_gs_colour's, with workgroup barriers in place of kernel boundaries:
+// three wavefronts per 64-node tile form the three y-planes (same trees,
+// same update expression), WG_SLOTS tiles at a time; then Ad = K d in
+// row_product's order, BC rows stripped, and res = rhs - Ad when asked.
+// A colour pass of such a level is latency-bound as a kernel of its own.
+
+template <typename V>
+__global__ void __launch_bounds__(WG_TPB) k_gs_wg(Geo g, const float *__restrict__ K,
+                                                  const unsigned *__restrict__ NODE, const V *__restrict__ BI, V *d,
+                                                  const V *__restrict__ F, V *__restrict__ Ad,
+                                                  const V *__restrict__ rhs, V *__restrict__ res, int sweeps,
+                                                  int guess) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    V *ds = (V *)smem;   // [3 nno]
+    __shared__ V Ps[WG_SLOTS][3][3][64];   // [slot][plane][row][node]
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, slot = w / 3, pl = w - 3 * slot;
+    const int n3 = 3 * g.nno;
+    for (int idx = tid; idx < n3; idx += blockDim.x) ds[idx] = guess ? d[idx] : V(0);
+    __syncthreads();
+    for (int cyc = 0; cyc < sweeps; cyc++)
+        for (int c = 0; c < 8; c++) {
+            const int tiles = (g.cn[c] + 63) >> 6;
+            for (int t0 = 0; t0 < tiles; t0 += WG_SLOTS) {   // uniform over the workgroup
+                const int q = g.cs[c] + (t0 + slot) * 64 + lane;
+                const bool act = t0 + slot < tiles && q < g.cs[c] + g.cn[c];
+                int k = 0, i = 0, j = 0;
+                if (act) colour_kij(g, c, q, k, i, j);
+                const int self = nid(g, k, i, j);
+                V Fr = V(0), Br = V(0);
+                if (act) {
+                    Fr = F[3 * self + pl];
+                    Br = BI[3 * self + pl];
+                    V out[3];
+                    const float *kp = K + ktile(q);
+                    if (pl == 0)
+                        plane_rows<0>(g, kp, k, i, j, self, (const V *)ds, out);
+                    else if (pl == 1)
+                        plane_rows<1>(g, kp, k, i, j, self, (const V *)ds, out);
+                    else
+                        plane_rows<2>(g, kp, k, i, j, self, (const V *)ds, out);
+                    Ps[slot][pl][0][lane] = out[0];
+                    Ps[slot][pl][1][lane] = out[1];
+                    Ps[slot][pl][2][lane] = out[2];
+                }
+                __syncthreads();
+                if (act) {
+                    const int r = pl;
+                    const V o = (Ps[slot][0][r][lane] + Ps[slot][1][r][lane]) + Ps[slot][2][r][lane];
+                    const float t = (float)((Fr - o) * Br);
+                    ds[3 * self + r] = ds[3 * self + r] + t;
+                }
+                __syncthreads();
+            }
+        }
+    for (int q = tid; q < g.pitch; q += blockDim.x) {
+        const int col = colour_valid(g, q);
+        if (col < 0) continue;
+        int k, i, j;
+        colour_kij(g, col, q, k, i, j);
+        const float *kp = K + ktile(q);
+        const V *u = ds;
+        V out[3];
+        row_product_t<V>(
+            g, [kp](int ci) { return kp[ci * 64]; }, k, i, j, [u](int idx) { return u[idx]; }, out);
+        const int n = nid(g, k, i, j);
+        strip3(NODE[n], out);
+        Ad[3 * n] = out[0];
+        Ad[3 * n + 1] = out[1];
+        Ad[3 * n + 2] = out[2];
+        if (res) {
+            res[3 * n] = rhs[3 * n] - out[0];
+            res[3 * n + 1] = rhs[3 * n + 1] - out[1];
+            res[3 * n + 2] = rhs[3 * n + 2] - out[2];
+        }
+    }
+    for (int idx = tid; idx < n3; idx += blockDim.x) d[idx] = ds[idx];
+}
+
+template <typename V>
+size_t wg_stage_bytes(const Level &L) {
+    return (size_t)3 * L.g.nno * sizeof(V);
+}
+
+template <typename V>
+bool gs_wg_ok(const Level &L) {
+    return !L.Kh && L.g.nno <= WG_MAX_NNO &&
+           wg_stage_bytes<V>(L) + sizeof(V) * WG_SLOTS * 9 * 64 <= lds_limit();
+}
+template bool gs_wg_ok<double>(const Level &);
+template bool gs_wg_ok<float>(const Level &);
+
+template <typename V>
+void gs_wg(hipStream_t s, const Level &L, const V *BI, V *d, const V *F, V *Ad, const V *rhs, V *res, int sweeps,
+           int guess, hipEvent_t e0, hipEvent_t e1) {
+    auto kern = k_gs_wg<V>;
+    const float *K = L.K;
+    if (e0)
+        hipExtLaunchKernelGGL(kern, dim3(1), dim3(WG_TPB), wg_stage_bytes<V>(L), s, e0, e1, 0, L.g, K, L.NODE, BI, d,
+                              F, Ad, rhs, res, sweeps, guess);
+    else
+        hipLaunchKernelGGL(kern, dim3(1), dim3(WG_TPB), wg_stage_bytes<V>(L), s, L.g, K, L.NODE, BI, d, F, Ad, rhs,
+                           res, sweeps, guess);
+}
+template void gs_wg<double>(hipStream_t, const Level &, const double *, double *, const double *, double *,
+                            const double *, double *, int, int, hipEvent_t, hipEvent_t);
+template void gs_wg<float>(hipStream_t, const Level &, const float *, float *, const float *, float *, const float *,
+                           float *, int, int, hipEvent_t, hipEvent_t);
 
 // ---------------------------------------------------------------------------
 // Coarsest level as a matrix.  The gauss_seidel call on the coarsest level of

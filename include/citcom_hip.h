@@ -162,7 +162,8 @@ int chip_timestep(chip_ctx *c, chip_pg_stats *pg, chip_stokes_stats *st, float *
 int chip_general_stokes_solver(chip_ctx *c, chip_stokes_stats *st);
 
 /* ---- measurement ---------------------------------------------------------- */
-/* kernel: "gs_colour" or "matvec".  While enabled, every launch of that kernel
+/* kernel: "gs_colour", "matvec" or "gs_wg" (the one-workgroup smoother call
+ * of small levels).  While enabled, every launch of that kernel
  * is bracketed by HIP events on the work stream; read returns the number of
  * launches, their summed duration and the algorithmic bytes they moved. */
 int chip_profile_enable(chip_ctx *c, const char *kernel, int on);
@@ -174,6 +175,9 @@ long chip_device_bytes(chip_ctx *c);
  * "small_levels": 1 (default) smooths levels whose stiffness fits in LDS in
  * one single-workgroup kernel per gauss_seidel call, 0 launches one kernel
  * per colour pass on every level; results are identical.
+ * "gs_wg": 1 (default) also runs a whole gauss_seidel call in one workgroup
+ * on levels of at most 1024 nodes whose stiffness does not fit in LDS;
+ * 0 launches one kernel per colour pass there; results are identical.
  * "profile_level": >= 0 restricts chip_profile_enable to that level's
  * launches, -1 (default) profiles every level.
  * "print_convergence": 1 prints one line per Uzawa iteration and per inner

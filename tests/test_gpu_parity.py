@@ -283,6 +283,32 @@ def test_smoother_kernels_bitwise(name, small, pyoracle):
     c.close()
 
 
+@pytest.mark.parametrize("mixed", [0, 1, 2])
+@pytest.mark.parametrize("name", ["busse1a", "tdep64"])
+def test_one_workgroup_smoother_same_bits(name, mixed):
+    """A whole gauss_seidel call in one workgroup (unknowns in LDS, stiffness
+    read through L2: busse1a's 405-node level, tdep64's 729-node level) gives
+    the per-colour launches' iterate bit for bit, in the FP64,
+    FP32 and FP16-stiffness forms: the step-0 solve and two timesteps agree
+    exactly with the kernel on and off."""
+    from citcom_amd import deck
+    from citcom_amd.hip import Citcom
+    p = problem(name)
+    runs = []
+    for wg in (1, 0):
+        c = Citcom(p, mixed=mixed)
+        c.set_option("gs_wg", wg)
+        c.initialize(deck.seeded_temperature(p, seed=3))
+        st = c.general_stokes_solver()
+        for _ in range(2):
+            r = c.timestep(with_nu=False)
+        runs.append((c.download("U"), c.download("P"), c.download("T"), st["mg_cycles"], r["stokes"]["mg_cycles"]))
+        c.close()
+    say(name, "mixed", mixed, "one-workgroup smoother on/off: MG calls", runs[0][3:], runs[1][3:])
+    for a, b in zip(runs[0], runs[1]):
+        assert np.array_equal(a, b)
+
+
 def test_unsupported_problem_is_refused():
     """Viscosity laws other than rheol 0, 3 and 11, and unknown precision
     forms, are refused at setup, not computed as something else."""
