@@ -177,10 +177,16 @@ void do_gs(chip_ctx *c, int lev, V *d, const V *F, V *Ad, int sweeps, int guess,
 // is a copy of fl[lev] in the reference, and fl[lev] is not read again).
 // This enqueues the whole fixed kernel sequence; |F|^2 lands in dots[D_TMP].
 // the coarsest-level smoother call of multi_grid (AU[levmin] is not read
-// afterwards): the coarse matrix where there is one
+// afterwards): the learned correction when it is switched on (launch errors
+// surface at the multigrid call's own checks), else the coarse matrix where
+// there is one
 template <typename V>
 void coarse_smooth(chip_ctx *c, V *vel, const V *rhs) {
     const Level &L = c->L[c->levmin];
+    if (c->cnet_on) {
+        (void)cnet_apply(S(c), c, L, rhs, vel);
+        return;
+    }
     if (L.Mc && c->small_levels)
         coarse_apply(S(c), L, rhs, vel);
     else
@@ -384,6 +390,7 @@ int rebuild(chip_ctx *c) {
     for (int lev = c->levmin; lev <= c->levmax; lev++) elt_g(q, c->L[lev]);
     for (int lev = c->levmax; lev >= c->levmin; lev--) node_ks(q, c->L[lev], c->p);
     for (int lev = c->levmax; lev >= c->levmin; lev--) build_bpi(q, c->L[lev], c->p.precond);
+    if (c->cnet_w) cnet_logeta(q, c->L[c->levmin], c->cnet_logeta);
     if (c->L[c->levmin].Mc) coarse_build(q, c->L[c->levmin], c->p.v_steps_low);
     for (int lev = c->levmin; lev <= c->levmax; lev++)
         if (c->L[lev].Ma) affine_build(q, c->L[lev], c->L[lev].affine_sweeps);
@@ -726,6 +733,17 @@ int chip_apply(chip_ctx *c, const char *op, int lev, const double *in, const dou
         CHIP_CHECK(up(b, in2, nv));
         coarse_apply(q, L, b, a);
         CHIP_CHECK(down(out, a, nv));
+    } else if (o == "cnet") {
+        if (lev != c->levmin || !c->cnet_layers) {
+            set_error("cnet: the coarsest level, after chip_set_coarse_net");
+            return -1;
+        }
+        CHIP_CHECK(up(b, in2, nv));
+        if (cnet_apply(q, c, L, (const double *)b, a)) {
+            set_error("cnet: launch failed");
+            return -1;
+        }
+        CHIP_CHECK(down(out, a, nv));
     } else if (o == "project") {
         if (lev == c->levmin) {
             set_error("project: no coarser level");
@@ -858,6 +876,32 @@ int chip_profile_read(chip_ctx *c, long *launches, double *total_ms, double *byt
 
 long chip_device_bytes(chip_ctx *c) { return c ? c->dev_bytes : -1; }
 
+int chip_set_coarse_net(chip_ctx *c, int layers, const uint16_t *weights, const float *bias, const float *out_scale) {
+    CHIP_NEED_CTX(c);
+    if (layers < 1 || layers > CNET_MAX_LAYERS || !weights || !bias || !out_scale) {
+        set_error("chip_set_coarse_net: 1.." + std::to_string(CNET_MAX_LAYERS) +
+                  " layers and non-null weights, biases and output scales");
+        return -1;
+    }
+    if (alloc_coarse_net(c)) return -1;
+    hipStream_t q = c->stream;
+    CHIP_CHECK(hipMemcpyAsync(c->cnet_w, weights, (size_t)layers * 16 * 27 * 16 * sizeof(uint16_t),
+                              hipMemcpyHostToDevice, q));
+    CHIP_CHECK(hipMemcpyAsync(c->cnet_b, bias, (size_t)layers * 16 * sizeof(float), hipMemcpyHostToDevice, q));
+    CHIP_CHECK(hipMemcpyAsync(c->cnet_ys, out_scale, 3 * sizeof(float), hipMemcpyHostToDevice, q));
+    // EVI of the coarsest level holds the last rebuild's viscosity (or the
+    // isoviscous start); every later rebuild forms the feature again
+    cnet_logeta(q, c->L[c->levmin], c->cnet_logeta);
+    CHIP_CHECK(hipGetLastError());
+    CHIP_CHECK(hipStreamSynchronize(q));
+    c->cnet_layers = layers;
+    if (c->mg_graph) {   // the layer count is part of the captured sequence
+        CHIP_CHECK(hipGraphExecDestroy(c->mg_graph));
+        c->mg_graph = nullptr;
+    }
+    return 0;
+}
+
 int chip_set_option(chip_ctx *c, const char *name, int value) {
     CHIP_NEED_CTX(c);
     const std::string n(name ? name : "");
@@ -873,8 +917,12 @@ int chip_set_option(chip_ctx *c, const char *name, int value) {
         c->print_convergence = value != 0;
         return 0;
     }
-    if (n == "small_levels" || n == "gs_wg") {
-        (n == "small_levels" ? c->small_levels : c->gs_wg) = value != 0;
+    if (n == "coarse_net" && value && !c->cnet_layers) {
+        set_error("chip_set_option: coarse_net needs chip_set_coarse_net first");
+        return -1;
+    }
+    if (n == "small_levels" || n == "gs_wg" || n == "coarse_net") {
+        (n == "small_levels" ? c->small_levels : n == "gs_wg" ? c->gs_wg : c->cnet_on) = value != 0;
         if (c->mg_graph) {   // the captured kernel sequence changes
             CHIP_CHECK(hipStreamSynchronize(c->stream));
             CHIP_CHECK(hipGraphExecDestroy(c->mg_graph));

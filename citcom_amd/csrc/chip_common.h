@@ -123,6 +123,15 @@ struct chip_ctx {
     bool use_graphs = true;  // replay multi_grid as a captured HIP graph
     bool small_levels = true;  // single-workgroup smoother on small levels
     bool gs_wg = true;         // ... including levels whose stiffness is read through L2
+    // learned coarse-level correction (chip_set_coarse_net): a 3-D CNN on the
+    // coarsest level in place of its smoother call; BF16 weights
+    // [layers][16][27][16], FP32 biases [layers][16], output scales [3],
+    // per-node log10 viscosity, activations [nno][16], the input scale
+    int cnet_layers = 0;
+    bool cnet_on = false;
+    uint16_t *cnet_w = nullptr, *cnet_x0 = nullptr, *cnet_x1 = nullptr;
+    float *cnet_b = nullptr, *cnet_ys = nullptr, *cnet_yf = nullptr, *cnet_logeta = nullptr;
+    double *cnet_s = nullptr;
     bool print_convergence = false;  // per-iteration lines on stderr, as E->control.print_convergence
     hipGraphExec_t mg_graph = nullptr;
     chip::Prof prof;
@@ -133,6 +142,9 @@ struct chip_ctx {
 namespace chip {
 // setup (chip_setup.cpp)
 int build_mesh(chip_ctx *c);
+// device buffers of the learned coarse-level correction (once per context)
+constexpr int CNET_MAX_LAYERS = 8;
+int alloc_coarse_net(chip_ctx *c);
 // errors
 void set_error(const std::string &s);
 }  // namespace chip

@@ -61,6 +61,15 @@ void div(hipStream_t s, const Level &L, const double *U, double *divU);
 void grad(hipStream_t s, const Level &L, const double *P, double *gradP);
 void strip(hipStream_t s, const Level &L, double *v);
 
+// ---- learned coarse-level correction (no reference counterpart) ----------
+// per node of level L: log10 viscosity, the mean over each adjacent
+// element's Gauss points averaged over those elements
+void cnet_logeta(hipStream_t s, const Level &L, float *out);
+// d = the network's coarse correction for right-hand side F on the coarsest
+// level: inputs (F / rms F, log10 eta), output scaled back by rms F, BC rows 0
+template <typename V>
+int cnet_apply(hipStream_t s, const chip_ctx *c, const Level &L, const V *F, V *d);
+
 // ---- reductions: out[m] = sum a_m . b_m over n entries (m < nd <= 4), FP64
 // (only a0, b0 fix the vector type; the others may be nullptr)
 template <typename T>

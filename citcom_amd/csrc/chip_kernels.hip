@@ -15,6 +15,7 @@
 #include <cmath>
 
 #include "chip_kernels.h"
+#include "../../include/citcom_learn.h"
 
 namespace chip {
 
@@ -2131,5 +2132,110 @@ void heat_flux(hipStream_t s, chip_ctx *c, const float *T, const float *V) {
     hipLaunchKernelGGL(k_hf_gather, dim3(nblk(L.g.nno)), dim3(TPB), 0, s, L.g, L.TWW, L.MASS, c->Eres, c->hf);
     hipLaunchKernelGGL(k_hf_surface, dim3(1), dim3(TPB), 0, s, L.g, L.ECO, c->hf, c->dots + 24);
 }
+
+// ---------------------------------------------------------------------------
+// Learned coarse-level correction (no reference counterpart).  The coarsest
+// level's smoother call of multi_grid (General_matrix_functions.c:574, :613)
+// is replaced by a small 3-D CNN run with the BF16 MFMA kernels of
+// learn_kernels.hip: inputs F / rms(F) (three channels) and log10 viscosity,
+// output the correction / rms(F).  The inner solve still iterates to its own
+// residual criterion.
+
+__global__ void __launch_bounds__(TPB) k_cnet_logeta(Geo g, const float *__restrict__ EVI, float *__restrict__ out) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nno) return;
+    const int k = n % g.noz, i = (n / g.noz) % g.nox, j = n / (g.noz * g.nox);
+    float acc = 0.f;
+    int cnt = 0;
+    for (int dj = -1; dj <= 0; dj++)
+        for (int di = -1; di <= 0; di++)
+            for (int dk = -1; dk <= 0; dk++) {
+                const int ez = k + dk, ex = i + di, ey = j + dj;
+                if (ez < 0 || ex < 0 || ey < 0 || ez >= g.elz || ex >= g.elx || ey >= g.ely) continue;
+                const size_t e = (size_t)ez + (size_t)g.elz * (ex + (size_t)g.elx * ey);
+                float m = 0.f;
+                for (int q = 0; q < 8; q++) m += log10f(EVI[e * 8 + q]);
+                acc += 0.125f * m;
+                cnt++;
+            }
+    out[n] = acc / (float)cnt;
+}
+
+void cnet_logeta(hipStream_t s, const Level &L, float *out) {
+    hipLaunchKernelGGL(k_cnet_logeta, dim3(nblk(L.g.nno)), dim3(TPB), 0, s, L.g, (const float *)L.EVI, out);
+}
+
+// round to the nearest bf16, ties to even (finite values)
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+    const unsigned u = __float_as_uint(f);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+// one workgroup: sc[0] = rms(F), X[n][16] = (F[3n..3n+2] / rms, log10 eta, 0...)
+template <typename V>
+__global__ void __launch_bounds__(TPB) k_cnet_pack(int nno, const V *__restrict__ F, const float *__restrict__ logeta,
+                                                   uint16_t *__restrict__ X, double *__restrict__ sc) {
+    __shared__ double sh[16];
+    __shared__ double inv_sh;
+    double v[1] = {0.0};
+    for (int i = threadIdx.x; i < 3 * nno; i += blockDim.x) v[0] += (double)F[i] * (double)F[i];
+    block_sum<1>(v, sh);
+    if (threadIdx.x == 0) {
+        const double rms = sqrt(v[0] / (3.0 * nno));
+        sc[0] = rms;
+        inv_sh = rms > 0.0 ? 1.0 / rms : 0.0;
+    }
+    __syncthreads();
+    const double inv = inv_sh;
+    for (int idx = threadIdx.x; idx < nno * 16; idx += blockDim.x) {
+        const int n = idx >> 4, ch = idx & 15;
+        float x = 0.f;
+        if (ch < 3)
+            x = (float)((double)F[3 * n + ch] * inv);
+        else if (ch == 3)
+            x = logeta[n];
+        X[idx] = bf16_rne(x);
+    }
+}
+
+// d = Y[n][0..2] * ys * rms(F), BC rows zero
+template <typename V>
+__global__ void __launch_bounds__(TPB) k_cnet_unpack(Geo g, const float *__restrict__ Y, const float *__restrict__ ys,
+                                                     const double *__restrict__ sc, const unsigned *__restrict__ NODE,
+                                                     V *__restrict__ d) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nno) return;
+    const double rms = sc[0];
+    V out[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) out[r] = (V)((double)Y[(size_t)n * 16 + r] * (double)ys[r] * rms);
+    strip3(NODE[n], out);
+    d[3 * n] = out[0];
+    d[3 * n + 1] = out[1];
+    d[3 * n + 2] = out[2];
+}
+
+template <typename V>
+int cnet_apply(hipStream_t s, const chip_ctx *c, const Level &L, const V *F, V *d) {
+    auto pack = k_cnet_pack<V>;
+    hipLaunchKernelGGL(pack, dim3(1), dim3(TPB), 0, s, L.g.nno, F, (const float *)c->cnet_logeta, c->cnet_x0,
+                       c->cnet_s);
+    const uint16_t *in = c->cnet_x0;
+    uint16_t *bufs[2] = {c->cnet_x1, c->cnet_x0};
+    for (int li = 0; li < c->cnet_layers; li++) {
+        const bool last = li == c->cnet_layers - 1;
+        uint16_t *out = bufs[li & 1];
+        if (cl_conv3d_fwd(in, 1, L.g.nox, L.g.noy, L.g.noz, c->cnet_w + (size_t)li * 16 * 27 * 16, c->cnet_b + li * 16,
+                          last ? 0 : 1, out, last ? c->cnet_yf : nullptr, (void *)s))
+            return -1;
+        in = out;
+    }
+    auto unpack = k_cnet_unpack<V>;
+    hipLaunchKernelGGL(unpack, dim3(nblk(L.g.nno)), dim3(TPB), 0, s, L.g, (const float *)c->cnet_yf,
+                       (const float *)c->cnet_ys, (const double *)c->cnet_s, L.NODE, d);
+    return 0;
+}
+template int cnet_apply<double>(hipStream_t, const chip_ctx *, const Level &, const double *, double *);
+template int cnet_apply<float>(hipStream_t, const chip_ctx *, const Level &, const float *, float *);
 
 }  // namespace chip

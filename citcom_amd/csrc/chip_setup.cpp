@@ -249,6 +249,17 @@ int dev_upload(chip_ctx *c, T **ptr, const std::vector<T> &v) {
 
 }  // namespace
 
+int alloc_coarse_net(chip_ctx *c) {
+    if (c->cnet_w) return 0;
+    const size_t nno = (size_t)c->L[c->levmin].g.nno;
+    if (dev_alloc(c, &c->cnet_w, (size_t)CNET_MAX_LAYERS * 16 * 27 * 16) ||
+        dev_alloc(c, &c->cnet_b, (size_t)CNET_MAX_LAYERS * 16) || dev_alloc(c, &c->cnet_ys, 3) ||
+        dev_alloc(c, &c->cnet_x0, nno * 16) || dev_alloc(c, &c->cnet_x1, nno * 16) ||
+        dev_alloc(c, &c->cnet_yf, nno * 16) || dev_alloc(c, &c->cnet_logeta, nno) || dev_alloc(c, &c->cnet_s, 1))
+        return -1;
+    return 0;
+}
+
 int build_mesh(chip_ctx *c) {
     const chip_problem &p = c->p;
     Master m;

@@ -110,6 +110,7 @@ def lib():
         L.chip_profile_enable.argtypes = [P, C.c_char_p, C.c_int]
         L.chip_profile_read.argtypes = [P, C.POINTER(C.c_long), C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.chip_set_option.argtypes = [P, C.c_char_p, C.c_int]
+        L.chip_set_coarse_net.argtypes = [P, C.c_int, P, P, P]
         L.chip_device_bytes.restype = C.c_long
         L.chip_device_bytes.argtypes = [P]
         _lib = L
@@ -270,7 +271,7 @@ class Citcom:
         """Single operator on host arrays (see chip_apply in include/citcom_hip.h)."""
         n3, ne = 3 * self.nno(lev), self.nel(lev)
         out_n = {"matvec": n3, "gs0": n3, "gs1": n3, "coarse": n3, "affine0": n3, "affine1": n3, "div": ne, "grad": n3,
-                 "strip": n3,
+                 "strip": n3, "cnet": n3,
                  "project": 3 * self.nno(lev - 1) if lev > 0 else 0, "interp": n3}[op]
         out = np.zeros(out_n)
         out2 = np.zeros(n3)
@@ -297,6 +298,17 @@ class Citcom:
 
     def set_option(self, name: str, value: int) -> None:
         _check(lib().chip_set_option(self.h, name.encode(), int(value)), f"set_option {name}")
+
+    def set_coarse_net(self, weights: np.ndarray, bias: np.ndarray, out_scale: np.ndarray) -> None:
+        """learned coarse-level correction (chip_set_coarse_net): weights as BF16
+        bits [layers][16][27][16], bias [layers][16], out_scale [3]"""
+        w = np.ascontiguousarray(weights, dtype=np.uint16)
+        b = np.ascontiguousarray(bias, dtype=np.float32)
+        ys = np.ascontiguousarray(out_scale, dtype=np.float32)
+        if w.ndim != 4 or w.shape[1:] != (16, 27, 16) or b.shape != (w.shape[0], 16) or ys.shape != (3,):
+            raise ChipError("set_coarse_net: weights [layers][16][27][16], bias [layers][16], out_scale [3]")
+        _check(lib().chip_set_coarse_net(self.h, w.shape[0], w.ctypes.data, b.ctypes.data, ys.ctypes.data),
+               "set_coarse_net")
 
     def device_bytes(self) -> int:
         return lib().chip_device_bytes(self.h)

@@ -1,4 +1,5 @@
-"""Learned warm start for the Uzawa solve (no counterpart in the reference).
+"""Learned warm start for the Uzawa solve and learned coarse-level correction
+for its multigrid (no counterpart in the reference).
 
 A small 3-D convolutional network (4 layers of 3x3x3 convolutions, 16
 channels, ReLU) maps (temperature, log10 viscosity) on a coarse multigrid
@@ -13,6 +14,14 @@ Training data come from the plain solver itself (random temperature fields
 and viscosity contrasts); training runs on the GPU through the BF16 MFMA
 kernels of csrc/learn_kernels.hip (include/citcom_learn.h).  PyTorch holds
 device buffers only.
+
+The coarse-level correction is a second network of the same kind on the
+coarsest multigrid level: (right-hand side / its RMS, log10 viscosity) ->
+the level's exact solve / that RMS, trained on the coarsest operators of
+random temperature fields.  Installed with Citcom.set_coarse_net, it takes
+the place of the coarsest level's smoother call inside every V-cycle
+(General_matrix_functions.c:574, :613); each inner solve still iterates to
+its own residual criterion.
 """
 from __future__ import annotations
 
@@ -59,15 +68,16 @@ def _p(t):
 class ConvNet:
     """Weights (FP32 masters + BF16 copies) and activations on the device."""
 
-    def __init__(self, layers: int = 4, seed: int = 0):
+    def __init__(self, layers: int = 4, seed: int = 0, nin: int = NIN, nout: int = NOUT):
         import torch
         self.torch = torch
         g = torch.Generator().manual_seed(seed)
         self.layers = layers
+        self.nout = nout
         self.w, self.b = [], []
         for li in range(layers):
-            cin = NIN if li == 0 else CH
-            cout = NOUT if li == layers - 1 else CH
+            cin = nin if li == 0 else CH
+            cout = nout if li == layers - 1 else CH
             w = torch.zeros(CH, 27, CH)
             w[:cout, :, :cin] = torch.randn(cout, 27, cin, generator=g) * (2.0 / (27 * cin)) ** 0.5
             self.w.append(w.cuda())
@@ -135,8 +145,8 @@ class ConvNet:
         nblk = (nvox * CH + 255) // 256
         part = torch.empty(nblk, dtype=torch.float32, device=yf.device)
         dy = torch.empty(yf.shape, dtype=torch.bfloat16, device=yf.device)
-        _ok(L.cl_mse(_p(yf), _p(target), nvox, NOUT, _p(dy), _p(part), None), "mse")
-        return float(part.sum()) / (nvox * NOUT), dy
+        _ok(L.cl_mse(_p(yf), _p(target), nvox, self.nout, _p(dy), _p(part), None), "mse")
+        return float(part.sum()) / (nvox * self.nout), dy
 
 
 # ---------------------------------------------------------------------------
@@ -187,9 +197,13 @@ class Sample:
     contrast: float
 
 
-def problem_with_contrast(base: str, contrast: float) -> _deck.Problem:
+def problem_with_contrast(base: str, contrast: float, levels: int | None = None) -> _deck.Problem:
+    """the workload with viscosity contrast exp(viscE) = contrast, optionally
+    with fewer levels (the same coarsest grid under a smaller finest one)"""
     keys = _deck.workload(base)
     keys["viscE"] = repr(float(np.log(contrast)))
+    if levels is not None:
+        keys["levels"] = levels
     return _deck.problem_from_deck({k: str(v) for k, v in keys.items()})
 
 
@@ -238,14 +252,14 @@ def make_samples(base: str, n: int, seed: int, log=None) -> list[Sample]:
 
 
 def train(samples: list[Sample], dims, epochs: int = 400, batch: int = 16, lr: float = 2e-3, seed: int = 0,
-          log=None) -> tuple[ConvNet, list[float]]:
+          log=None, nin: int = NIN, nout: int = NOUT) -> tuple[ConvNet, list[float]]:
     """Adam on the mean squared error of the targets scaled per channel to
     unit RMS (the scales are kept with the network)."""
     import torch
-    net = ConvNet(seed=seed)
+    net = ConvNet(seed=seed, nin=nin, nout=nout)
     ys = np.stack([s.y for s in samples])
     net.yscale = np.ones(CH, np.float32)
-    net.yscale[:NOUT] = np.sqrt((ys[..., :NOUT].astype(np.float64) ** 2).mean(axis=(0, 1))) + 1e-30
+    net.yscale[:nout] = np.sqrt((ys[..., :nout].astype(np.float64) ** 2).mean(axis=(0, 1))) + 1e-30
     X = torch.tensor(np.stack([s.x for s in samples])).to(torch.bfloat16).cuda()
     Y = torch.tensor(ys / net.yscale).cuda()
     n = len(samples)
@@ -295,3 +309,130 @@ def warm_start(net: ConvNet, c: Citcom, p: _deck.Problem, T: np.ndarray) -> None
         u[d::3][(node & bit) != 0] = 0.0
     c.upload("U", u)
     c.upload("P", pel.reshape(-1))
+
+
+# ---------------------------------------------------------------------------
+# learned coarse-level correction
+
+
+def coarse_logeta(c: Citcom, p: _deck.Problem) -> np.ndarray:
+    """log10 viscosity per coarsest-level node: the mean over each adjacent
+    element's Gauss points, averaged over those elements (k_cnet_logeta)"""
+    evi = c.download("EVI", 0).reshape(-1, 8).astype(np.float64)
+    return _nodal_from_elements(np.log10(evi).mean(axis=1), p.dims(0))
+
+
+def coarse_free(c: Citcom) -> np.ndarray:
+    """dofs of the coarsest level without a velocity boundary condition"""
+    node = c.download("NODE", 0)
+    free = np.ones(3 * len(node), bool)
+    for d, bit in enumerate((0x2, 0x8, 0x4)):   # VBX, VBY, VBZ
+        free[d::3][(node & bit) != 0] = False
+    return free
+
+
+def coarse_operator(c: Citcom) -> tuple[np.ndarray, np.ndarray]:
+    """the coarsest level's stiffness on its free dofs, column by column
+    from the device matvec"""
+    free = coarse_free(c)
+    idx = np.flatnonzero(free)
+    K = np.zeros((len(idx), len(idx)))
+    e = np.zeros(len(free))
+    for col, j in enumerate(idx):
+        e[j] = 1.0
+        K[:, col] = c.apply("matvec", 0, e)[idx]
+        e[j] = 0.0
+    return K, free
+
+
+def random_coarse_rhs(dims, rng: np.random.Generator) -> np.ndarray:
+    """white or smoothed (as restricted residuals are) random right-hand side
+    in node order, three components per node"""
+    nox, noy, noz = dims
+    F = rng.standard_normal((noy, nox, noz, 3))
+    if rng.random() < 0.5:
+        from scipy.ndimage import gaussian_filter
+        F = gaussian_filter(F, sigma=(1.0, 1.0, 1.0, 0.0), mode="nearest")
+    return F.reshape(-1)
+
+
+def coarse_features(F: np.ndarray, logeta: np.ndarray) -> tuple[np.ndarray, float]:
+    """network input for right-hand side F: (F / rms F, log10 eta); and rms F"""
+    s = float(np.sqrt(np.mean(F * F)))
+    x = np.zeros((len(logeta), CH), np.float32)
+    x[:, :3] = (F / (s if s > 0 else 1.0)).reshape(-1, 3)
+    x[:, 3] = logeta
+    return x, s
+
+
+@dataclasses.dataclass
+class CoarseSample:
+    x: np.ndarray    # [nv][16] inputs
+    y: np.ndarray    # [nv][16] exact coarse solve / rms F
+    F: np.ndarray    # right-hand side (stripped)
+    d: np.ndarray    # exact coarse solve
+    mc: np.ndarray   # the multigrid's coarsest-level smoother call on F
+
+
+def make_coarse_samples(base: str, levels: int, n_fields: int, n_rhs: int, seed: int,
+                        log=None) -> list[CoarseSample]:
+    """coarsest-level problems of the workload's family: random temperature
+    fields at the workload's viscosity law, n_rhs random right-hand sides
+    each, solved exactly on the free dofs"""
+    rng = np.random.default_rng(seed)
+    contrast = float(np.exp(float(_deck.workload(base)["viscE"])))
+    out = []
+    for k in range(n_fields):
+        p = problem_with_contrast(base, contrast, levels)
+        T = random_temperature(p, rng)
+        c = Citcom(p)
+        c.initialize(T)
+        c.get_system_viscosity()
+        c.construct_stiffness_B_matrix()
+        leta = coarse_logeta(c, p)
+        K, free = coarse_operator(c)
+        for _ in range(n_rhs):
+            F = random_coarse_rhs(p.dims(0), rng) * free
+            d = np.zeros_like(F)
+            d[free] = np.linalg.solve(K, F[free])
+            x, s = coarse_features(F, leta)
+            y = np.zeros_like(x)
+            y[:, :3] = (d / s).reshape(-1, 3)
+            out.append(CoarseSample(x, y, F, d, c.apply("coarse", 0, None, F)))
+        c.close()
+        if log and (k + 1) % 8 == 0:
+            log(f"coarse fields {k + 1}/{n_fields}")
+    return out
+
+
+def train_coarse(samples: list[CoarseSample], dims, epochs: int = 300, seed: int = 0,
+                 log=None) -> tuple[ConvNet, list[float]]:
+    return train(samples, dims, epochs=epochs, seed=seed, log=log, nin=4, nout=3)
+
+
+def coarse_predict(net: ConvNet, x: np.ndarray, s: float, dims) -> np.ndarray:
+    """host-side restatement of the device correction for one input (tests):
+    the same network kernels, scaled back by rms F (BC rows not stripped)"""
+    return (predict(net, x, dims)[:, :3].astype(np.float64) * s).reshape(-1)
+
+
+def coarse_relative_errors(net: ConvNet, samples: list[CoarseSample], dims) -> tuple[float, float]:
+    """mean relative L2 error against the exact coarse solve: the network's,
+    and the coarsest-level smoother call's"""
+    en, es = [], []
+    for smp in samples:
+        s = float(np.sqrt(np.mean(smp.F * smp.F)))
+        dn = coarse_predict(net, smp.x, s, dims) * (smp.F != 0)
+        nd = np.linalg.norm(smp.d)
+        en.append(np.linalg.norm(dn - smp.d) / nd)
+        es.append(np.linalg.norm(smp.mc - smp.d) / nd)
+    return float(np.mean(en)), float(np.mean(es))
+
+
+def install_coarse_net(net: ConvNet, c: Citcom, on: bool = True) -> None:
+    """hand the network to the context and switch the correction on"""
+    import torch
+    w = np.stack([wb.view(torch.int16).cpu().numpy().view(np.uint16) for wb in net.wbf])
+    b = np.stack([bb.cpu().numpy() for bb in net.b])
+    c.set_coarse_net(w, b, net.yscale[:3])
+    c.set_option("coarse_net", int(on))

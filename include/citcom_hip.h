@@ -16,6 +16,8 @@
 #ifndef CITCOM_HIP_H
 #define CITCOM_HIP_H
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -120,6 +122,8 @@ int chip_construct_stiffness_B_matrix(chip_ctx *c);
  *     "affine0"/"affine1" (the level above) its multigrid smoother call
  *         (down_heavy sweeps) as an affine map: out = Mb in2 (zero guess) or
  *         Ma in + Mb in2
+ *     "cnet" (coarsest level) the learned coarse-level correction for the
+ *         right-hand side in2 (after chip_set_coarse_net)
  *     "project" project_vector lev -> lev-1 (Solver_multigrid.c:72-159)
  *     "interp"  interp_vector lev-1 -> lev (Solver_multigrid.c:173-298)
  *     "div" assemble_div_u (Element_calculations.c:691-720)
@@ -180,10 +184,25 @@ long chip_device_bytes(chip_ctx *c);
  * 0 launches one kernel per colour pass there; results are identical.
  * "profile_level": >= 0 restricts chip_profile_enable to that level's
  * launches, -1 (default) profiles every level.
+ * "coarse_net": 1 replaces the coarsest level's smoother call in multigrid
+ * by the learned correction (chip_set_coarse_net), 0 (default) does not.
  * "print_convergence": 1 prints one line per Uzawa iteration and per inner
  * solve on stderr (generate_log_message, Stokes_flow_Incomp.c:501-526;
  * General_matrix_functions.c:494-498). */
 int chip_set_option(chip_ctx *c, const char *name, int value);
+
+/* ---- learned coarse-level correction (no counterpart in the reference) ---- */
+/* A 3-D CNN on the coarsest multigrid level (BF16 MFMA convolutions, FP32
+ * accumulation; the kernels of citcom_learn.h) that takes the place of the
+ * coarsest level's smoother call in multi_grid (General_matrix_functions.c
+ * :574, :613) once the option "coarse_net" is 1.  Node inputs: the
+ * right-hand side over its RMS (3 channels), log10 viscosity (1 channel);
+ * output channels 0..2 times out_scale[0..2] times that RMS, boundary rows
+ * zero.  weights: BF16 bits [layers][16 out][27 taps][16 in], bias
+ * [layers][16]; layers 1..8, ReLU after all but the last.  Copied to the
+ * device; the inner solve keeps iterating to its own residual criterion. */
+int chip_set_coarse_net(chip_ctx *c, int layers, const uint16_t *weights, const float *bias,
+                        const float *out_scale);
 
 #ifdef __cplusplus
 }

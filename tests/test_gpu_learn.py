@@ -177,3 +177,68 @@ def test_warm_start_gain():
     say(f"Uzawa iterations on {len(test_set)} held-out problems: cold start {np.mean(cold_its):.1f} "
         f"({cold_its}), learned warm start {np.mean(warm_its):.1f} ({warm_its})")
     assert np.mean(warm_its) < np.mean(cold_its)
+
+
+def test_coarse_correction_device_matches_host():
+    """The coarse correction as the solver runs it (input packing, the four
+    MFMA convolutions, scaling back, boundary rows) against the same network
+    driven from the host on the same right-hand side."""
+    from citcom_amd import deck, learn
+    from citcom_amd.hip import Citcom
+    from conftest import problem
+    p = problem("tiny_tdep")
+    c = Citcom(p)
+    c.initialize(deck.perturbed_temperature(p))
+    c.get_system_viscosity()
+    c.construct_stiffness_B_matrix()
+    net = learn.ConvNet(seed=3, nin=4, nout=3)
+    net.yscale = np.ones(16, np.float32)
+    net.yscale[:3] = (0.5, 2.0, 1.5)
+    learn.install_coarse_net(net, c, on=False)
+    free = learn.coarse_free(c)
+    F = np.random.default_rng(5).standard_normal(len(free)) * free
+    dev = c.apply("cnet", 0, None, F)
+    x, s = learn.coarse_features(F, learn.coarse_logeta(c, p))
+    host = learn.coarse_predict(net, x, s, p.dims(0)) * free
+    c.close()
+    say(f"coarse correction, device vs host network: rel L2 {rel_l2(dev, host):.2e}")
+    assert np.all(dev[~free] == 0.0)
+    # the log10-viscosity feature is formed in FP32 on the device and FP64 on
+    # the host, so its BF16 rounding can differ
+    assert rel_l2(dev, host) < 2e-2
+
+
+def test_coarse_correction_in_multigrid():
+    """Train the coarse-level correction on tiny_tdep's family (random
+    temperature fields at its viscosity law; target: the coarsest level's
+    exact solve), then use it in every V-cycle of the Stokes solve at
+    accuracy 1e-9: each inner solve still iterates to its own criterion, so
+    the velocity and pressure equal the plain solve's to 1e-6."""
+    from citcom_amd import deck, learn
+    from citcom_amd.hip import Citcom
+    from conftest import problem
+    t0 = time.time()
+    train_set = learn.make_coarse_samples("tiny_tdep", 3, 24, 16, seed=1)
+    test_set = learn.make_coarse_samples("tiny_tdep", 3, 4, 8, seed=2)
+    p = problem("tiny_tdep@1e-9")
+    dims = p.dims(0)
+    net, losses = learn.train_coarse(train_set, dims, epochs=300)
+    en, es = learn.coarse_relative_errors(net, test_set, dims)
+    say(f"coarse correction: {len(train_set)} samples, loss {losses[0]:.3e} -> {losses[-1]:.3e}, "
+        f"{time.time() - t0:.1f} s; held-out error against the exact coarse solve {en:.3f} "
+        f"(the {p.v_steps_low}-sweep smoother: {es:.3f})")
+    assert losses[-1] < 0.2 * losses[0]
+    runs = {}
+    for on in (0, 1):
+        c = Citcom(p)
+        c.initialize(deck.perturbed_temperature(p))
+        if on:
+            learn.install_coarse_net(net, c)
+        st = c.general_stokes_solver()
+        runs[on] = (c.download("U"), c.download("P"), st)
+        c.close()
+    (U0, P0, s0), (U1, P1, s1) = runs[0], runs[1]
+    eu, ep = rel_l2(U1, U0), rel_l2(P1 - P1.mean(), P0 - P0.mean())
+    say(f"tiny_tdep @1e-9: Uzawa {s0['iterations']} / {s1['iterations']} its, multigrid calls "
+        f"{s0['mg_cycles']} (coarse smoother) / {s1['mg_cycles']} (learned correction); U {eu:.2e}, P {ep:.2e}")
+    assert eu < 1e-6 and ep < 1e-6
