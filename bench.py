@@ -236,20 +236,24 @@ def cpu_baseline(name, seed, mg_calls_per_step):
     return cpu_baseline_port(name, seed, mg_calls_per_step)
 
 
-def run_form(p, T, mixed, steps, warmup, graphs, profile):
+def run_form(p, T, mixed, steps, warmup, graphs, profile, coarse_net=None):
     """Timestep loop of one form: step-0 solve and warmup untimed, then `steps`
     timed steps; with profile, one more step with kernel-stamped events on
-    every colour pass and one on every matvec."""
+    every colour pass and one on every matvec.  coarse_net: a trained
+    learned coarse-level correction to use in every V-cycle."""
     import torch
 
     from citcom_amd.hip import Citcom
 
-    form = {0: "plain", 1: "mixed", 2: "mixed16"}[int(mixed)]
+    form = {0: "plain", 1: "mixed", 2: "mixed16"}[int(mixed)] + ("+learned" if coarse_net is not None else "")
     t0 = time.perf_counter()
     c = Citcom(p, mixed=mixed)
     if not graphs:
         c.set_option("graphs", 0)
     c.initialize(T)
+    if coarse_net is not None:
+        from citcom_amd import learn
+        learn.install_coarse_net(coarse_net, c)
     log(f"[{form}] setup {time.perf_counter() - t0:.1f} s, device memory {c.device_bytes() / 2**30:.2f} GiB")
     t0 = time.perf_counter()
     st0 = c.general_stokes_solver()
@@ -287,6 +291,26 @@ def run_form(p, T, mixed, steps, warmup, graphs, profile):
     return out
 
 
+def learned_form(p, T, args):
+    """FP32 cycles with the learned coarse-level correction: train it on the
+    workload family's coarsest-level problems (random temperature fields
+    under a three-level mesh on the same coarse grid, exact coarse solves),
+    then time the mixed form with it in every V-cycle"""
+    from citcom_amd import learn
+    t0 = time.perf_counter()
+    levels = min(3, p.levels)
+    train_set = learn.make_coarse_samples(args.config, levels, 48, 16, seed=args.seed)
+    test_set = learn.make_coarse_samples(args.config, levels, 6, 8, seed=args.seed + 1)
+    net, losses = learn.train_coarse(train_set, p.dims(0), epochs=400)
+    err_net, err_smoother = learn.coarse_relative_errors(net, test_set, p.dims(0))
+    t_train = time.perf_counter() - t0
+    log(f"[mixed+learned] coarse correction trained on {len(train_set)} coarse problems in {t_train:.1f} s: "
+        f"held-out error {err_net:.3f} (coarse smoother {err_smoother:.3f})")
+    out = run_form(p, T, 1, args.compare_steps, 1, not args.no_graphs, profile=False, coarse_net=net)
+    out.update(train_s=t_train, coarse_error=err_net, smoother_error=err_smoother, samples=len(train_set))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -319,11 +343,12 @@ def main():
     T = deck.seeded_temperature(p, seed=args.seed) if p.tdepv else deck.perturbed_temperature(p)
     mixed = {"plain": 0, "mixed": 1, "mixed16": 2}[args.form]
     main_run = run_form(p, T, mixed, args.steps, args.warmup, not args.no_graphs, profile=True)
-    plain = fp16 = None
+    plain = fp16 = learned = None
     if mixed and args.compare_steps > 0:
         plain = run_form(p, T, 0, args.compare_steps, 1, not args.no_graphs, profile=False)
         if mixed == 1:
             fp16 = run_form(p, T, 2, args.compare_steps, 1, not args.no_graphs, profile=False)
+            learned = learned_form(p, T, args)
 
     # known-size copy (256 MiB read + 256 MiB written): the reference point that
     # scales hardware byte counters (FETCH_SIZE / WRITE_SIZE) into HBM bytes
@@ -389,6 +414,17 @@ def main():
                                   "stored in FP16 (FP32 arithmetic; the FP64 residual keeps the reference's FP32 "
                                   "stiffness); same Uzawa iterations and multigrid calls")
         log(f"mixed16 form: {fp16['s_per_step']:.3f} s/step")
+    if learned is not None:
+        config["mixed_learned_s_per_step"] = learned["s_per_step"]
+        config["mixed_learned_speedup_vs_plain_fp64"] = plain["s_per_step"] / learned["s_per_step"]
+        config["mixed_learned_mg_calls_per_step"] = learned["mg_calls_per_step"]
+        config["learned_coarse_correction"] = (
+            f"FP32 cycles with a BF16-MFMA 3-D CNN in place of the coarsest level's smoother call in every "
+            f"V-cycle, trained in {learned['train_s']:.0f} s (inside this run, untimed) on "
+            f"{learned['samples']} exact coarse solves of the workload family; held-out error "
+            f"{learned['coarse_error']:.3f} against {learned['smoother_error']:.3f} for the coarse smoother; "
+            f"{learned['s_per_step'] / s_per_step:.3f}x the mixed form's time per step")
+        log(f"mixed+learned form: {learned['s_per_step']:.3f} s/step, {learned['mg_calls_per_step']:.0f} MG calls")
     finest = {"achieved": ach_f, "frac": ach_f / PEAK_HBM_GBS, "launches": n_f,
               "avg_launch_ms": ms_f / max(n_f, 1), "algorithmic_bytes_per_launch": bytes_f / max(n_f, 1),
               "traffic": measured_traffic(kernel, args.config, "finest_level_hbm_bytes_per_launch")}
