@@ -177,14 +177,14 @@ void do_gs(chip_ctx *c, int lev, V *d, const V *F, V *Ad, int sweeps, int guess,
 // is a copy of fl[lev] in the reference, and fl[lev] is not read again).
 // This enqueues the whole fixed kernel sequence; |F|^2 lands in dots[D_TMP].
 // the coarsest-level smoother call of multi_grid (AU[levmin] is not read
-// afterwards): the learned correction when it is switched on (launch errors
-// surface at the multigrid call's own checks), else the coarse matrix where
-// there is one
+// afterwards): the learned correction when it is switched on (a failed
+// launch is reported by multi_grid), else the coarse matrix where there is
+// one
 template <typename V>
 void coarse_smooth(chip_ctx *c, V *vel, const V *rhs) {
     const Level &L = c->L[c->levmin];
     if (c->cnet_on) {
-        (void)cnet_apply(S(c), c, L, rhs, vel);
+        if (cnet_apply(S(c), c, L, rhs, vel)) c->cnet_failed = true;
         return;
     }
     if (L.Mc && c->small_levels)
@@ -261,12 +261,23 @@ int multi_grid(chip_ctx *c, double *F, double *residual) {
             CHIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             enqueue_multi_grid(c, F);
             CHIP_CHECK(hipStreamEndCapture(c->stream, &graph));
+            if (c->cnet_failed) {
+                (void)hipGraphDestroy(graph);
+                c->cnet_failed = false;
+                set_error("multi_grid: the learned coarse correction failed to launch");
+                return -1;
+            }
             CHIP_CHECK(hipGraphInstantiate(&c->mg_graph, graph, nullptr, nullptr, 0));
             CHIP_CHECK(hipGraphDestroy(graph));
         }
         CHIP_CHECK(hipGraphLaunch(c->mg_graph, c->stream));
     } else {
         enqueue_multi_grid(c, F);
+        if (c->cnet_failed) {
+            c->cnet_failed = false;
+            set_error("multi_grid: the learned coarse correction failed to launch");
+            return -1;
+        }
     }
     if (sync_dots(c)) return -1;
     *residual = std::sqrt(c->h_dots[D_TMP] / (3.0 * c->L[c->levmax].g.nno));

@@ -129,6 +129,7 @@ struct chip_ctx {
     // per-node log10 viscosity, activations [nno][16], the input scale
     int cnet_layers = 0;
     bool cnet_on = false;
+    bool cnet_failed = false;   // a launch of the correction failed while enqueueing
     uint16_t *cnet_w = nullptr, *cnet_x0 = nullptr, *cnet_x1 = nullptr;
     float *cnet_b = nullptr, *cnet_ys = nullptr, *cnet_yf = nullptr, *cnet_logeta = nullptr;
     double *cnet_s = nullptr;
