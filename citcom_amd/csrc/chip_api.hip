@@ -894,6 +894,10 @@ int chip_set_coarse_net(chip_ctx *c, int layers, const uint16_t *weights, const 
                   " layers and non-null weights, biases and output scales");
         return -1;
     }
+    if (!cnet_fits(c->L[c->levmin].g)) {
+        set_error("chip_set_coarse_net: the coarsest level has more than 1024 nodes");
+        return -1;
+    }
     if (alloc_coarse_net(c)) return -1;
     hipStream_t q = c->stream;
     CHIP_CHECK(hipMemcpyAsync(c->cnet_w, weights, (size_t)layers * 16 * 27 * 16 * sizeof(uint16_t),

@@ -126,13 +126,12 @@ struct chip_ctx {
     // learned coarse-level correction (chip_set_coarse_net): a 3-D CNN on the
     // coarsest level in place of its smoother call; BF16 weights
     // [layers][16][27][16], FP32 biases [layers][16], output scales [3],
-    // per-node log10 viscosity, activations [nno][16], the input scale
+    // per-node log10 viscosity
     int cnet_layers = 0;
     bool cnet_on = false;
     bool cnet_failed = false;   // a launch of the correction failed while enqueueing
-    uint16_t *cnet_w = nullptr, *cnet_x0 = nullptr, *cnet_x1 = nullptr;
-    float *cnet_b = nullptr, *cnet_ys = nullptr, *cnet_yf = nullptr, *cnet_logeta = nullptr;
-    double *cnet_s = nullptr;
+    uint16_t *cnet_w = nullptr;
+    float *cnet_b = nullptr, *cnet_ys = nullptr, *cnet_logeta = nullptr;
     bool print_convergence = false;  // per-iteration lines on stderr, as E->control.print_convergence
     hipGraphExec_t mg_graph = nullptr;
     chip::Prof prof;

@@ -66,7 +66,9 @@ void strip(hipStream_t s, const Level &L, double *v);
 // element's Gauss points averaged over those elements
 void cnet_logeta(hipStream_t s, const Level &L, float *out);
 // d = the network's coarse correction for right-hand side F on the coarsest
-// level: inputs (F / rms F, log10 eta), output scaled back by rms F, BC rows 0
+// level: inputs (F / rms F, log10 eta), output scaled back by rms F, BC rows
+// 0; one workgroup, for levels cnet_fits accepts (at most 1024 nodes)
+bool cnet_fits(const Geo &g);
 template <typename V>
 int cnet_apply(hipStream_t s, const chip_ctx *c, const Level &L, const V *F, V *d);
 

@@ -15,7 +15,6 @@
 #include <cmath>
 
 #include "chip_kernels.h"
-#include "../../include/citcom_learn.h"
 
 namespace chip {
 
@@ -2136,10 +2135,17 @@ void heat_flux(hipStream_t s, chip_ctx *c, const float *T, const float *V) {
 // ---------------------------------------------------------------------------
 // Learned coarse-level correction (no reference counterpart).  The coarsest
 // level's smoother call of multi_grid (General_matrix_functions.c:574, :613)
-// is replaced by a small 3-D CNN run with the BF16 MFMA kernels of
-// learn_kernels.hip: inputs F / rms(F) (three channels) and log10 viscosity,
-// output the correction / rms(F).  The inner solve still iterates to its own
-// residual criterion.
+// is replaced by a small 3-D CNN (trained by the BF16 MFMA kernels of
+// learn_kernels.hip): inputs F / rms(F) (three channels) and log10
+// viscosity, output the correction / rms(F).  The inner solve still
+// iterates to its own residual criterion.
+
+// one workgroup of CNET_TPB threads runs the whole correction; its two
+// activation buffers ([nv][16] bf16 each) take 64 nv bytes of LDS
+constexpr int CNET_TPB = 512;
+constexpr int CNET_MAX_NV = 1024;
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
 __global__ void __launch_bounds__(TPB) k_cnet_logeta(Geo g, const float *__restrict__ EVI, float *__restrict__ out) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2171,69 +2177,102 @@ __device__ __forceinline__ uint16_t bf16_rne(float f) {
     return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
 }
 
-// one workgroup: sc[0] = rms(F), X[n][16] = (F[3n..3n+2] / rms, log10 eta, 0...)
+
+__device__ __forceinline__ bf16x4_t cnet_load4(const uint16_t *p) {
+    const uint2 r = *reinterpret_cast<const uint2 *>(p);
+    bf16x4_t a;
+    a[0] = (short)(r.x & 0xFFFF);
+    a[1] = (short)(r.x >> 16);
+    a[2] = (short)(r.y & 0xFFFF);
+    a[3] = (short)(r.y >> 16);
+    return a;
+}
+
+// The whole correction in one workgroup: rms(F) and the input (F / rms,
+// log10 eta) into LDS, the layers with activations in LDS (the products,
+// operand layout and rounding of k_conv_fwd in learn_kernels.hip: one
+// 16x16x16 bf16 MFMA per tap, FP32 accumulation, bias, ReLU, bf16), the
+// last layer's three output channels times out_scale times rms into d,
+// BC rows zero.
 template <typename V>
-__global__ void __launch_bounds__(TPB) k_cnet_pack(int nno, const V *__restrict__ F, const float *__restrict__ logeta,
-                                                   uint16_t *__restrict__ X, double *__restrict__ sc) {
+__global__ void __launch_bounds__(CNET_TPB) k_cnet(Geo g, const V *__restrict__ F, const float *__restrict__ logeta,
+                                                   const uint16_t *__restrict__ W, const float *__restrict__ bias,
+                                                   int layers, const float *__restrict__ ys,
+                                                   const unsigned *__restrict__ NODE, V *__restrict__ d) {
+    extern __shared__ __align__(16) uint16_t act[];   // [2][nv][16]
     __shared__ double sh[16];
-    __shared__ double inv_sh;
+    __shared__ double sc_sh[2];
+    const int nv = g.nno, l = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
     double v[1] = {0.0};
-    for (int i = threadIdx.x; i < 3 * nno; i += blockDim.x) v[0] += (double)F[i] * (double)F[i];
+    for (int i = threadIdx.x; i < 3 * nv; i += blockDim.x) v[0] += (double)F[i] * (double)F[i];
     block_sum<1>(v, sh);
     if (threadIdx.x == 0) {
-        const double rms = sqrt(v[0] / (3.0 * nno));
-        sc[0] = rms;
-        inv_sh = rms > 0.0 ? 1.0 / rms : 0.0;
+        const double rms = sqrt(v[0] / (3.0 * nv));
+        sc_sh[0] = rms;
+        sc_sh[1] = rms > 0.0 ? 1.0 / rms : 0.0;
     }
     __syncthreads();
-    const double inv = inv_sh;
-    for (int idx = threadIdx.x; idx < nno * 16; idx += blockDim.x) {
+    const double rms = sc_sh[0], inv = sc_sh[1];
+    for (int idx = threadIdx.x; idx < nv * 16; idx += blockDim.x) {
         const int n = idx >> 4, ch = idx & 15;
         float x = 0.f;
         if (ch < 3)
             x = (float)((double)F[3 * n + ch] * inv);
         else if (ch == 3)
             x = logeta[n];
-        X[idx] = bf16_rne(x);
+        act[idx] = bf16_rne(x);
+    }
+    __syncthreads();
+    const int o = l & 15, kq = 4 * (l >> 4);
+    for (int li = 0; li < layers; li++) {
+        const uint16_t *X = act + (size_t)(li & 1) * nv * 16;
+        uint16_t *Y = act + (size_t)((li + 1) & 1) * nv * 16;
+        const uint16_t *Wl = W + (size_t)li * 16 * 27 * 16;
+        const float bo = bias[li * 16 + o];
+        const bool last = li == layers - 1;
+        for (int v0 = wv * 16; v0 < nv; v0 += nw * 16) {   // uniform per wavefront
+            const int m = v0 + (l & 15);
+            const bool in = m < nv;
+            const int k = m % g.noz, i = (m / g.noz) % g.nox, j = m / (g.noz * g.nox);
+            f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int t = 0; t < 27; t++) {   // tap t = (dz+1) + 3(dx+1) + 9(dy+1)
+                const int dz = t % 3 - 1, dx = (t / 3) % 3 - 1, dy = t / 9 - 1;
+                const bool ok = in && (unsigned)(k + dz) < (unsigned)g.noz && (unsigned)(i + dx) < (unsigned)g.nox &&
+                                (unsigned)(j + dy) < (unsigned)g.noy;
+                bf16x4_t a = {0, 0, 0, 0};
+                if (ok) a = cnet_load4(X + (size_t)(m + dz + g.noz * (dx + g.nox * dy)) * 16 + kq);
+                const bf16x4_t w = cnet_load4(Wl + ((size_t)o * 27 + t) * 16 + kq);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, w, acc, 0, 0, 0);
+            }
+            for (int i = 0; i < 4; i++) {
+                const int vv = v0 + kq + i;
+                if (vv >= nv) continue;
+                float y = acc[i] + bo;
+                if (!last) {
+                    Y[(size_t)vv * 16 + o] = bf16_rne(y > 0.f ? y : 0.f);
+                } else if (o < 3) {
+                    V out = (V)((double)y * (double)ys[o] * rms);
+                    const unsigned f = NODE[vv];
+                    if ((o == 0 && (f & VBX)) || (o == 1 && (f & VBY)) || (o == 2 && (f & VBZ))) out = V(0);
+                    d[3 * vv + o] = out;
+                }
+            }
+        }
+        __syncthreads();
     }
 }
 
-// d = Y[n][0..2] * ys * rms(F), BC rows zero
-template <typename V>
-__global__ void __launch_bounds__(TPB) k_cnet_unpack(Geo g, const float *__restrict__ Y, const float *__restrict__ ys,
-                                                     const double *__restrict__ sc, const unsigned *__restrict__ NODE,
-                                                     V *__restrict__ d) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= g.nno) return;
-    const double rms = sc[0];
-    V out[3];
-#pragma unroll
-    for (int r = 0; r < 3; r++) out[r] = (V)((double)Y[(size_t)n * 16 + r] * (double)ys[r] * rms);
-    strip3(NODE[n], out);
-    d[3 * n] = out[0];
-    d[3 * n + 1] = out[1];
-    d[3 * n + 2] = out[2];
-}
+bool cnet_fits(const Geo &g) { return g.nno <= CNET_MAX_NV; }
 
 template <typename V>
 int cnet_apply(hipStream_t s, const chip_ctx *c, const Level &L, const V *F, V *d) {
-    auto pack = k_cnet_pack<V>;
-    hipLaunchKernelGGL(pack, dim3(1), dim3(TPB), 0, s, L.g.nno, F, (const float *)c->cnet_logeta, c->cnet_x0,
-                       c->cnet_s);
-    const uint16_t *in = c->cnet_x0;
-    uint16_t *bufs[2] = {c->cnet_x1, c->cnet_x0};
-    for (int li = 0; li < c->cnet_layers; li++) {
-        const bool last = li == c->cnet_layers - 1;
-        uint16_t *out = bufs[li & 1];
-        if (cl_conv3d_fwd(in, 1, L.g.nox, L.g.noy, L.g.noz, c->cnet_w + (size_t)li * 16 * 27 * 16, c->cnet_b + li * 16,
-                          last ? 0 : 1, out, last ? c->cnet_yf : nullptr, (void *)s))
-            return -1;
-        in = out;
-    }
-    auto unpack = k_cnet_unpack<V>;
-    hipLaunchKernelGGL(unpack, dim3(nblk(L.g.nno)), dim3(TPB), 0, s, L.g, (const float *)c->cnet_yf,
-                       (const float *)c->cnet_ys, (const double *)c->cnet_s, L.NODE, d);
-    return 0;
+    if (!cnet_fits(L.g)) return -1;
+    auto kern = k_cnet<V>;
+    hipLaunchKernelGGL(kern, dim3(1), dim3(CNET_TPB), (size_t)2 * L.g.nno * 16 * sizeof(uint16_t), s, L.g, F,
+                       (const float *)c->cnet_logeta, (const uint16_t *)c->cnet_w, (const float *)c->cnet_b,
+                       c->cnet_layers, (const float *)c->cnet_ys, L.NODE, d);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 template int cnet_apply<double>(hipStream_t, const chip_ctx *, const Level &, const double *, double *);
 template int cnet_apply<float>(hipStream_t, const chip_ctx *, const Level &, const float *, float *);

@@ -252,11 +252,11 @@ def make_samples(base: str, n: int, seed: int, log=None) -> list[Sample]:
 
 
 def train(samples: list[Sample], dims, epochs: int = 400, batch: int = 16, lr: float = 2e-3, seed: int = 0,
-          log=None, nin: int = NIN, nout: int = NOUT) -> tuple[ConvNet, list[float]]:
+          log=None, nin: int = NIN, nout: int = NOUT, layers: int = 4) -> tuple[ConvNet, list[float]]:
     """Adam on the mean squared error of the targets scaled per channel to
     unit RMS (the scales are kept with the network)."""
     import torch
-    net = ConvNet(seed=seed, nin=nin, nout=nout)
+    net = ConvNet(layers=layers, seed=seed, nin=nin, nout=nout)
     ys = np.stack([s.y for s in samples])
     net.yscale = np.ones(CH, np.float32)
     net.yscale[:nout] = np.sqrt((ys[..., :nout].astype(np.float64) ** 2).mean(axis=(0, 1))) + 1e-30
@@ -405,9 +405,9 @@ def make_coarse_samples(base: str, levels: int, n_fields: int, n_rhs: int, seed:
     return out
 
 
-def train_coarse(samples: list[CoarseSample], dims, epochs: int = 300, seed: int = 0,
+def train_coarse(samples: list[CoarseSample], dims, epochs: int = 300, seed: int = 0, layers: int = 4,
                  log=None) -> tuple[ConvNet, list[float]]:
-    return train(samples, dims, epochs=epochs, seed=seed, log=log, nin=4, nout=3)
+    return train(samples, dims, epochs=epochs, seed=seed, log=log, nin=4, nout=3, layers=layers)
 
 
 def coarse_predict(net: ConvNet, x: np.ndarray, s: float, dims) -> np.ndarray:

@@ -199,7 +199,8 @@ int chip_set_option(chip_ctx *c, const char *name, int value);
  * right-hand side over its RMS (3 channels), log10 viscosity (1 channel);
  * output channels 0..2 times out_scale[0..2] times that RMS, boundary rows
  * zero.  weights: BF16 bits [layers][16 out][27 taps][16 in], bias
- * [layers][16]; layers 1..8, ReLU after all but the last.  Copied to the
+ * [layers][16]; layers 1..8, ReLU after all but the last; a coarsest level
+ * of at most 1024 nodes (one workgroup runs the network).  Copied to the
  * device; the inner solve keeps iterating to its own residual criterion. */
 int chip_set_coarse_net(chip_ctx *c, int layers, const uint16_t *weights, const float *bias,
                         const float *out_scale);
