@@ -41,6 +41,22 @@ __device__ __forceinline__ int nbr(const Vol &g, int v, int t) {
     return kk + g.nz * (ii + g.nx * jj);
 }
 
+// the same with v's (k, i, j) formed once: in a loop over the taps, which
+// is then unrolled, no division is left per tap
+struct Cell {
+    int v, k, i, j;
+    bool in;
+};
+__device__ __forceinline__ Cell cell(const Vol &g, int v) {
+    return {v, v % g.nz, (v / g.nz) % g.nx, v / (g.nz * g.nx), v < g.nv};
+}
+__device__ __forceinline__ int nbr(const Vol &g, const Cell &c, int t) {
+    const int dz = t % 3 - 1, dx = (t / 3) % 3 - 1, dy = t / 9 - 1;
+    const bool ok = c.in && (unsigned)(c.k + dz) < (unsigned)g.nz && (unsigned)(c.i + dx) < (unsigned)g.nx &&
+                    (unsigned)(c.j + dy) < (unsigned)g.ny;
+    return ok ? c.v + dz + g.nz * (dx + g.nx * dy) : -1;
+}
+
 __device__ __forceinline__ bf16x4 load4(const uint16_t *p) {
     const uint2 r = *reinterpret_cast<const uint2 *>(p);
     bf16x4 a;
@@ -61,9 +77,11 @@ __global__ void __launch_bounds__(256) k_conv_fwd(Vol g, const uint16_t *__restr
     if (v0 >= g.nv) return;
     const uint16_t *Xb = X + (size_t)b * g.nv * CH;
     const int m = v0 + (l & 15), kq = 4 * (l >> 4);
+    const Cell cm = cell(g, m);
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
     for (int t = 0; t < TAPS; t++) {
-        const int vn = m < g.nv ? nbr(g, m, t) : -1;
+        const int vn = nbr(g, cm, t);
         bf16x4 a = {0, 0, 0, 0};
         if (vn >= 0) a = load4(Xb + (size_t)vn * CH + kq);
         const bf16x4 w = load4(W + ((size_t)(l & 15) * TAPS + t) * CH + kq);
@@ -100,11 +118,13 @@ __global__ void __launch_bounds__(256) k_conv_bwd_data(Vol g, const uint16_t *__
     if (v0 >= g.nv) return;
     const uint16_t *dYb = dY + (size_t)b * g.nv * CH;
     const int m = v0 + (l & 15), kq = 4 * (l >> 4);
+    const Cell cm = cell(g, m);
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
     for (int t = 0; t < TAPS; t++) {
         // the output voxel u with u + off_t = m is the neighbour of m at the
         // mirrored tap 26 - t
-        const int u = m < g.nv ? nbr(g, m, 26 - t) : -1;
+        const int u = nbr(g, cm, 26 - t);
         bf16x4 a = {0, 0, 0, 0};
         if (u >= 0) a = load4(dYb + (size_t)u * CH + kq);
         const bf16x4 w = load4(&Wt[t][l & 15][kq]);
