@@ -398,7 +398,8 @@ def make_coarse_samples(base: str, levels: int, n_fields: int, n_rhs: int, seed:
             x, s = coarse_features(F, leta)
             y = np.zeros_like(x)
             y[:, :3] = (d / s).reshape(-1, 3)
-            out.append(CoarseSample(x, y, F, d, c.apply("coarse", 0, None, F)))
+            mc = c.apply("gs0", 0, None, F, sweeps=p.v_steps_low)[0]   # what the coarse matrix encodes
+            out.append(CoarseSample(x, y, F, d, mc))
         c.close()
         if log and (k + 1) % 8 == 0:
             log(f"coarse fields {k + 1}/{n_fields}")
