@@ -816,9 +816,11 @@ __global__ void __launch_bounds__(256) k_affine_apply(const double *__restrict__
                                                       V *__restrict__ out, int n3) {
     const int t = threadIdx.x & 63, i = blockIdx.x * 4 + (threadIdx.x >> 6);
     double S = 0.0;
-    if (i < n3) {
+    if (i < n3) {   // unrolled so that the loads of several terms are in flight; the sum keeps its order
         if (din)
+#pragma unroll 8
             for (int j = t; j < n3; j += 64) S = S + Ma[(size_t)i * n3 + j] * (double)din[j];
+#pragma unroll 8
         for (int j = t; j < n3; j += 64) S = S + Mb[(size_t)i * n3 + j] * (double)F[j];
     }
 #pragma unroll
@@ -843,6 +845,7 @@ __global__ void __launch_bounds__(256) k_coarse_apply(const double *__restrict__
     const int t = threadIdx.x & 15, i = blockIdx.x * 16 + (threadIdx.x >> 4);
     double S = 0.0;
     if (i < n3)
+#pragma unroll 8
         for (int j = t; j < n3; j += 16) S = S + M[(size_t)i * n3 + j] * (double)F[j];
     S = S + __shfl_xor(S, 1, 64);
     S = S + __shfl_xor(S, 2, 64);
