@@ -317,8 +317,89 @@ __global__ void __launch_bounds__(256) k_matvec(Geo g, const KT *__restrict__ K,
     }
 }
 
+// The same product with all three planes' coefficients and vector entries
+// loaded before the first addition (one memory round trip instead of
+// three; the sums keep row_product's order).  For small levels, whose
+// matvec is latency-bound: 243 + 81 live values per thread, one wavefront
+// per workgroup.
+template <typename V>
+__device__ __forceinline__ void row_product_all(const Geo &g, const float *__restrict__ kp, int k, int i, int j,
+                                                const V *__restrict__ u, V out[3]) {
+    const int self = nid(g, k, i, j);
+    float kv[243];
+    V uv[81];
+#pragma unroll
+    for (int dy = -1; dy <= 1; dy++) {
+        const int P = dy + 1;
+        const bool oy = (unsigned)(j + dy) < (unsigned)g.noy;
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) {
+            const bool ox = oy && (unsigned)(i + dx) < (unsigned)g.nox;
+#pragma unroll
+            for (int dz = -1; dz <= 1; dz++) {
+                const int b = (dz + 1) + 3 * (dx + 1);
+                const bool ok = ox && (unsigned)(k + dz) < (unsigned)g.noz;
+                const int m = ok ? self + dz + g.noz * (dx + g.nox * dy) : self;
+                uv[27 * P + 3 * b] = u[3 * m];
+                uv[27 * P + 3 * b + 1] = u[3 * m + 1];
+                uv[27 * P + 3 * b + 2] = u[3 * m + 2];
+                const int s0 = 9 * (b + 9 * P);
+#pragma unroll
+                for (int t = 0; t < 9; t++) kv[s0 + t] = kp[(s0 + t) * 64];
+            }
+        }
+    }
+    out[0] = out[1] = out[2] = V(0);
+#pragma unroll
+    for (int P = 0; P < 3; P++)
+#pragma unroll
+        for (int b = 0; b < 9; b++)
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                const float *kb = kv + 81 * P + 9 * b + 3 * r;
+                const V *ub = uv + 27 * P + 3 * b;
+                out[r] += kb[0] * ub[0] + kb[1] * ub[1] + kb[2] * ub[2];
+            }
+}
+
+constexpr int MV_SMALL_NNO = 20000;   // levels up to this size use k_matvec_small
+
+template <typename V>
+__global__ void __launch_bounds__(64) k_matvec_small(Geo g, const float *__restrict__ K,
+                                                     const unsigned *__restrict__ NODE, const V *__restrict__ u,
+                                                     V *__restrict__ Au, const V *__restrict__ rhs,
+                                                     V *__restrict__ res) {
+    const int q = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int col = colour_valid(g, q);
+    if (col < 0) return;
+    int k, i, j;
+    colour_kij(g, col, q, k, i, j);
+    V out[3];
+    row_product_all(g, K + ktile(q), k, i, j, u, out);
+    const int n = nid(g, k, i, j);
+    strip3(NODE[n], out);
+    Au[3 * n] = out[0];
+    Au[3 * n + 1] = out[1];
+    Au[3 * n + 2] = out[2];
+    if (res) {
+        res[3 * n] = rhs[3 * n] - out[0];
+        res[3 * n + 1] = rhs[3 * n + 1] - out[1];
+        res[3 * n + 2] = rhs[3 * n + 2] - out[2];
+    }
+}
+
 template <typename V>
 void matvec(hipStream_t s, const Level &L, const V *u, V *Au, const V *rhs, V *res, hipEvent_t e0, hipEvent_t e1) {
+    if (!L.Kh && L.g.nno <= MV_SMALL_NNO) {
+        auto kern = k_matvec_small<V>;
+        const float *K = L.K;
+        if (e0)
+            hipExtLaunchKernelGGL(kern, dim3(nblk(L.g.pitch, 64)), dim3(64), 0, s, e0, e1, 0, L.g, K, L.NODE, u, Au,
+                                  rhs, res);
+        else
+            hipLaunchKernelGGL(kern, dim3(nblk(L.g.pitch, 64)), dim3(64), 0, s, L.g, K, L.NODE, u, Au, rhs, res);
+        return;
+    }
     if constexpr (sizeof(V) == 4) {
         if (L.Kh) {
             const _Float16 *Kh = (const _Float16 *)L.Kh;
