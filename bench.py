@@ -269,10 +269,16 @@ def run_form(p, T, mixed, steps, warmup, graphs, profile, coarse_net=None):
     for k, r in enumerate(recs):
         log(f"[{form}] step {k}: dt {r['pg']['timestep']:.3e} Nu_top {r['Nu_top']:.5f} "
             f"Uzawa {r['stokes']['iterations']} MG {r['stokes']['mg_cycles']}")
+    if coarse_net is not None:
+        still_on, fallbacks = c.coarse_net_state()
+        log(f"[{form}] learned coarse correction {'in use' if still_on else 'switched off'}; "
+            f"{fallbacks} inner solve(s) fell back to the plain multigrid")
     out = {"s_per_step": elapsed / steps,
            "uzawa_iterations_per_step": float(np.mean([r["stokes"]["iterations"] for r in recs])),
            "mg_calls_per_step": float(np.mean([r["stokes"]["mg_cycles"] for r in recs])),
            "Nu_top": float(recs[-1]["Nu_top"])}
+    if coarse_net is not None:
+        out["coarse_net_in_use"], out["fallbacks"] = still_on, fallbacks
     if profile:
         c.profile("gs_colour")
         c.timestep()
@@ -423,6 +429,8 @@ def main():
             f"V-cycle, trained in {learned['train_s']:.0f} s (inside this run, untimed) on "
             f"{learned['samples']} exact coarse solves of the workload family; held-out error "
             f"{learned['coarse_error']:.3f} against {learned['smoother_error']:.3f} for the coarse smoother; "
+            f"{'in use to the end' if learned['coarse_net_in_use'] else 'switched off by the plain-multigrid fallback'} "
+            f"({learned['fallbacks']} inner solve(s) restarted without it); "
             f"{learned['s_per_step'] / s_per_step:.3f}x the mixed form's time per step")
         log(f"mixed+learned form: {learned['s_per_step']:.3f} s/step, {learned['mg_calls_per_step']:.0f} MG calls")
     finest = {"achieved": ach_f, "frac": ach_f / PEAK_HBM_GBS, "launches": n_f,

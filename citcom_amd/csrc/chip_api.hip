@@ -302,10 +302,31 @@ int solve_del2_u(chip_ctx *c, double *d0, const double *F, double acc, int *vali
     acc = std::max(acc, r0 * c->p.accuracy);
     *valid = (residual < acc) ? 0 : 1;
     int cnt = 0;
+    double prev = residual;
     while (residual > acc) {
         if (multi_grid(c, c->sr, &residual)) return -1;
         if (!c->p.mixed) axpy_scalar(S(c), d0, c->L[c->levmax].vel, 1.0, neq);
         cnt++;
+        // the plain multigrid is the fallback of the learned coarse
+        // correction: a call that does not reduce the residual (or gives a
+        // non-finite one) switches the correction off for the rest of the
+        // run and restarts this solve with the coarsest level's smoother
+        if (c->cnet_on && !(residual < prev)) {
+            CHIP_CHECK(hipStreamSynchronize(S(c)));
+            c->cnet_on = false;
+            c->cnet_fallbacks++;
+            if (c->mg_graph) {
+                CHIP_CHECK(hipGraphExecDestroy(c->mg_graph));
+                c->mg_graph = nullptr;
+            }
+            if (c->print_convergence)
+                std::fprintf(stderr, " *  learned coarse correction off: residual %.3e after %.3e\n", residual, prev);
+            copy(S(c), c->sr, F, neq);
+            fill(S(c), d0, 0.0, neq);
+            residual = prev = r0;
+            continue;
+        }
+        prev = residual;
         if (cnt > 10000) {
             set_error("solve_del2_u: multigrid does not converge");
             return -1;
@@ -914,6 +935,13 @@ int chip_set_coarse_net(chip_ctx *c, int layers, const uint16_t *weights, const 
         CHIP_CHECK(hipGraphExecDestroy(c->mg_graph));
         c->mg_graph = nullptr;
     }
+    return 0;
+}
+
+int chip_coarse_net_state(chip_ctx *c, int *on, int *fallbacks) {
+    CHIP_NEED_CTX(c);
+    if (on) *on = c->cnet_on ? 1 : 0;
+    if (fallbacks) *fallbacks = c->cnet_fallbacks;
     return 0;
 }
 

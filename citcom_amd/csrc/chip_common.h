@@ -130,6 +130,7 @@ struct chip_ctx {
     int cnet_layers = 0;
     bool cnet_on = false;
     bool cnet_failed = false;   // a launch of the correction failed while enqueueing
+    int cnet_fallbacks = 0;     // inner solves restarted without it (no residual reduction)
     uint16_t *cnet_w = nullptr;
     float *cnet_b = nullptr, *cnet_ys = nullptr, *cnet_logeta = nullptr;
     bool print_convergence = false;  // per-iteration lines on stderr, as E->control.print_convergence

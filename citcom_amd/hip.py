@@ -111,6 +111,7 @@ def lib():
         L.chip_profile_read.argtypes = [P, C.POINTER(C.c_long), C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.chip_set_option.argtypes = [P, C.c_char_p, C.c_int]
         L.chip_set_coarse_net.argtypes = [P, C.c_int, P, P, P]
+        L.chip_coarse_net_state.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.chip_device_bytes.restype = C.c_long
         L.chip_device_bytes.argtypes = [P]
         _lib = L
@@ -298,6 +299,13 @@ class Citcom:
 
     def set_option(self, name: str, value: int) -> None:
         _check(lib().chip_set_option(self.h, name.encode(), int(value)), f"set_option {name}")
+
+    def coarse_net_state(self) -> tuple[bool, int]:
+        """(learned coarse correction still in use, inner solves that fell back
+        to the plain multigrid)"""
+        on, fb = C.c_int(), C.c_int()
+        _check(lib().chip_coarse_net_state(self.h, C.byref(on), C.byref(fb)), "coarse_net_state")
+        return bool(on.value), fb.value
 
     def set_coarse_net(self, weights: np.ndarray, bias: np.ndarray, out_scale: np.ndarray) -> None:
         """learned coarse-level correction (chip_set_coarse_net): weights as BF16

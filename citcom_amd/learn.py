@@ -377,13 +377,15 @@ class CoarseSample:
 def make_coarse_samples(base: str, levels: int, n_fields: int, n_rhs: int, seed: int,
                         log=None) -> list[CoarseSample]:
     """coarsest-level problems of the workload's family: random temperature
-    fields at the workload's viscosity law, n_rhs random right-hand sides
+    fields under the workload's own viscosity law (every material layer as
+    the deck has it) with `levels` levels, n_rhs random right-hand sides
     each, solved exactly on the free dofs"""
     rng = np.random.default_rng(seed)
-    contrast = float(np.exp(float(_deck.workload(base)["viscE"])))
+    keys = _deck.workload(base)
+    keys["levels"] = levels
     out = []
     for k in range(n_fields):
-        p = problem_with_contrast(base, contrast, levels)
+        p = _deck.problem_from_deck({kk: str(v) for kk, v in keys.items()})
         T = random_temperature(p, rng)
         c = Citcom(p)
         c.initialize(T)

@@ -242,3 +242,30 @@ def test_coarse_correction_in_multigrid():
     say(f"tiny_tdep @1e-9: Uzawa {s0['iterations']} / {s1['iterations']} its, multigrid calls "
         f"{s0['mg_cycles']} (coarse smoother) / {s1['mg_cycles']} (learned correction); U {eu:.2e}, P {ep:.2e}")
     assert eu < 1e-6 and ep < 1e-6
+
+
+def test_coarse_correction_fallback():
+    """The plain multigrid is the learned correction's fallback: an untrained
+    network with a huge output scale makes the first multigrid call raise the
+    residual; the solve switches the correction off, restarts with the
+    coarsest level's smoother and reaches the plain solve's U and P."""
+    from citcom_amd import deck, learn
+    from citcom_amd.hip import Citcom
+    from conftest import problem
+    p = problem("tiny_tdep@1e-9")
+    net = learn.ConvNet(seed=7, nin=4, nout=3)
+    net.yscale = np.full(16, 1e6, np.float32)
+    runs = {}
+    for on in (0, 1):
+        c = Citcom(p)
+        c.initialize(deck.perturbed_temperature(p))
+        if on:
+            learn.install_coarse_net(net, c)
+        st = c.general_stokes_solver()
+        runs[on] = (c.download("U"), c.download("P"), st, c.coarse_net_state() if on else None)
+        c.close()
+    (U0, P0, s0, _), (U1, P1, s1, state) = runs[0], runs[1]
+    eu, ep = rel_l2(U1, U0), rel_l2(P1 - P1.mean(), P0 - P0.mean())
+    say(f"fallback: network state {state}, Uzawa {s0['iterations']} / {s1['iterations']} its; U {eu:.2e}, P {ep:.2e}")
+    assert state[0] is False and state[1] >= 1
+    assert eu < 1e-6 and ep < 1e-6
