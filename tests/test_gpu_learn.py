@@ -269,3 +269,40 @@ def test_coarse_correction_fallback():
     say(f"fallback: network state {state}, Uzawa {s0['iterations']} / {s1['iterations']} its; U {eu:.2e}, P {ep:.2e}")
     assert state[0] is False and state[1] >= 1
     assert eu < 1e-6 and ep < 1e-6
+
+
+def test_coarse_correction_against_oracle():
+    """k_cnet (the whole correction in one workgroup) against the numpy
+    restatement of its layers (oracle/learn_oracle.py: the same BF16
+    operands, products summed in FP64, BF16 activations between layers)."""
+    import learn_oracle as lo
+    from citcom_amd import deck, learn
+    from citcom_amd.hip import Citcom
+    from conftest import problem
+    p = problem("tiny_tdep")
+    c = Citcom(p)
+    c.initialize(deck.perturbed_temperature(p))
+    c.get_system_viscosity()
+    c.construct_stiffness_B_matrix()
+    net = learn.ConvNet(seed=11, nin=4, nout=3)
+    net.yscale = np.ones(16, np.float32)
+    net.yscale[:3] = (1.5, 0.5, 2.0)
+    learn.install_coarse_net(net, c, on=False)
+    free = learn.coarse_free(c)
+    F = np.random.default_rng(9).standard_normal(len(free)) * free
+    dev = c.apply("cnet", 0, None, F)
+    x, s = learn.coarse_features(F, learn.coarse_logeta(c, p))
+    c.close()
+    dims = p.dims(0)
+    a = lo.to_bf16(x)[None]
+    for li in range(net.layers):
+        w = net.wbf[li].float().cpu().numpy()
+        b = net.b[li].cpu().numpy()
+        a_bf, a_f = lo.conv_fwd(a, w, b, int(li < net.layers - 1), dims)
+        a = a_bf if li < net.layers - 1 else a_f
+    ref = (a[0][:, :3].astype(np.float64) * net.yscale[:3] * s).reshape(-1) * free
+    say(f"coarse correction against the oracle: rel L2 {rel_l2(dev, ref):.2e}")
+    # FP32 MFMA accumulation against FP64 sums, through BF16 activations
+    # (measured 4.4e-8; a BF16 rounding flip of one activation stays far
+    # below the bound)
+    assert rel_l2(dev, ref) < 1e-4
