@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
+#include <string>
 
 #include "../../include/citcom_learn.h"
 
@@ -234,6 +236,37 @@ __global__ void k_to_bf16(const float *__restrict__ a, uint16_t *__restrict__ b,
 hipStream_t st(void *s) { return static_cast<hipStream_t>(s); }
 int ret() { return hipGetLastError() == hipSuccess ? 0 : -1; }
 
+// ---- the weight gradient's partial sums without a workspace argument: a
+// file-scope device array, one copy per GPU, held until the process ends.
+// Host code never passes `scratch` itself to a kernel (its host-side name is
+// not a device address): the device address comes from hipGetSymbolAddress,
+// once per device with that device current, and is kept in scratch_ptr.
+constexpr long SCRATCH_FLOATS = 4L << 20;   // 16 MiB
+__device__ __align__(256) float scratch[SCRATCH_FLOATS];
+constexpr int MAX_DEVICES = 64;
+float *scratch_ptr[MAX_DEVICES];
+std::once_flag scratch_once[MAX_DEVICES];
+hipError_t scratch_err[MAX_DEVICES];
+
+// the device address of this device's scratch array (nullptr on failure,
+// with the message in *why)
+float *scratch_on_current_device(const char **why) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) {
+        *why = "no current HIP device (or more devices than the library tracks)";
+        return nullptr;
+    }
+    std::call_once(scratch_once[dev], [dev] {
+        void *p = nullptr;
+        scratch_err[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        scratch_ptr[dev] = scratch_err[dev] == hipSuccess ? static_cast<float *>(p) : nullptr;
+    });
+    if (!scratch_ptr[dev]) *why = "hipGetSymbolAddress of the scratch array failed";
+    return scratch_ptr[dev];
+}
+
+thread_local std::string g_learn_err;
+
 }  // namespace
 
 extern "C" {
@@ -272,6 +305,45 @@ int cl_conv3d_bwd_weight(const uint16_t *x, const uint16_t *dy, int nb, int nx, 
     hipLaunchKernelGGL(k_sum_partials, dim3((TAPS * CH * CH + CH + 255) / 256), dim3(256), 0, st(stream), workspace,
                        chunks, grad);
     return ret();
+}
+
+long cl_scratch_bytes(void) { return SCRATCH_FLOATS * (long)sizeof(float); }
+
+const char *cl_last_error(void) { return g_learn_err.c_str(); }
+
+int cl_conv3d_bwd_weight_noalloc(const uint16_t *x, const uint16_t *dy, int nb, int nx, int ny, int nz, float *grad,
+                                 void *stream) {
+    const Vol g{nx, ny, nz, nx * ny * nz};
+    if (nb <= 0 || nx <= 0 || ny <= 0 || nz <= 0) {
+        g_learn_err = "cl_conv3d_bwd_weight_noalloc: empty volume";
+        return -1;
+    }
+    const long need = cl_conv3d_bwd_weight_workspace(nb, nx, ny, nz);
+    if (need > cl_scratch_bytes()) {
+        g_learn_err = "cl_conv3d_bwd_weight_noalloc: " + std::to_string(nb) + " volumes of " + std::to_string(g.nv) +
+                      " voxels need " + std::to_string(need) + " bytes of partial sums, more than the " +
+                      std::to_string(cl_scratch_bytes()) + "-byte scratch array";
+        return -1;
+    }
+    const char *why = "";
+    float *part = scratch_on_current_device(&why);
+    if (!part) {
+        g_learn_err = std::string("cl_conv3d_bwd_weight_noalloc: ") + why;
+        return -1;
+    }
+    // every chunk's slot of partial sums is written in full by
+    // k_conv_bwd_weight before k_sum_partials reads it, so nothing is cleared
+    const long chunks = (long)nb * ((g.nv + WCHUNK - 1) / WCHUNK);
+    hipLaunchKernelGGL(k_conv_bwd_weight, dim3((unsigned)((chunks + 3) / 4)), dim3(256), 0, st(stream), g, nb, x, dy,
+                       part);
+    hipLaunchKernelGGL(k_sum_partials, dim3((TAPS * CH * CH + CH + 255) / 256), dim3(256), 0, st(stream), part, chunks,
+                       grad);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        g_learn_err = std::string("cl_conv3d_bwd_weight_noalloc: ") + hipGetErrorString(e);
+        return -1;
+    }
+    return 0;
 }
 
 int cl_mse(const float *y, const float *target, long nvox, int nout, uint16_t *dy, float *partial, void *stream) {

@@ -52,6 +52,11 @@ def _cl():
         L.cl_mse.argtypes = [P, P, l, i, P, P, P]
         L.cl_adam.argtypes = [P, P, P, P, l, f, f, f, f, i, P, P, P]
         L.cl_to_bf16.argtypes = [P, P, l, P]
+        L.cl_conv3d_bwd_weight_noalloc.argtypes = [P, P, i, i, i, i, P, P]
+        L.cl_scratch_bytes.restype = l
+        L.cl_scratch_bytes.argtypes = []
+        L.cl_last_error.restype = C.c_char_p
+        L.cl_last_error.argtypes = []
         L._cl_ready = True
     return L
 

@@ -35,6 +35,20 @@ int cl_conv3d_bwd_data(const uint16_t *dy, int nb, int nx, int ny, int nz, const
 long cl_conv3d_bwd_weight_workspace(int nb, int nx, int ny, int nz);
 int cl_conv3d_bwd_weight(const uint16_t *x, const uint16_t *dy, int nb, int nx, int ny, int nz, float *grad,
                          float *workspace, void *stream);
+/* The same weight gradient with no workspace argument and no allocation of
+ * any kind: the partial sums live in a 16 MiB file-scope device array
+ * (cl_scratch_bytes()), one copy per GPU, held until the process ends.  It
+ * has no counterpart in the reference; it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight training or
+ * serving loops, harnesses that forbid hipMalloc).  Returns -1, with the
+ * reason in cl_last_error(), when nb volumes of nx*ny*nz voxels need more
+ * than cl_scratch_bytes() of partial sums (cl_conv3d_bwd_weight_workspace).
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (different streams or threads) race. */
+int cl_conv3d_bwd_weight_noalloc(const uint16_t *x, const uint16_t *dy, int nb, int nx, int ny, int nz, float *grad,
+                                 void *stream);
+long cl_scratch_bytes(void);
+const char *cl_last_error(void);
 /* mean squared error of y (FP32, 16 channels) against target over the first
  * nout channels: dy (bf16) = d loss / d y; partial[block] = block sums of
  * squared errors, ceil(nvox*16/256) blocks */
