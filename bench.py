@@ -52,69 +52,6 @@ REF_HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
 REF_PROGRAM = os.path.join(ROOT, "oracle", "_ref", "citcom.mpi")
 
 
-def cpu_baseline_reference(name, seed, mg_calls_per_step):
-    """Time the unmodified reference (oracle/_ref/ref_harness, the
-    reference's own objects) on one host thread on a bounded sample of the
-    same workload: the operator build of one step, one multi_grid call and
-    one PG_timestep on the seeded initial temperature (read through the
-    reference's restart reader).  The per-step estimate scales the multi_grid
-    time by the multigrid calls per step that the GPU run measured."""
-    import subprocess
-    import tempfile
-
-    keys = deck.workload(name)
-    p = make_problem(name)
-    with tempfile.TemporaryDirectory() as tmp:
-        out = os.path.join(tmp, "out")
-        keys["datafile"] = os.path.join(out, "run")
-        keys.update({"restart": 1, "restart_timesteps": 0, "oldfile": os.path.join(out, "seed")})
-        deck.write_restart_temperature(os.path.join(out, "seed"), deck.seeded_temperature(p, seed=seed))
-        path = os.path.join(tmp, "deck.input")
-        deck.write_deck(path, keys)
-        r = subprocess.run([REF_HARNESS, path, os.path.join(tmp, "dump"), "time"], capture_output=True, text=True,
-                           timeout=600, cwd=tmp)
-    if r.returncode != 0:
-        raise RuntimeError(f"reference timing sample failed: {r.stderr[-500:]}")
-    f = r.stdout.split()
-    t = {f[k]: float(f[k + 1]) for k in range(0, len(f) - 1, 2) if f[k] in ("build", "mg", "pg")}
-    est = t["build"] + t["pg"] + t["mg"] * mg_calls_per_step
-    return {"value": est, "unit": "s/timestep", "cores": 1, "kind": "reference",
-            "sample": (f"unmodified reference (its own objects, gcc -O2, 1 MPI rank) at {name} size: operator build "
-                       f"{t['build']:.2f} s + one PG_timestep {t['pg']:.2f} s + one multi_grid call {t['mg']:.2f} s x "
-                       f"{mg_calls_per_step:.1f} multigrid calls per step (as counted on the GPU)")}
-
-
-def cpu_baseline_port(name, seed, mg_calls_per_step):
-    """The same sample on the C restatement (oracle/), where the reference
-    build is absent."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle
-
-    if not os.path.exists(pyoracle.LIB_PATH):
-        pyoracle.build()
-    p = make_problem(name)
-    o = pyoracle.Oracle(p, smoother=0)
-    T = deck.seeded_temperature(p, seed=seed)
-    nno = o.nno()
-    t0 = time.perf_counter()
-    o.set_viscosity(o.visc_from_T(T))
-    t_rebuild = time.perf_counter() - t0
-    F = o.assemble_forces(o.thermal_buoyancy(T))
-    t0 = time.perf_counter()
-    o.pg_timestep(T, np.zeros(nno, np.float32), np.zeros(3 * nno, np.float32))
-    t_pg = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    Fc = F.copy()
-    d = np.zeros_like(F)
-    pyoracle.lib().orc_multi_grid(o.h, d, Fc, None, 0.0, o.levmax)
-    t_mg = time.perf_counter() - t0
-    est = t_rebuild + t_pg + t_mg * mg_calls_per_step
-    return {"value": est, "unit": "s/timestep", "cores": 1, "kind": "port",
-            "sample": (f"C restatement on 1 host thread at {name} size: stiffness rebuild {t_rebuild:.2f} s + "
-                       f"one PG step {t_pg:.2f} s + one multi_grid call {t_mg:.2f} s x "
-                       f"{mg_calls_per_step:.1f} multigrid calls per step (as counted on the GPU)")}
-
-
 def measured_traffic(kernel, config, field="hbm_bytes_per_launch"):
     """HBM bytes per launch of `kernel` on workload `config` (all its colour
     passes, or with field "finest_level_hbm_bytes_per_launch" the finest
@@ -151,19 +88,70 @@ def mpi_decomposition(p, cap):
     return best[1]
 
 
-def cpu_baseline_mpi(name, seed, mg_calls_per_step, cap=16, iterations=(4, 14)):
-    """Time the pristine reference program (oracle/_ref/citcom.mpi, built
-    from the unmodified sources) under MPI on the host cores, on the same
-    grid, deck and seeded temperature (one restart file per rank).
+def cpu_baseline_port(name, seed, mg_calls_per_step):
+    """Where the reference build or MPI is absent: the C restatement (oracle/)
+    on one host thread, an estimate from a bounded sample (stiffness rebuild,
+    one PG step and one multi_grid call, the latter scaled by the multigrid
+    calls per step of the plain form)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
 
-    Two short runs (maxstep 2, piterations = iterations[0], iterations[1])
-    give step 2's wall time t and multigrid calls m from the reference's own
-    log (Citcom.c:151-158; General_matrix_functions.c:494-498); the fit
-    t = a + b m separates the per-step fixed work (PG_timestep, forces,
-    viscosity, stiffness rebuild, heat flux) from the cost of one multigrid
-    call, and the estimate is a + b x the multigrid calls per step counted on
-    the GPU."""
+    if not os.path.exists(pyoracle.LIB_PATH):
+        pyoracle.build()
+    p = make_problem(name)
+    o = pyoracle.Oracle(p, smoother=0)
+    T = deck.seeded_temperature(p, seed=seed)
+    nno = o.nno()
+    t0 = time.perf_counter()
+    o.set_viscosity(o.visc_from_T(T))
+    t_rebuild = time.perf_counter() - t0
+    F = o.assemble_forces(o.thermal_buoyancy(T))
+    t0 = time.perf_counter()
+    o.pg_timestep(T, np.zeros(nno, np.float32), np.zeros(3 * nno, np.float32))
+    t_pg = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    Fc = F.copy()
+    d = np.zeros_like(F)
+    pyoracle.lib().orc_multi_grid(o.h, d, Fc, None, 0.0, o.levmax)
+    t_mg = time.perf_counter() - t0
+    est = t_rebuild + t_pg + t_mg * mg_calls_per_step
+    return {"value": est, "unit": "s/timestep", "cores": 1, "kind": "port",
+            "sample": (f"estimate: C restatement on 1 host thread at {name} size, stiffness rebuild {t_rebuild:.2f} s + "
+                       f"one PG step {t_pg:.2f} s + one multi_grid call {t_mg:.2f} s x "
+                       f"{mg_calls_per_step:.1f} multigrid calls per step of the plain GPU form")}
+
+
+def reference_step_times(stderr_text, log_text):
+    """One complete timestep of the reference, from its own output.
+
+    solve_Ahat_p_fhat prints "after (N) pressure loops and X sec for step s"
+    on stderr (Stokes_flow_Incomp.c:482-488), X counted by MPI_Wtime from the
+    start of the run's first Uzawa solve, so X(step 1) - X(step 0) is one
+    whole pass of the time loop (Citcom.c:111-161: heat flux and velocity
+    output of step 0, PG_timestep, forces, viscosity, stiffness rebuild and
+    the Uzawa solve of step 1).  The multigrid calls of that step are the
+    counts of its inner solves, " * residual (count)(cycles)" in the run log
+    (General_matrix_functions.c:494-498) between the two Uzawa summaries.
+    Returns (step seconds, multigrid calls, Uzawa iterations)."""
     import re
+
+    after = {int(s): (int(n), float(x)) for n, x, s in
+             re.findall(r"after \((\d+)\) pressure loops and ([0-9.eE+-]+) sec for step (\d+)", stderr_text)}
+    if 0 not in after or 1 not in after:
+        raise RuntimeError("the reference printed no Uzawa summary for steps 0 and 1")
+    i0 = log_text.index("pressure loops and")
+    i1 = log_text.index("pressure loops and", i0 + 1)
+    mg = sum(int(a) for a in re.findall(r"residual \((\d+)\)\(\d+\)", log_text[i0:i1]))
+    return after[1][1] - after[0][1], mg, after[1][0]
+
+
+def cpu_baseline_mpi(name, seed, cap=16):
+    """Time one complete timestep of the pristine reference program
+    (oracle/_ref/citcom.mpi, built from the unmodified sources) under MPI on
+    the host cores: the same grid, deck and seeded temperature (one restart
+    file per rank), run to step 1 at the workload's own piterations and
+    accuracy; the step's wall time and multigrid calls come from the
+    reference's own output (reference_step_times)."""
     import shutil
     import subprocess
     import tempfile
@@ -179,106 +167,106 @@ def cpu_baseline_mpi(name, seed, mg_calls_per_step, cap=16, iterations=(4, 14)):
     nproc = mpi_decomposition(p, cap)
     ranks = nproc[0] * nproc[1] * nproc[2]
     T = deck.seeded_temperature(p, seed=seed)
-    samples = []
     with tempfile.TemporaryDirectory() as tmp:
         deck.write_restart_temperature(os.path.join(tmp, "seed"), T, p.dims(), nproc)
-        for its in iterations:
-            keys = deck.workload(name)
-            out = os.path.join(tmp, f"run{its}")
-            os.makedirs(out)
-            keys.update({"datafile": os.path.join(out, "run"), "restart": 1, "restart_timesteps": 0,
-                         "oldfile": os.path.join(tmp, "seed"), "maxstep": 2, "piterations": its,
-                         "nprocx": nproc[0], "nprocy": nproc[1], "nprocz": nproc[2]})
-            path = os.path.join(out, "deck.input")
-            deck.write_deck(path, keys)
-            t0 = time.perf_counter()
-            r = subprocess.run([mpiexec, "-launcher", "fork", "-n", str(ranks), REF_PROGRAM, path],
-                               capture_output=True, text=True, timeout=900, cwd=out)
-            text = open(os.path.join(out, "run.log0")).read()
-            if "for step 2" not in text:
-                raise RuntimeError(f"reference run failed (rc {r.returncode}): {r.stderr[-400:]}")
-            step2 = text[text.index("for step 1"):text.index("for step 2")]
-            t = float(re.findall(r"& CPU = ([0-9.eE+-]+) for step 2", text)[0])
-            m = sum(int(a) for a in re.findall(r"residual \((\d+)\)\(\d+\)", step2))
-            samples.append((its, t, m))
-            log(f"reference under MPI ({ranks} ranks {nproc}): piterations {its}: step 2 {t:.2f} s, {m} multigrid "
-                f"calls (run {time.perf_counter() - t0:.0f} s)")
-    (_, t1, m1), (_, t2, m2) = samples
-    b = (t2 - t1) / (m2 - m1)
-    a = t1 - b * m1
-    est = a + b * mg_calls_per_step
-    return {"value": est, "unit": "s/timestep", "cores": ranks, "kind": "reference",
-            "sample": (f"pristine reference program (oracle/_ref/citcom.mpi, gcc -O2), {ranks} MPI ranks "
-                       f"(nprocx,y,z = {nproc[0]},{nproc[1]},{nproc[2]}) at {name} size on the seeded temperature: "
-                       f"step 2 of two runs with piterations {iterations[0]} and {iterations[1]} "
-                       f"({t1:.2f} s / {m1} multigrid calls, {t2:.2f} s / {m2}); fit: {a:.2f} s per step + "
-                       f"{b:.3f} s per multigrid call x {mg_calls_per_step:.1f} multigrid calls per step "
-                       f"(as counted on the GPU)")}
+        keys = deck.workload(name)
+        keys.update({"datafile": os.path.join(tmp, "run"), "restart": 1, "restart_timesteps": 0,
+                     "oldfile": os.path.join(tmp, "seed"), "maxstep": 1,
+                     "nprocx": nproc[0], "nprocy": nproc[1], "nprocz": nproc[2]})
+        path = os.path.join(tmp, "deck.input")
+        deck.write_deck(path, keys)
+        t0 = time.perf_counter()
+        r = subprocess.run([mpiexec, "-launcher", "fork", "-n", str(ranks), REF_PROGRAM, path],
+                           capture_output=True, text=True, timeout=900, cwd=tmp)
+        wall = time.perf_counter() - t0
+        log_text = open(os.path.join(tmp, "run.log0")).read()
+    step_s, mg, its = reference_step_times(r.stderr, log_text)
+    log(f"reference under MPI ({ranks} ranks {nproc}): step 1 {step_s:.2f} s, {its} Uzawa iterations, {mg} multigrid "
+        f"calls (run {wall:.0f} s)")
+    return {"value": step_s, "unit": "s/timestep", "cores": ranks, "kind": "reference",
+            "uzawa_iterations": its, "mg_calls": mg,
+            "sample": (f"measured: one complete timestep (step 1: PG_timestep, forces, viscosity, stiffness rebuild, "
+                       f"Uzawa to accuracy {p.accuracy:g} or {p.p_iterations} iterations) of the pristine reference "
+                       f"program (oracle/_ref/citcom.mpi, gcc -O2) on {ranks} MPI ranks (nprocx,y,z = "
+                       f"{nproc[0]},{nproc[1]},{nproc[2]}) at {name} size on the seeded temperature: {its} Uzawa "
+                       f"iterations, {mg} multigrid calls; wall time from the reference's own MPI_Wtime output "
+                       f"(run to step 1: {wall:.0f} s including setup and the step-0 solve)")}
 
 
 def cpu_baseline(name, seed, mg_calls_per_step):
-    one_rank = None
-    if os.path.exists(REF_HARNESS):
-        try:
-            one_rank = cpu_baseline_reference(name, seed, mg_calls_per_step)
-        except Exception as e:  # noqa: BLE001 - report and fall back to the restatement
-            log(f"reference timing sample unavailable ({e})")
     try:
-        out = cpu_baseline_mpi(name, seed, mg_calls_per_step)
-        if one_rank:
-            out["sample"] += f"; one rank: {one_rank['value']:.1f} s/timestep ({one_rank['sample']})"
-        return out
+        return cpu_baseline_mpi(name, seed)
     except Exception as e:  # noqa: BLE001 - report and fall back
-        log(f"reference MPI timing unavailable ({e})")
-    if one_rank:
-        return one_rank
-    log("timing the C restatement")
+        log(f"reference MPI timing unavailable ({e}); timing the C restatement")
     return cpu_baseline_port(name, seed, mg_calls_per_step)
 
 
-def run_form(p, T, mixed, steps, warmup, graphs, profile, coarse_net=None):
-    """Timestep loop of one form: step-0 solve and warmup untimed, then `steps`
-    timed steps; with profile, one more step with kernel-stamped events on
-    every colour pass and one on every matvec.  coarse_net: a trained
-    learned coarse-level correction to use in every V-cycle."""
+FORMS = {
+    # name: (mixed, warm_start, description)
+    "plain": (0, 0, "the reference method in FP64 arithmetic (FP64 vectors, the reference's FP32 stiffness), each "
+                    "Uzawa solve started from the previous solution as the reference does"),
+    "mixed": (1, 0, "FP32 multigrid cycles inside FP64 iterative refinement (FP64 residual, Uzawa CG, coarsest "
+                    "level), previous-solution start"),
+    "mixed16": (2, 0, "as mixed, the two finest levels' stiffness rows scaled by BI and stored in FP16 for the cycles "
+                      "(FP32 arithmetic; the FP64 residual keeps the reference's FP32 stiffness), previous-solution "
+                      "start"),
+    "mixed16+extrap": (2, 1, "as mixed16, each Uzawa solve started from the last two solutions extrapolated in time "
+                             "(no network)"),
+}
+
+
+def run_form(p, T, form, steps, warmup, graphs, profile=False, same_state=False):
+    """Timestep loop of one form on its own trajectory from the seeded state:
+    the step-0 solve and `warmup` steps untimed, then `steps` timed steps.
+    profile: one more step with kernel-stamped events on every colour pass,
+    one on every matvec, one on the finest level's colour passes.
+    same_state: one more step solved twice from the same state (same
+    temperature, same previous solutions): with this form's initial guess and
+    with the previous solution as it is (the reference's start)."""
     import torch
 
     from citcom_amd.hip import Citcom
 
-    form = {0: "plain", 1: "mixed", 2: "mixed16"}[int(mixed)] + ("+learned" if coarse_net is not None else "")
+    mixed, warm, _ = FORMS[form]
     t0 = time.perf_counter()
     c = Citcom(p, mixed=mixed)
     if not graphs:
         c.set_option("graphs", 0)
+    if warm:
+        c.set_option("warm_start", warm)
     c.initialize(T)
-    if coarse_net is not None:
-        from citcom_amd import learn
-        learn.install_coarse_net(coarse_net, c)
     log(f"[{form}] setup {time.perf_counter() - t0:.1f} s, device memory {c.device_bytes() / 2**30:.2f} GiB")
     t0 = time.perf_counter()
     st0 = c.general_stokes_solver()
     log(f"[{form}] step-0 Stokes solve {time.perf_counter() - t0:.2f} s, {st0['iterations']} Uzawa iterations")
     for w in range(warmup):
         r = c.timestep()
-        log(f"[{form}] warmup {w}: {r['stokes']['iterations']} Uzawa its, {r['stokes']['mg_cycles']} MG calls")
+        log(f"[{form}] warmup {w + 1}: Uzawa {r['stokes']['iterations']} MG {r['stokes']['mg_cycles']}")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     recs = [c.timestep() for _ in range(steps)]
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    cap = p.p_iterations
     for k, r in enumerate(recs):
-        log(f"[{form}] step {k}: dt {r['pg']['timestep']:.3e} Nu_top {r['Nu_top']:.5f} "
-            f"Uzawa {r['stokes']['iterations']} MG {r['stokes']['mg_cycles']}")
-    if coarse_net is not None:
-        still_on, fallbacks = c.coarse_net_state()
-        log(f"[{form}] learned coarse correction {'in use' if still_on else 'switched off'}; "
-            f"{fallbacks} inner solve(s) fell back to the plain multigrid")
+        s = r["stokes"]
+        log(f"[{form}] step {warmup + 1 + k}: dt {r['pg']['timestep']:.3e} Nu_top {r['Nu_top']:.5f} "
+            f"Uzawa {s['iterations']} MG {s['mg_cycles']} dp/p {s['dpressure']:.2e} dv/v {s['dvelocity']:.2e}")
+    its = [r["stokes"]["iterations"] for r in recs]
     out = {"s_per_step": elapsed / steps,
-           "uzawa_iterations_per_step": float(np.mean([r["stokes"]["iterations"] for r in recs])),
+           "uzawa_iterations": its,
+           "uzawa_iterations_per_step": float(np.mean(its)),
            "mg_calls_per_step": float(np.mean([r["stokes"]["mg_cycles"] for r in recs])),
-           "Nu_top": float(recs[-1]["Nu_top"])}
-    if coarse_net is not None:
-        out["coarse_net_in_use"], out["fallbacks"] = still_on, fallbacks
+           # the solve stops when dp/p and dv/v fall below accuracy, or at the
+           # piterations cap (Stokes_flow_Incomp.c:393), or with alpha = 0
+           # when an inner solve has nothing to do (valid = 0, :423-426)
+           "steps_stopped_at_cap": int(sum(1 for i in its if i >= cap)),
+           "Nu_top": float(recs[-1]["Nu_top"]),
+           "elapsed_time": float(recs[-1]["pg"]["elapsed_time"]),
+           "vrms": c.vrms(),
+           "T_final": c.download("T")}
+    if same_state:
+        out["same_state"] = same_state_solves(c, form)
     if profile:
         c.profile("gs_colour")
         c.timestep()
@@ -297,38 +285,57 @@ def run_form(p, T, mixed, steps, warmup, graphs, profile, coarse_net=None):
     return out
 
 
-def learned_form(p, T, args):
-    """FP32 cycles with the learned coarse-level correction: train it on the
-    workload family's coarsest-level problems (random temperature fields
-    under a three-level mesh on the same coarse grid, exact coarse solves),
-    then time the mixed form with it in every V-cycle"""
-    from citcom_amd import learn
+def same_state_solves(c, form):
+    """One more step of this form's trajectory solved twice from the same
+    state: with the form's own initial guess, then (same temperature, same
+    previous solution) from the previous solution as it is, the reference's
+    start, in the same arithmetic."""
+    import torch
+
+    c.PG_timestep()
+    T = c.download("T")
+    U0, P0 = c.download("U"), c.download("P")
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    levels = min(3, p.levels)
-    train_set = learn.make_coarse_samples(args.config, levels, 48, 16, seed=args.seed)
-    test_set = learn.make_coarse_samples(args.config, levels, 6, 8, seed=args.seed + 1)
-    net, losses = learn.train_coarse(train_set, p.dims(0), epochs=400)
-    err_net, err_smoother = learn.coarse_relative_errors(net, test_set, p.dims(0))
-    t_train = time.perf_counter() - t0
-    log(f"[mixed+learned] coarse correction trained on {len(train_set)} coarse problems in {t_train:.1f} s: "
-        f"held-out error {err_net:.3f} (coarse smoother {err_smoother:.3f})")
-    out = run_form(p, T, 1, args.compare_steps, 1, not args.no_graphs, profile=False, coarse_net=net)
-    out.update(train_s=t_train, coarse_error=err_net, smoother_error=err_smoother, samples=len(train_set))
+    a = c.general_stokes_solver()
+    torch.cuda.synchronize()
+    ta = time.perf_counter() - t0
+    Ua, Pa = c.download("U"), c.download("P")
+    c.set_option("warm_start", 0)
+    c.upload("T", T)
+    c.thermal_buoyancy()
+    c.upload("U", U0)
+    c.upload("P", P0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b = c.general_stokes_solver()
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    Ub, Pb = c.download("U"), c.download("P")
+    rel = lambda x, y: float(np.linalg.norm(x - y) / np.linalg.norm(y))  # noqa: E731
+    out = {"form_guess": {"uzawa_iterations": a["iterations"], "mg_calls": a["mg_cycles"], "s": ta,
+                          "dpressure": a["dpressure"], "dvelocity": a["dvelocity"]},
+           "previous_solution": {"uzawa_iterations": b["iterations"], "mg_calls": b["mg_cycles"], "s": tb,
+                                 "dpressure": b["dpressure"], "dvelocity": b["dvelocity"]},
+           "U_rel_diff": rel(Ua, Ub), "P_rel_diff": rel(Pa - Pa.mean(), Pb - Pb.mean())}
+    log(f"[{form}] same state, form's guess: Uzawa {a['iterations']} in {ta:.2f} s (dp/p {a['dpressure']:.1e}); "
+        f"previous solution: Uzawa {b['iterations']} in {tb:.2f} s (dp/p {b['dpressure']:.1e}); "
+        f"U differs by {out['U_rel_diff']:.1e}, P by {out['P_rel_diff']:.1e}")
     return out
+
+HEADLINE = "mixed16+extrap"
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="tdep128")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--form", choices=("mixed", "plain", "mixed16"), default="mixed",
-                    help="mixed: FP32 multigrid inside FP64 refinement; plain: the reference's FP64 arithmetic; "
-                         "mixed16: mixed with the fine levels' stiffness stored in FP16, scaled by BI")
-    ap.add_argument("--compare-steps", type=int, default=2,
-                    help="timed steps of the plain FP64 form run beside the mixed form (0: skip)")
+    ap.add_argument("--form", choices=tuple(FORMS), default=HEADLINE, help="the form whose time is the value")
+    ap.add_argument("--compare", default="plain,mixed16",
+                    help="forms timed beside it on the same window (comma-separated; '' for none)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="launch multigrid kernels one by one")
     ap.add_argument("--piterations", type=int, default=None,
@@ -347,14 +354,11 @@ def main():
     nox, noy, noz = p.dims()
     log(f"config {args.config}: {nox - 1}x{noy - 1}x{noz - 1} elements, {p.levels} levels")
     T = deck.seeded_temperature(p, seed=args.seed) if p.tdepv else deck.perturbed_temperature(p)
-    mixed = {"plain": 0, "mixed": 1, "mixed16": 2}[args.form]
-    main_run = run_form(p, T, mixed, args.steps, args.warmup, not args.no_graphs, profile=True)
-    plain = fp16 = learned = None
-    if mixed and args.compare_steps > 0:
-        plain = run_form(p, T, 0, args.compare_steps, 1, not args.no_graphs, profile=False)
-        if mixed == 1:
-            fp16 = run_form(p, T, 2, args.compare_steps, 1, not args.no_graphs, profile=False)
-            learned = learned_form(p, T, args)
+    graphs = not args.no_graphs
+    main_run = run_form(p, T, args.form, args.steps, args.warmup, graphs, profile=True, same_state=True)
+    others = {}
+    for f in [f for f in args.compare.split(",") if f and f != args.form]:
+        others[f] = run_form(p, T, f, args.steps, args.warmup, graphs)
 
     # known-size copy (256 MiB read + 256 MiB written): the reference point that
     # scales hardware byte counters (FETCH_SIZE / WRITE_SIZE) into HBM bytes
@@ -391,48 +395,40 @@ def main():
     log(f"gs_colour: {n_gs} launches, {ms_gs:.1f} ms total, {achieved:.0f} GB/s; "
         f"matvec: {n_mv} launches, {ms_mv:.1f} ms, {(bytes_mv / max(ms_mv, 1e-9)) / 1e6:.0f} GB/s")
     s_per_step = main_run["s_per_step"]
-    # <vector type, stiffness type>; mixed16 runs the FP16 variant on the two
-    # finest levels and the FP32 one below, both averaged here
-    kernel = {0: "k_gs_colour<double, float>", 1: "k_gs_colour<float, float>", 2: "k_gs_colour<float, _Float16>"}[mixed]
+    mixed = FORMS[args.form][0]
+    # <vector type, stiffness type>; the FP16 variant runs on the two finest
+    # levels and the FP32 one below, both averaged here
+    kernel = {0: "k_gs_colour<double, float>", 1: "k_gs_colour<float, float>",
+              2: "k_gs_colour<float, _Float16> (two finest levels) and <float, float>"}[mixed]
     visc = (f"viscosity exp(-{p.viscE[0]:.4g} T)" + (f", {p.num_mat} layers (visc0 {p.visc0[:p.num_mat]})"
                                                       if p.num_mat > 1 else "")) if p.tdepv else "isoviscous"
+
+    def summary(name, r):
+        return {"form": FORMS[name][2], "s_per_step": r["s_per_step"],
+                "uzawa_iterations_per_step": r["uzawa_iterations_per_step"],
+                "uzawa_iterations": r["uzawa_iterations"], "mg_calls_per_step": r["mg_calls_per_step"],
+                "steps_stopped_at_piterations_cap": r["steps_stopped_at_cap"], "Nu_top_last": r["Nu_top"],
+                "vrms_last": r["vrms"], "model_time_last": r["elapsed_time"]}
+
+    forms = {args.form: summary(args.form, main_run)}
+    for f, r in others.items():
+        forms[f] = summary(f, r)
+        if f != "plain":
+            continue
+        # the forms run their own trajectories from the same seeded state; how
+        # far apart they are after the same number of steps
+        forms[args.form]["T_rel_diff_vs_plain_last"] = float(
+            np.linalg.norm(main_run["T_final"] - r["T_final"]) / np.linalg.norm(r["T_final"]))
     config = {"workload": f"{args.config}: {nox - 1}x{noy - 1}x{noz - 1} elements, {p.levels} MG levels, "
-                          f"Ra={p.rayleigh:g}, {visc}, accuracy={p.accuracy:g}",
-              "form": {1: "FP32 multigrid cycles inside FP64 iterative refinement (FP64 residual, Uzawa CG, "
-                          "coarsest level); FP32 stiffness as in the reference",
-                       2: "FP32 multigrid cycles inside FP64 iterative refinement; the two finest levels' "
-                          "stiffness rows scaled by BI and stored in FP16 for the cycles (the FP64 residual uses "
-                          "the reference's FP32 stiffness)",
-                       0: "plain: FP64 vectors, the reference's FP32 stiffness"}[mixed],
-              "uzawa_iterations_per_step": main_run["uzawa_iterations_per_step"],
-              "mg_calls_per_step": main_run["mg_calls_per_step"]}
-    if plain is not None:
-        config["plain_fp64_s_per_step"] = plain["s_per_step"]
-        config["speedup_vs_plain_fp64"] = plain["s_per_step"] / s_per_step
-        config["learned_warm_start"] = ("citcom_amd.learn (BF16 MFMA 3-D CNN) is the Uzawa initial guess where no "
-                                        "previous solution exists; in this time loop the previous step's U, P are "
-                                        "the initial guess, as in the reference, so it is not used here")
-        log(f"plain FP64 form: {plain['s_per_step']:.3f} s/step; {args.form} form {s_per_step:.3f} s/step")
-    if fp16 is not None:
-        config["mixed16_s_per_step"] = fp16["s_per_step"]
-        config["mixed16_speedup_vs_plain_fp64"] = plain["s_per_step"] / fp16["s_per_step"]
-        config["mixed16_form"] = ("the same cycles with the two finest levels' stiffness rows scaled by BI and "
-                                  "stored in FP16 (FP32 arithmetic; the FP64 residual keeps the reference's FP32 "
-                                  "stiffness); same Uzawa iterations and multigrid calls")
-        log(f"mixed16 form: {fp16['s_per_step']:.3f} s/step")
-    if learned is not None:
-        config["mixed_learned_s_per_step"] = learned["s_per_step"]
-        config["mixed_learned_speedup_vs_plain_fp64"] = plain["s_per_step"] / learned["s_per_step"]
-        config["mixed_learned_mg_calls_per_step"] = learned["mg_calls_per_step"]
-        config["learned_coarse_correction"] = (
-            f"FP32 cycles with a BF16-MFMA 3-D CNN in place of the coarsest level's smoother call in every "
-            f"V-cycle, trained in {learned['train_s']:.0f} s (inside this run, untimed) on "
-            f"{learned['samples']} exact coarse solves of the workload family; held-out error "
-            f"{learned['coarse_error']:.3f} against {learned['smoother_error']:.3f} for the coarse smoother; "
-            f"{'in use to the end' if learned['coarse_net_in_use'] else 'switched off by the plain-multigrid fallback'} "
-            f"({learned['fallbacks']} inner solve(s) restarted without it); "
-            f"{learned['s_per_step'] / s_per_step:.3f}x the mixed form's time per step")
-        log(f"mixed+learned form: {learned['s_per_step']:.3f} s/step, {learned['mg_calls_per_step']:.0f} MG calls")
+                          f"Ra={p.rayleigh:g}, {visc}, accuracy={p.accuracy:g}, piterations={p.p_iterations}",
+              "form": args.form, "window": f"steps {args.warmup + 1}..{args.warmup + args.steps} of each form's own "
+                                           f"trajectory from the same seeded state (step-0 solve and "
+                                           f"{args.warmup} warmup steps untimed), every form on the same window",
+              "forms": forms, "same_state_check": main_run["same_state"]}
+    for f, r in others.items():
+        config[f"speedup_vs_{f}"] = r["s_per_step"] / s_per_step
+        log(f"{f}: {r['s_per_step']:.3f} s/step, {r['uzawa_iterations_per_step']:.1f} Uzawa iterations; "
+            f"{args.form}: {s_per_step:.3f} s/step ({r['s_per_step'] / s_per_step:.2f}x)")
     finest = {"achieved": ach_f, "frac": ach_f / PEAK_HBM_GBS, "launches": n_f,
               "avg_launch_ms": ms_f / max(n_f, 1), "algorithmic_bytes_per_launch": bytes_f / max(n_f, 1),
               "traffic": measured_traffic(kernel, args.config, "finest_level_hbm_bytes_per_launch")}
@@ -456,7 +452,8 @@ def main():
     }
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(args.config, args.seed, main_run["mg_calls_per_step"])
+        mg_plain = others["plain"]["mg_calls_per_step"] if "plain" in others else main_run["mg_calls_per_step"]
+        out["cpu_baseline"] = cpu_baseline(args.config, args.seed, mg_plain)
     print(json.dumps(out), flush=True)
 
 

@@ -9,7 +9,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libcitcom_hip.so")
-SOURCES = ["chip_setup.cpp", "chip_kernels.hip", "chip_api.hip", "learn_kernels.hip"]
+SOURCES = ["chip_setup.cpp", "chip_kernels.hip", "chip_api.hip", "learn_kernels.hip", "unet_kernels.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=off keeps the reference's rounding points (no fused
 # multiply-add), which the bit-for-bit parity of the operators relies on.

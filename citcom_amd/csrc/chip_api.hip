@@ -13,6 +13,7 @@
 #include <string>
 
 #include "chip_kernels.h"
+#include "../../include/citcom_learn.h"
 
 namespace chip {
 
@@ -30,6 +31,7 @@ enum : int {
     D_VV = 9,      // [9] V.V, [10] u1.u1
     D_PP = 11,     // [11] P.P, [12] Ah.Ah, [13] s2.s2
     D_INIT = 14,   // [14] r1.r1, [15] V.V
+    D_WS = 16,     // [16..19] warm-start feature scales
     D_HF = 24,     // [24..27] heat-flux sums
 };
 
@@ -471,6 +473,7 @@ int pg_timestep(chip_ctx *c, chip_pg_stats *st) {
         }
     } while (iredo == 1 && last_sub <= 5);
     c->elapsed_time += timestep;
+    c->t_now += timestep;
     conform_temperature(q, c, c->T);
     thermal_buoyancy(q, c, c->T, c->buoy);
     CHIP_CHECK(hipGetLastError());
@@ -493,18 +496,102 @@ int heat_flux_nu(chip_ctx *c, float *nu) {
     return 0;
 }
 
+// the initial guess of the Uzawa solve: the previous solution (the
+// reference, Stokes_flow_Incomp.c:354-366 start from E->U, E->P as the last
+// solve left them) or, with the warm start, that solution extrapolated in
+// time from the last two: x + r (x - x_prev), r = dt_now / dt_prev; with
+// warm_start = 2 the U-Net's correction of that extrapolation is added.
+// When training samples are recorded (or the network runs) the network's
+// input is formed here and the target after the solve (warm_finish).
+int warm_start_guess(chip_ctx *c) {
+    c->feat_pending = false;
+    if (!c->warm_start || c->hist < 1) return 0;
+    hipStream_t q = S(c);
+    const Level &L = c->L[c->levmax];
+    const long nno = L.g.nno, neq = 3L * nno;
+    const double dt_now = c->t_now - c->t_sol[0], dt_prev = c->t_sol[0] - c->t_sol[1];
+    const double r = (c->hist >= 2 && dt_prev > 0.0 && dt_now > 0.0) ? dt_now / dt_prev : 0.0;
+    extrapolate(q, c->U, c->Uprev, r, neq);
+    extrapolate(q, c->P, c->Pprev, r, L.g.nel);
+    const bool net = c->warm_start == 2 && c->wn_set && c->hist >= 2;
+    const bool rec = c->rec_x && c->rec_y && c->hist >= 2;
+    if (!net && !rec) return 0;
+    // features: the temperature's departure from its own extrapolation, the
+    // nodal pressures, log10 viscosity (EVI is this step's), and their scales
+    warm_dT(q, c->T, c->Th0, c->Th1, (float)r, nno, c->wn_dT);
+    e2n(q, L, c->P, nullptr, c->wn_pn);
+    e2n(q, L, c->P, c->Pprev, c->wn_dpn);
+    cnet_logeta(q, L, c->wn_logeta);
+    dots(q, c, (int)neq, 1, (const double *)c->U, (const double *)c->U, nullptr, nullptr, nullptr, nullptr, nullptr,
+         nullptr, c->dots + D_WS);
+    dots(q, c, (int)nno, 3, (const float *)c->wn_dT, (const float *)c->wn_dT, (const float *)c->wn_pn,
+         (const float *)c->wn_pn, (const float *)c->buoy, (const float *)c->buoy, nullptr, nullptr, c->dots + D_WS + 1);
+    if (sync_dots(c)) return -1;
+    const double *h = c->h_dots + D_WS;
+    const double s_u = std::sqrt(h[0] / neq), s_dT = std::sqrt(h[1] / nno), s_p = std::sqrt(h[2] / nno);
+    const double s_th = std::sqrt(h[3] / nno) / std::max(1e-30, (double)c->p.rayleigh);
+    // the extrapolation's error scales with the relative size of the
+    // temperature's departure (the Stokes response is linear in the buoyancy)
+    const double rel = s_th > 0.0 ? s_dT / s_th : 0.0;
+    const double s_eu = s_u * rel, s_ep = s_p * rel;
+    double *w = c->wscale;
+    w[0] = s_u, w[1] = s_p, w[2] = s_dT, w[3] = s_th, w[4] = s_eu, w[5] = s_ep, w[6] = r, w[7] = 1.0;
+    auto inv = [](double v) { return v > 0.0 ? (float)(1.0 / v) : 0.0f; };
+    warm_features(q, L, c->T, c->wn_dT, c->wn_logeta, c->U, c->Uprev, c->wn_pn, c->wn_dpn, inv(s_dT), inv(s_u),
+                  inv(s_p), c->wn_x);
+    copy(q, c->Ue, c->U, neq);
+    copy(q, c->Pe, c->P, L.g.nel);
+    if (rec) {
+        CHIP_CHECK(hipMemcpyAsync(c->rec_x, c->wn_x, 16 * (size_t)nno * sizeof(uint16_t), hipMemcpyDeviceToDevice, q));
+        c->feat_pending = s_eu > 0.0 && s_ep > 0.0;
+    }
+    if (net && s_eu > 0.0 && s_ep > 0.0) {
+        if (cl_unet_forward(c->wn_x, 1, L.g.nox, L.g.noy, L.g.noz, c->wn_w, c->wn_b, c->wn_out, c->wn_ws, q)) {
+            set_error("warm start: the U-Net's forward pass failed to launch");
+            return -1;
+        }
+        warm_apply(q, L, c->wn_out, s_eu, s_ep, c->U, c->P);
+    }
+    CHIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+// after the solve: the training target of a recorded sample, then the
+// solve joins the history
+int warm_finish(chip_ctx *c) {
+    if (!c->warm_start) return 0;
+    hipStream_t q = S(c);
+    const Level &L = c->L[c->levmax];
+    if (c->feat_pending) {
+        e2n(q, L, c->P, c->Pe, c->wn_dpn);
+        warm_target(q, L, c->U, c->Ue, c->wn_dpn, 1.0 / c->wscale[4], 1.0 / c->wscale[5], c->rec_y);
+        c->feat_pending = false;
+    }
+    fcopy(q, c->Th1, c->Th0, L.g.nno);
+    fcopy(q, c->Th0, c->T, L.g.nno);
+    c->t_sol[1] = c->t_sol[0];
+    c->t_sol[0] = c->t_now;
+    c->hist++;
+    CHIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 int stokes_step(chip_ctx *c, chip_stokes_stats *st) {
     hipStream_t q = S(c);
     const Level &L = c->L[c->levmax];
+    // get_system_viscosity + construct_stiffness_B_matrix (Drive_solvers.c:123-126);
+    // the viscosity first, so that the warm start reads this step's
+    const bool update = !c->built || c->p.visc_update;
+    if (update) visc_from_T(q, c, c->T, c->L[c->levmax].EVI);
+    if (warm_start_guess(c)) return -1;
     conform_velocity(q, L, c->U);
     assemble_forces(q, c, c->buoy, c->F);
-    // get_system_viscosity + construct_stiffness_B_matrix (Drive_solvers.c:123-126)
-    if (!c->built || c->p.visc_update) {
+    if (update) {
         c->built = true;
-        visc_from_T(q, c, c->T, c->L[c->levmax].EVI);
         if (rebuild(c)) return -1;
     }
-    return uzawa(c, st);
+    if (uzawa(c, st)) return -1;
+    return warm_finish(c);
 }
 
 }  // namespace
@@ -945,6 +1032,47 @@ int chip_coarse_net_state(chip_ctx *c, int *on, int *fallbacks) {
     return 0;
 }
 
+int chip_set_warm_net(chip_ctx *c, const uint16_t *weights, long n_weights, const float *bias, long n_bias) {
+    CHIP_NEED_CTX(c);
+    long nw = 0, nb = 0;
+    cl_unet_params(&nw, &nb);
+    if (!weights || !bias || n_weights != nw || n_bias != nb) {
+        set_error("chip_set_warm_net: the U-Net takes " + std::to_string(nw) + " weights and " + std::to_string(nb) +
+                  " biases");
+        return -1;
+    }
+    const Geo &g = c->L[c->levmax].g;
+    for (int n : {g.nox, g.noy, g.noz})
+        if (n < 17 || (n - 1) % 8) {
+            set_error("chip_set_warm_net: the finest grid needs n - 1 divisible by 8 and n >= 17 nodes per axis");
+            return -1;
+        }
+    if (alloc_warm_start(c) || alloc_warm_net(c)) return -1;
+    CHIP_CHECK(hipMemcpyAsync(c->wn_w, weights, (size_t)nw * sizeof(uint16_t), hipMemcpyHostToDevice, c->stream));
+    CHIP_CHECK(hipMemcpyAsync(c->wn_b, bias, (size_t)nb * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    CHIP_CHECK(hipStreamSynchronize(c->stream));
+    c->wn_set = true;
+    return 0;
+}
+
+int chip_warm_record(chip_ctx *c, void *features, void *targets) {
+    CHIP_NEED_CTX(c);
+    if ((features == nullptr) != (targets == nullptr)) {
+        set_error("chip_warm_record: give both buffers or neither");
+        return -1;
+    }
+    if (features && alloc_warm_start(c)) return -1;
+    c->rec_x = static_cast<uint16_t *>(features);
+    c->rec_y = static_cast<float *>(targets);
+    return 0;
+}
+
+int chip_warm_scales(chip_ctx *c, double *out) {
+    CHIP_NEED_CTX(c);
+    for (int i = 0; i < 8; i++) out[i] = c->wscale[i];
+    return 0;
+}
+
 int chip_set_option(chip_ctx *c, const char *name, int value) {
     CHIP_NEED_CTX(c);
     const std::string n(name ? name : "");
@@ -958,6 +1086,21 @@ int chip_set_option(chip_ctx *c, const char *name, int value) {
     }
     if (n == "print_convergence") {
         c->print_convergence = value != 0;
+        return 0;
+    }
+    if (n == "warm_start") {   // a new history starts with the next solve
+        if (value < 0 || value > 2) {
+            set_error("chip_set_option: warm_start is 0 (previous solution), 1 (extrapolated) or 2 (extrapolated "
+                      "and corrected by the learned network)");
+            return -1;
+        }
+        if (value == 2 && !c->wn_set) {
+            set_error("chip_set_option: warm_start 2 needs chip_set_warm_net first");
+            return -1;
+        }
+        if (value && alloc_warm_start(c)) return -1;
+        c->warm_start = value;
+        c->hist = 0;
         return 0;
     }
     if (n == "coarse_net" && value && !c->cnet_layers) {

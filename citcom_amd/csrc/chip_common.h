@@ -133,6 +133,30 @@ struct chip_ctx {
     int cnet_fallbacks = 0;     // inner solves restarted without it (no residual reduction)
     uint16_t *cnet_w = nullptr;
     float *cnet_b = nullptr, *cnet_ys = nullptr, *cnet_logeta = nullptr;
+    // warm start of the Uzawa solve (option "warm_start"): 0 the previous
+    // solution as it is (the reference); 1 extrapolated in time from the last
+    // two solutions; 2 the same plus the learned correction (chip_set_warm_net)
+    int warm_start = 0;
+    int hist = 0;                 // solutions kept since the option was set
+    double t_now = 0.0;           // model time, the sum of the PG timesteps (FP64)
+    double t_sol[2] = {0.0, 0.0}; // times of the last two solves
+    double *Uprev = nullptr, *Pprev = nullptr;  // the solution before the last
+    float *Th0 = nullptr, *Th1 = nullptr;       // T at the last two solves
+    // learned correction of the extrapolated guess (chip_set_warm_net): the
+    // U-Net's BF16 weights and FP32 biases, its input [nno][16] and output
+    // [nno][16], its workspace, and the finest-level scratch of the features
+    uint16_t *wn_w = nullptr, *wn_x = nullptr;
+    float *wn_b = nullptr, *wn_out = nullptr;
+    void *wn_ws = nullptr;
+    float *wn_dT = nullptr, *wn_pn = nullptr, *wn_dpn = nullptr, *wn_logeta = nullptr;
+    double *Ue = nullptr, *Pe = nullptr;        // the extrapolated guess of the current solve
+    bool wn_set = false;
+    // training samples (chip_warm_record): the features and the target of
+    // each solve that has two earlier ones, into caller device buffers
+    uint16_t *rec_x = nullptr;
+    float *rec_y = nullptr;
+    bool feat_pending = false;   // this solve's features were formed; its target follows the solve
+    double wscale[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // s_u, s_p, s_dT, s_theta, s_eu, s_ep, r, formed
     bool print_convergence = false;  // per-iteration lines on stderr, as E->control.print_convergence
     hipGraphExec_t mg_graph = nullptr;
     chip::Prof prof;
@@ -146,6 +170,10 @@ int build_mesh(chip_ctx *c);
 // device buffers of the learned coarse-level correction (once per context)
 constexpr int CNET_MAX_LAYERS = 8;
 int alloc_coarse_net(chip_ctx *c);
+// history buffers of the warm start (once per context)
+int alloc_warm_start(chip_ctx *c);
+// the learned correction's buffers (once per context)
+int alloc_warm_net(chip_ctx *c);
 // errors
 void set_error(const std::string &s);
 }  // namespace chip

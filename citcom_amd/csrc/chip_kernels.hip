@@ -1306,6 +1306,19 @@ __global__ void k_axpy(double *y, const double *x, double a, long n) {
 }
 void axpy_scalar(hipStream_t s, double *y, const double *x, double a, long n) { hipLaunchKernelGGL(k_axpy, dim3(nblk(n)), dim3(TPB), 0, s, y, x, a, n); }
 
+// the warm start's extrapolation in time: x <- x + r (x - xprev), then
+// xprev <- the old x (r = 0: xprev <- x, x unchanged)
+__global__ void k_extrapolate(double *x, double *xprev, double r, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double xn = x[i];
+    x[i] = xn + r * (xn - xprev[i]);
+    xprev[i] = xn;
+}
+void extrapolate(hipStream_t s, double *x, double *xprev, double r, long n) {
+    hipLaunchKernelGGL(k_extrapolate, dim3(nblk(n)), dim3(TPB), 0, s, x, xprev, r, n);
+}
+
 template <typename V>
 __global__ void k_sub(V *r, const V *a, const V *b, long n) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2360,5 +2373,127 @@ int cnet_apply(hipStream_t s, const chip_ctx *c, const Level &L, const V *F, V *
 }
 template int cnet_apply<double>(hipStream_t, const chip_ctx *, const Level &, const double *, double *);
 template int cnet_apply<float>(hipStream_t, const chip_ctx *, const Level &, const float *, float *);
+
+}  // namespace chip
+
+// ---------------------------------------------------------------------------
+// the learned warm start's inputs and outputs on the finest level (no
+// counterpart in the reference: the network's output is only the Uzawa
+// solve's initial guess)
+
+namespace chip {
+
+// dT = T - (Th0 + r (Th0 - Th1)): the temperature's departure from its own
+// extrapolation in time over the last two solves
+__global__ void k_warm_dT(const float *__restrict__ T, const float *__restrict__ Th0, const float *__restrict__ Th1,
+                          float r, long n, float *__restrict__ dT) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dT[i] = T[i] - (Th0[i] + r * (Th0[i] - Th1[i]));
+}
+void warm_dT(hipStream_t s, const float *T, const float *Th0, const float *Th1, float r, long n, float *dT) {
+    hipLaunchKernelGGL(k_warm_dT, dim3(nblk(n)), dim3(TPB), 0, s, T, Th0, Th1, r, n, dT);
+}
+
+// nodal value of an element field: the mean of (Pa - Pb) over the node's
+// elements (Pb optional)
+__global__ void k_e2n(Geo g, const double *__restrict__ Pa, const double *__restrict__ Pb, float *__restrict__ out) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nno) return;
+    const int k = n % g.noz, i = (n / g.noz) % g.nox, j = n / (g.noz * g.nox);
+    double acc = 0.0;
+    int cnt = 0;
+    for (int dj = -1; dj <= 0; dj++)
+        for (int di = -1; di <= 0; di++)
+            for (int dk = -1; dk <= 0; dk++) {
+                const int ez = k + dk, ex = i + di, ey = j + dj;
+                if (ez < 0 || ex < 0 || ey < 0 || ez >= g.elz || ex >= g.elx || ey >= g.ely) continue;
+                const size_t e = (size_t)ez + (size_t)g.elz * (ex + (size_t)g.elx * ey);
+                acc += Pb ? Pa[e] - Pb[e] : Pa[e];
+                cnt++;
+            }
+    out[n] = (float)(acc / cnt);
+}
+void e2n(hipStream_t s, const Level &L, const double *Pa, const double *Pb, float *out) {
+    hipLaunchKernelGGL(k_e2n, dim3(nblk(L.g.nno)), dim3(TPB), 0, s, L.g, Pa, Pb, out);
+}
+
+__device__ __forceinline__ uint16_t warm_bf16(float f) {
+    const unsigned u = __float_as_uint(f);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+// the network's input, 16 bf16 channels per node: T - 1/2, dT / s_dT,
+// log10(eta) / 2.5, U / s_u (3), nodal P / s_p, (U - U_prev) / s_u (3),
+// nodal (P - P_prev) / s_p, z (node index over noz - 1), zeros.  U, P are
+// the extrapolated guess, U_prev, P_prev the last solution.
+__global__ void k_warm_features(Geo g, const float *__restrict__ T, const float *__restrict__ dT,
+                                const float *__restrict__ logeta, const double *__restrict__ U,
+                                const double *__restrict__ Uprev, const float *__restrict__ pn,
+                                const float *__restrict__ dpn, float inv_dT, float inv_u, float inv_p,
+                                uint16_t *__restrict__ x) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nno) return;
+    float f[16];
+    f[0] = T[n] - 0.5f;
+    f[1] = dT[n] * inv_dT;
+    f[2] = logeta[n] * 0.4f;
+    for (int d = 0; d < 3; d++) {
+        f[3 + d] = (float)U[3 * (size_t)n + d] * inv_u;
+        f[7 + d] = (float)(U[3 * (size_t)n + d] - Uprev[3 * (size_t)n + d]) * inv_u;
+    }
+    f[6] = pn[n] * inv_p;
+    f[10] = dpn[n] * inv_p;
+    f[11] = (float)(n % g.noz) / (float)(g.noz - 1);
+    for (int q = 12; q < 16; q++) f[q] = 0.f;
+    uint4 *o = reinterpret_cast<uint4 *>(x + (size_t)n * 16);
+    for (int q = 0; q < 2; q++) {
+        uint4 w;
+        w.x = warm_bf16(f[8 * q + 0]) | ((uint32_t)warm_bf16(f[8 * q + 1]) << 16);
+        w.y = warm_bf16(f[8 * q + 2]) | ((uint32_t)warm_bf16(f[8 * q + 3]) << 16);
+        w.z = warm_bf16(f[8 * q + 4]) | ((uint32_t)warm_bf16(f[8 * q + 5]) << 16);
+        w.w = warm_bf16(f[8 * q + 6]) | ((uint32_t)warm_bf16(f[8 * q + 7]) << 16);
+        o[q] = w;
+    }
+}
+void warm_features(hipStream_t s, const Level &L, const float *T, const float *dT, const float *logeta,
+                   const double *U, const double *Uprev, const float *pn, const float *dpn, float inv_dT, float inv_u,
+                   float inv_p, uint16_t *x) {
+    hipLaunchKernelGGL(k_warm_features, dim3(nblk(L.g.nno)), dim3(TPB), 0, s, L.g, T, dT, logeta, U, Uprev, pn, dpn,
+                       inv_dT, inv_u, inv_p, x);
+}
+
+// the network's correction: U += s_eu out[n][0..2] at every node; P[e] +=
+// s_ep (mean of out[.][3] over the element's eight nodes)
+__global__ void k_warm_apply(Geo g, const float *__restrict__ out, double s_eu, double s_ep, double *__restrict__ U,
+                             double *__restrict__ P) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 3L * g.nno) U[i] += s_eu * (double)out[(i / 3) * 16 + i % 3];
+    if (i < g.nel) {
+        const int ez = (int)(i % g.elz), ex = (int)((i / g.elz) % g.elx), ey = (int)(i / ((long)g.elz * g.elx));
+        double m = 0.0;
+        for (int q = 0; q < 8; q++) {
+            const int k = ez + (q & 1), ii = ex + ((q >> 1) & 1), j = ey + (q >> 2);
+            m += (double)out[((size_t)k + (size_t)g.noz * (ii + (size_t)g.nox * j)) * 16 + 3];
+        }
+        P[i] += s_ep * 0.125 * m;
+    }
+}
+void warm_apply(hipStream_t s, const Level &L, const float *out, double s_eu, double s_ep, double *U, double *P) {
+    hipLaunchKernelGGL(k_warm_apply, dim3(nblk(3L * L.g.nno)), dim3(TPB), 0, s, L.g, out, s_eu, s_ep, U, P);
+}
+
+// the training target of one solve: (U - Ue) / s_eu per node (3 channels)
+// and the nodal mean of (P - Pe) / s_ep, Ue, Pe the extrapolated guess
+__global__ void k_warm_target(Geo g, const double *__restrict__ U, const double *__restrict__ Ue,
+                              const float *__restrict__ dpn, double inv_eu, double inv_ep, float *__restrict__ y) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nno) return;
+    for (int d = 0; d < 3; d++) y[4 * (size_t)n + d] = (float)((U[3 * (size_t)n + d] - Ue[3 * (size_t)n + d]) * inv_eu);
+    y[4 * (size_t)n + 3] = (float)(dpn[n] * inv_ep);
+}
+void warm_target(hipStream_t s, const Level &L, const double *U, const double *Ue, const float *dpn, double inv_eu,
+                 double inv_ep, float *y) {
+    hipLaunchKernelGGL(k_warm_target, dim3(nblk(L.g.nno)), dim3(TPB), 0, s, L.g, U, Ue, dpn, inv_eu, inv_ep, y);
+}
 
 }  // namespace chip

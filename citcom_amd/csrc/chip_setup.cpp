@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "chip_common.h"
+#include "../../include/citcom_learn.h"
 #include "chip_kernels.h"
 
 namespace chip {
@@ -256,6 +257,31 @@ int alloc_coarse_net(chip_ctx *c) {
         dev_alloc(c, &c->cnet_b, (size_t)CNET_MAX_LAYERS * 16) || dev_alloc(c, &c->cnet_ys, 3) ||
         dev_alloc(c, &c->cnet_logeta, nno))
         return -1;
+    return 0;
+}
+
+int alloc_warm_start(chip_ctx *c) {
+    if (c->Uprev) return 0;
+    const size_t nno = (size_t)c->L[c->levmax].g.nno, nel = (size_t)c->L[c->levmax].g.nel;
+    return (dev_alloc(c, &c->Uprev, 3 * nno) || dev_alloc(c, &c->Pprev, nel) || dev_alloc(c, &c->Th0, nno) ||
+            dev_alloc(c, &c->Th1, nno) || dev_alloc(c, &c->wn_dT, nno) || dev_alloc(c, &c->wn_pn, nno) ||
+            dev_alloc(c, &c->wn_dpn, nno) || dev_alloc(c, &c->wn_logeta, nno) || dev_alloc(c, &c->wn_x, 16 * nno) ||
+            dev_alloc(c, &c->Ue, 3 * nno) || dev_alloc(c, &c->Pe, nel))
+               ? -1
+               : 0;
+}
+
+int alloc_warm_net(chip_ctx *c) {
+    if (c->wn_w) return 0;
+    const Geo &g = c->L[c->levmax].g;
+    long nw = 0, nb = 0;
+    cl_unet_params(&nw, &nb);
+    const long ws = cl_unet_workspace(1, g.nox, g.noy, g.noz);
+    char *wsp = nullptr;
+    if (dev_alloc(c, &c->wn_w, (size_t)nw) || dev_alloc(c, &c->wn_b, (size_t)nb) ||
+        dev_alloc(c, &c->wn_out, 16 * (size_t)g.nno) || dev_alloc(c, &wsp, (size_t)ws))
+        return -1;
+    c->wn_ws = wsp;
     return 0;
 }
 

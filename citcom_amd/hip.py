@@ -112,6 +112,9 @@ def lib():
         L.chip_set_option.argtypes = [P, C.c_char_p, C.c_int]
         L.chip_set_coarse_net.argtypes = [P, C.c_int, P, P, P]
         L.chip_coarse_net_state.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.chip_set_warm_net.argtypes = [P, P, C.c_long, P, C.c_long]
+        L.chip_warm_record.argtypes = [P, P, P]
+        L.chip_warm_scales.argtypes = [P, P]
         L.chip_device_bytes.restype = C.c_long
         L.chip_device_bytes.argtypes = [P]
         _lib = L
@@ -317,6 +320,23 @@ class Citcom:
             raise ChipError("set_coarse_net: weights [layers][16][27][16], bias [layers][16], out_scale [3]")
         _check(lib().chip_set_coarse_net(self.h, w.shape[0], w.ctypes.data, b.ctypes.data, ys.ctypes.data),
                "set_coarse_net")
+
+    def set_warm_net(self, weights_bf16: np.ndarray, bias: np.ndarray) -> None:
+        """the learned warm start's U-Net (chip_set_warm_net): BF16 bits of the
+        weights in layer order, FP32 biases"""
+        w = np.ascontiguousarray(weights_bf16, dtype=np.uint16)
+        b = np.ascontiguousarray(bias, dtype=np.float32)
+        _check(lib().chip_set_warm_net(self.h, w.ctypes.data, w.size, b.ctypes.data, b.size), "set_warm_net")
+
+    def warm_record(self, features_ptr: int | None, targets_ptr: int | None) -> None:
+        """record training samples into device buffers (chip_warm_record)"""
+        _check(lib().chip_warm_record(self.h, C.c_void_p(features_ptr or None), C.c_void_p(targets_ptr or None)),
+               "warm_record")
+
+    def warm_scales(self) -> np.ndarray:
+        out = np.zeros(8)
+        _check(lib().chip_warm_scales(self.h, out.ctypes.data), "warm_scales")
+        return out
 
     def device_bytes(self) -> int:
         return lib().chip_device_bytes(self.h)

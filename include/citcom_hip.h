@@ -186,6 +186,14 @@ long chip_device_bytes(chip_ctx *c);
  * launches, -1 (default) profiles every level.
  * "coarse_net": 1 replaces the coarsest level's smoother call in multigrid
  * by the learned correction (chip_set_coarse_net), 0 (default) does not.
+ * "warm_start": 0 (default) starts each Uzawa solve from the previous
+ * solution as the last solve left it, as the reference does
+ * (Stokes_flow_Incomp.c:354-366 start from E->U, E->P); 1 starts it from
+ * the last two solutions extrapolated in time, U + r (U - U_prev) and
+ * P + r (P - P_prev) with r = dt_now / dt_prev (the PG timesteps since the
+ * two solves), once two solves have been kept; 2 adds the learned U-Net's
+ * correction of that extrapolation (chip_set_warm_net).  Setting the option
+ * starts a new history.  The solve still iterates to the same criterion.
  * "print_convergence": 1 prints one line per Uzawa iteration and per inner
  * solve on stderr (generate_log_message, Stokes_flow_Incomp.c:501-526;
  * General_matrix_functions.c:494-498). */
@@ -210,6 +218,32 @@ int chip_set_coarse_net(chip_ctx *c, int layers, const uint16_t *weights, const 
  * the coarsest level's smoother.  on: whether it is still in use;
  * fallbacks: how many times that happened. */
 int chip_coarse_net_state(chip_ctx *c, int *on, int *fallbacks);
+
+/* ---- learned warm start (no counterpart in the reference) ----------------
+ * The Uzawa solve's initial guess with the option "warm_start" = 2: the last
+ * two solutions extrapolated in time, corrected by a U-Net (BF16 MFMA
+ * convolutions, FP32 accumulation; cl_unet_forward of citcom_learn.h) on the
+ * finest level's nodes.  Its input per node, 16 bf16 channels: T - 1/2,
+ * dT / s_dT with dT = T - (T1 + r (T1 - T2)) (T1, T2 the temperatures of the
+ * last two solves), log10(eta) / 2.5 (Gauss-point mean of the adjacent
+ * elements, averaged), the extrapolated U / s_u (3), its nodal P / s_p
+ * (mean of the adjacent elements), the extrapolation's increments of U and
+ * P over the same scales (4), z, zeros; s_u, s_p, s_dT the RMS values.  Its
+ * output channels 0..2 times s_u s_dT / s_theta are added to U at every
+ * node and channel 3 times s_p s_dT / s_theta, averaged over an element's
+ * eight nodes, to its P; s_theta = RMS(buoyancy) / Ra.  The solve then runs
+ * to its own criterion, so accuracy is not traded.  Weights: bf16
+ * [594432] in layer order ([co][27][ci] each), biases FP32 [528]. */
+int chip_set_warm_net(chip_ctx *c, const uint16_t *weights, long n_weights, const float *bias, long n_bias);
+/* Training samples: with the warm start on, every solve that has two earlier
+ * ones writes the network's input (bf16 [nno][16]) to features and, after
+ * the solve, the target (FP32 [nno][4]: (U - U_extrapolated) / (s_u s_dT /
+ * s_theta) per node, the nodal mean of (P - P_extrapolated) / (s_p s_dT /
+ * s_theta)) to targets; device pointers, both NULL to stop. */
+int chip_warm_record(chip_ctx *c, void *features, void *targets);
+/* the last formed input's scales: s_u, s_p, s_dT, s_theta, output scale of
+ * U, output scale of P, r, 1 (0 before any) */
+int chip_warm_scales(chip_ctx *c, double *out8);
 
 #ifdef __cplusplus
 }

@@ -59,6 +59,63 @@ int cl_adam(float *w, const float *g, float *m, float *v, long n, float lr, floa
             uint16_t *wbf, const unsigned char *mask, void *stream);
 int cl_to_bf16(const float *a, uint16_t *b, long n, void *stream);
 
+/* ---- the warm start's U-Net: convolutions with any channel counts (ci, co
+ * multiples of 16 up to 256), activations bf16 [nb][nx*ny*nz][C] (channels
+ * fastest), weights bf16 [co][27 taps][ci].
+ * y = act(conv(x) + bias + res): bias (FP32 [co]) and res (bf16
+ * [nb][nv][co], the U-Net's skip connection) optional; relu != 0: ReLU;
+ * mirror != 0 reads tap t at offset -off_t (used by the data gradient); yf
+ * (optional) gets the FP32 values. */
+int cl_conv3d_fwd_c(const uint16_t *x, int nb, int nx, int ny, int nz, int ci, int co, const uint16_t *w,
+                    const float *bias, const uint16_t *res, int relu, int mirror, uint16_t *y, float *yf,
+                    void *stream);
+/* wt[c][t][o] = w[o][t][c] */
+int cl_conv3d_transpose_w(const uint16_t *w, int ci, int co, uint16_t *wt, void *stream);
+/* dx = conv^T(dy) with the transposed weights wt of cl_conv3d_transpose_w */
+int cl_conv3d_bwd_data_c(const uint16_t *dy, int nb, int nx, int ny, int nz, int ci, int co, const uint16_t *wt,
+                         uint16_t *dx, void *stream);
+/* grad = [dW (co*27*ci), db (co)] FP32, summed over voxels and the batch in
+ * a fixed order; workspace of cl_conv3d_bwd_weight_c_workspace bytes */
+long cl_conv3d_bwd_weight_c_workspace(int nb, int nx, int ny, int nz, int ci, int co);
+int cl_conv3d_bwd_weight_c(const uint16_t *x, const uint16_t *dy, int nb, int nx, int ny, int nz, int ci, int co,
+                           float *grad, float *workspace, void *stream);
+/* Grid transfers between node-coincident levels of a grid of nx, ny, nz
+ * nodes (all odd, >= 3) and the coarse grid of (n-1)/2+1 nodes per axis,
+ * ch channels: full-weighting restriction coarse[c] = scale * sum_d w(d)
+ * fine[2c+d] (w = product over axes of 1/2 at d = 0, 1/4 at d = +-1;
+ * neighbours outside left out) and trilinear interpolation fine[f] = scale *
+ * sum_c p(f-2c) coarse[c] (p = product of 1 at 0, 1/2 at +-1).  The
+ * restriction's transpose is the interpolation / 8. */
+int cl_restrict(const uint16_t *fine, int nb, int nx, int ny, int nz, int ch, float scale, uint16_t *coarse,
+                void *stream);
+int cl_interp(const uint16_t *coarse, int nb, int nx, int ny, int nz, int ch, float scale, uint16_t *fine,
+              void *stream);
+/* a += b, n bf16 values */
+int cl_add_bf16(uint16_t *a, const uint16_t *b, long n, void *stream);
+/* weighted mean squared error of y (FP32, ystride channels) against target
+ * (FP32 [nvox][nout]) over the first nout channels, channel weights wch
+ * (device FP32 [nout]): dy (bf16, ystride channels) = d loss / d y;
+ * partial[block] = block sums of the weighted squared errors,
+ * ceil(nvox*ystride/256) blocks */
+int cl_wmse(const float *y, int ystride, const float *target, long nvox, int nout, const float *wch, uint16_t *dy,
+            float *partial, void *stream);
+
+/* The warm start's U-Net (fixed architecture): 4 levels of (n-1)/2^l + 1
+ * nodes per axis (n - 1 divisible by 8, n >= 17), 16/32/48/64 channels, 15
+ * convolutions with weights concatenated in layer order, channel pairs
+ * (ci, co) = (16,16) (16,16) | (16,32) (32,32) | (32,48) (48,48) |
+ * (48,64) (64,64) | (64,48)+skip (48,48) | (48,32)+skip (32,32) |
+ * (32,16)+skip (16,16) (16,16); ReLU after all but the last.  Encoder:
+ * two convolutions per level, the second's output kept as that level's
+ * skip, then cl_restrict (scale 1); decoder: cl_interp (scale 1), a
+ * convolution plus the level's skip, a second convolution.  Input x: bf16
+ * [nb][nv][16]; out: FP32 [nb][nv][16] (the last convolution's values).
+ * cl_unet_params gives the weight and bias counts (594432, 528). */
+int cl_unet_params(long *n_weights, long *n_biases);
+long cl_unet_workspace(int nb, int nx, int ny, int nz);
+int cl_unet_forward(const uint16_t *x, int nb, int nx, int ny, int nz, const uint16_t *w, const float *bias,
+                    float *out, void *workspace, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
