@@ -210,8 +210,11 @@ FORMS = {
     "mixed16": (2, 0, "as mixed, the two finest levels' stiffness rows scaled by BI and stored in FP16 for the cycles "
                       "(FP32 arithmetic; the FP64 residual keeps the reference's FP32 stiffness), previous-solution "
                       "start"),
-    "mixed16+extrap": (2, 1, "as mixed16, each Uzawa solve started from the last two solutions extrapolated in time "
-                             "(no network)"),
+    "mixed16+extrap": (2, 1, "as mixed16, each Uzawa solve started from the last two solutions extrapolated along "
+                             "the sequence of solves, 2 x_n - x_(n-1) (no network)"),
+    "mixed16+learned": (2, 2, "as mixed16+extrap, the extrapolation corrected by the learned U-Net (BF16 MFMA, "
+                              "citcom_amd/weights/warm_unet.npz) before each Uzawa solve"),
+    "plain+extrap": (0, 1, "the plain FP64 form started from the extrapolation 2 x_n - x_(n-1) (no network)"),
 }
 
 
@@ -232,7 +235,12 @@ def run_form(p, T, form, steps, warmup, graphs, profile=False, same_state=False)
     c = Citcom(p, mixed=mixed)
     if not graphs:
         c.set_option("graphs", 0)
-    if warm:
+    if warm == 2:
+        from citcom_amd import unet
+        meta = unet.install(c)
+        log(f"[{form}] U-Net of {meta['params']} parameters trained on {meta['samples']} samples "
+            f"({meta['data_s']:.0f} s of solves, {meta['train_s']:.0f} s of training)")
+    elif warm:
         c.set_option("warm_start", warm)
     c.initialize(T)
     log(f"[{form}] setup {time.perf_counter() - t0:.1f} s, device memory {c.device_bytes() / 2**30:.2f} GiB")
@@ -286,44 +294,50 @@ def run_form(p, T, form, steps, warmup, graphs, profile=False, same_state=False)
 
 
 def same_state_solves(c, form):
-    """One more step of this form's trajectory solved twice from the same
-    state: with the form's own initial guess, then (same temperature, same
-    previous solution) from the previous solution as it is, the reference's
-    start, in the same arithmetic."""
+    """Two more steps of this form's trajectory; the second one's solve is
+    repeated from the same state (same temperature, same previous solutions)
+    with other initial guesses: the form's own, the plain extrapolation
+    2 x_n - x_(n-1), and the previous solution as it is (the reference's
+    start), in the same arithmetic.  The form's own solve is the first, so
+    the history it reads is intact."""
     import torch
 
+    Ua, Pa = c.download("U"), c.download("P")
+    c.timestep()
+    Un, Pn = c.download("U"), c.download("P")
     c.PG_timestep()
     T = c.download("T")
-    U0, P0 = c.download("U"), c.download("P")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    a = c.general_stokes_solver()
-    torch.cuda.synchronize()
-    ta = time.perf_counter() - t0
-    Ua, Pa = c.download("U"), c.download("P")
+    out = {}
+
+    def timed(tag):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = c.general_stokes_solver()
+        torch.cuda.synchronize()
+        out[tag] = {"uzawa_iterations": st["iterations"], "mg_calls": st["mg_cycles"], "s": time.perf_counter() - t0,
+                    "dpressure": st["dpressure"], "dvelocity": st["dvelocity"]}
+        return c.download("U"), c.download("P")
+
+    Uf, Pf = timed("form_guess")
     c.set_option("warm_start", 0)
-    c.upload("T", T)
-    c.thermal_buoyancy()
-    c.upload("U", U0)
-    c.upload("P", P0)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    b = c.general_stokes_solver()
-    torch.cuda.synchronize()
-    tb = time.perf_counter() - t0
-    Ub, Pb = c.download("U"), c.download("P")
+    sols = {}
+    for tag, (U0, P0) in (("extrapolation", (2 * Un - Ua, 2 * Pn - Pa)), ("previous_solution", (Un, Pn))):
+        c.upload("T", T)
+        c.thermal_buoyancy()
+        c.upload("U", U0)
+        c.upload("P", P0)
+        sols[tag] = timed(tag)
     rel = lambda x, y: float(np.linalg.norm(x - y) / np.linalg.norm(y))  # noqa: E731
-    out = {"form_guess": {"uzawa_iterations": a["iterations"], "mg_calls": a["mg_cycles"], "s": ta,
-                          "dpressure": a["dpressure"], "dvelocity": a["dvelocity"]},
-           "previous_solution": {"uzawa_iterations": b["iterations"], "mg_calls": b["mg_cycles"], "s": tb,
-                                 "dpressure": b["dpressure"], "dvelocity": b["dvelocity"]},
-           "U_rel_diff": rel(Ua, Ub), "P_rel_diff": rel(Pa - Pa.mean(), Pb - Pb.mean())}
-    log(f"[{form}] same state, form's guess: Uzawa {a['iterations']} in {ta:.2f} s (dp/p {a['dpressure']:.1e}); "
-        f"previous solution: Uzawa {b['iterations']} in {tb:.2f} s (dp/p {b['dpressure']:.1e}); "
-        f"U differs by {out['U_rel_diff']:.1e}, P by {out['P_rel_diff']:.1e}")
+    Ub, Pb = sols["previous_solution"]
+    out["U_rel_diff_form_vs_previous"] = rel(Uf, Ub)
+    out["P_rel_diff_form_vs_previous"] = rel(Pf - Pf.mean(), Pb - Pb.mean())
+    log(f"[{form}] same state: " + "; ".join(
+        f"{k} Uzawa {v['uzawa_iterations']} in {v['s']:.2f} s" for k, v in out.items() if isinstance(v, dict)) +
+        f"; U differs by {out['U_rel_diff_form_vs_previous']:.1e}, P by {out['P_rel_diff_form_vs_previous']:.1e}")
     return out
 
-HEADLINE = "mixed16+extrap"
+
+HEADLINE = "mixed16+learned"
 
 
 def main():

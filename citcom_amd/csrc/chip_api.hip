@@ -473,7 +473,6 @@ int pg_timestep(chip_ctx *c, chip_pg_stats *st) {
         }
     } while (iredo == 1 && last_sub <= 5);
     c->elapsed_time += timestep;
-    c->t_now += timestep;
     conform_temperature(q, c, c->T);
     thermal_buoyancy(q, c, c->T, c->buoy);
     CHIP_CHECK(hipGetLastError());
@@ -498,9 +497,11 @@ int heat_flux_nu(chip_ctx *c, float *nu) {
 
 // the initial guess of the Uzawa solve: the previous solution (the
 // reference, Stokes_flow_Incomp.c:354-366 start from E->U, E->P as the last
-// solve left them) or, with the warm start, that solution extrapolated in
-// time from the last two: x + r (x - x_prev), r = dt_now / dt_prev; with
-// warm_start = 2 the U-Net's correction of that extrapolation is added.
+// solve left them) or, with the warm start, that solution extrapolated
+// along the sequence of solves from the last two, 2 x_n - x_(n-1) (the
+// timestep follows the flow speed, so the change per step is more regular
+// than the change per unit time); with warm_start = 2 the U-Net's
+// correction of that extrapolation is added, once three solves are kept.
 // When training samples are recorded (or the network runs) the network's
 // input is formed here and the target after the solve (warm_finish).
 int warm_start_guess(chip_ctx *c) {
@@ -509,48 +510,49 @@ int warm_start_guess(chip_ctx *c) {
     hipStream_t q = S(c);
     const Level &L = c->L[c->levmax];
     const long nno = L.g.nno, neq = 3L * nno;
-    const double dt_now = c->t_now - c->t_sol[0], dt_prev = c->t_sol[0] - c->t_sol[1];
-    const double r = (c->hist >= 2 && dt_prev > 0.0 && dt_now > 0.0) ? dt_now / dt_prev : 0.0;
-    extrapolate(q, c->U, c->Uprev, r, neq);
-    extrapolate(q, c->P, c->Pprev, r, L.g.nel);
-    const bool net = c->warm_start == 2 && c->wn_set && c->hist >= 2;
-    const bool rec = c->rec_x && c->rec_y && c->hist >= 2;
+    const bool net = c->warm_start == 2 && c->wn_set && c->hist >= 3;
+    const bool rec = c->rec_x && c->rec_y && c->hist >= 3;
+    const double r = c->hist >= 2 ? 1.0 : 0.0;
+    extrapolate(q, c->U, c->Uprev, c->Uprev2, r, (net || rec) ? c->D2u : nullptr, neq);
+    extrapolate(q, c->P, c->Pprev, c->Pprev2, r, (net || rec) ? c->D2p : nullptr, L.g.nel);
+    c->wscale[6] = c->hist;
     if (!net && !rec) return 0;
     // features: the temperature's departure from its own extrapolation, the
     // nodal pressures, log10 viscosity (EVI is this step's), and their scales
-    warm_dT(q, c->T, c->Th0, c->Th1, (float)r, nno, c->wn_dT);
+    warm_dT(q, c->T, c->Th0, c->Th1, nno, c->wn_dT);
     e2n(q, L, c->P, nullptr, c->wn_pn);
     e2n(q, L, c->P, c->Pprev, c->wn_dpn);
+    e2n(q, L, c->D2p, nullptr, c->wn_d2pn);
     cnet_logeta(q, L, c->wn_logeta);
-    dots(q, c, (int)neq, 1, (const double *)c->U, (const double *)c->U, nullptr, nullptr, nullptr, nullptr, nullptr,
-         nullptr, c->dots + D_WS);
+    dots(q, c, (int)neq, 2, (const double *)c->U, (const double *)c->U, (const double *)c->D2u,
+         (const double *)c->D2u, nullptr, nullptr, nullptr, nullptr, c->dots + D_WS);
     dots(q, c, (int)nno, 3, (const float *)c->wn_dT, (const float *)c->wn_dT, (const float *)c->wn_pn,
-         (const float *)c->wn_pn, (const float *)c->buoy, (const float *)c->buoy, nullptr, nullptr, c->dots + D_WS + 1);
+         (const float *)c->wn_pn, (const float *)c->wn_d2pn, (const float *)c->wn_d2pn, nullptr, nullptr,
+         c->dots + D_WS + 2);
     if (sync_dots(c)) return -1;
     const double *h = c->h_dots + D_WS;
-    const double s_u = std::sqrt(h[0] / neq), s_dT = std::sqrt(h[1] / nno), s_p = std::sqrt(h[2] / nno);
-    const double s_th = std::sqrt(h[3] / nno) / std::max(1e-30, (double)c->p.rayleigh);
-    // the extrapolation's error scales with the relative size of the
-    // temperature's departure (the Stokes response is linear in the buoyancy)
-    const double rel = s_th > 0.0 ? s_dT / s_th : 0.0;
-    const double s_eu = s_u * rel, s_ep = s_p * rel;
+    const double s_u = std::sqrt(h[0] / neq), s_2u = std::sqrt(h[1] / neq), s_dT = std::sqrt(h[2] / nno),
+                 s_p = std::sqrt(h[3] / nno), s_2p = std::sqrt(h[4] / nno);
     double *w = c->wscale;
-    w[0] = s_u, w[1] = s_p, w[2] = s_dT, w[3] = s_th, w[4] = s_eu, w[5] = s_ep, w[6] = r, w[7] = 1.0;
+    w[0] = s_u, w[1] = s_p, w[2] = s_dT, w[3] = s_2u, w[4] = s_2p, w[5] = 0.0, w[7] = 1.0;
     auto inv = [](double v) { return v > 0.0 ? (float)(1.0 / v) : 0.0f; };
-    warm_features(q, L, c->T, c->wn_dT, c->wn_logeta, c->U, c->Uprev, c->wn_pn, c->wn_dpn, inv(s_dT), inv(s_u),
-                  inv(s_p), c->wn_x);
+    warm_features(q, L, c->T, c->wn_dT, c->wn_logeta, c->U, c->Uprev, c->D2u, c->wn_pn, c->wn_dpn, c->wn_d2pn,
+                  inv(s_dT), inv(s_u), inv(s_p), inv(s_2u), inv(s_2p), c->wn_x);
     copy(q, c->Ue, c->U, neq);
     copy(q, c->Pe, c->P, L.g.nel);
+    // the extrapolation's error is of the order of the second difference:
+    // the network's outputs are in those units
+    const bool scaled = s_2u > 0.0 && s_2p > 0.0;
     if (rec) {
         CHIP_CHECK(hipMemcpyAsync(c->rec_x, c->wn_x, 16 * (size_t)nno * sizeof(uint16_t), hipMemcpyDeviceToDevice, q));
-        c->feat_pending = s_eu > 0.0 && s_ep > 0.0;
+        c->feat_pending = scaled;
     }
-    if (net && s_eu > 0.0 && s_ep > 0.0) {
+    if (net && scaled) {
         if (cl_unet_forward(c->wn_x, 1, L.g.nox, L.g.noy, L.g.noz, c->wn_w, c->wn_b, c->wn_out, c->wn_ws, q)) {
             set_error("warm start: the U-Net's forward pass failed to launch");
             return -1;
         }
-        warm_apply(q, L, c->wn_out, s_eu, s_ep, c->U, c->P);
+        warm_apply(q, L, c->wn_out, s_2u, s_2p, c->U, c->P);
     }
     CHIP_CHECK(hipGetLastError());
     return 0;
@@ -564,13 +566,11 @@ int warm_finish(chip_ctx *c) {
     const Level &L = c->L[c->levmax];
     if (c->feat_pending) {
         e2n(q, L, c->P, c->Pe, c->wn_dpn);
-        warm_target(q, L, c->U, c->Ue, c->wn_dpn, 1.0 / c->wscale[4], 1.0 / c->wscale[5], c->rec_y);
+        warm_target(q, L, c->U, c->Ue, c->wn_dpn, 1.0 / c->wscale[3], 1.0 / c->wscale[4], c->rec_y);
         c->feat_pending = false;
     }
     fcopy(q, c->Th1, c->Th0, L.g.nno);
     fcopy(q, c->Th0, c->T, L.g.nno);
-    c->t_sol[1] = c->t_sol[0];
-    c->t_sol[0] = c->t_now;
     c->hist++;
     CHIP_CHECK(hipGetLastError());
     return 0;

@@ -134,14 +134,16 @@ struct chip_ctx {
     uint16_t *cnet_w = nullptr;
     float *cnet_b = nullptr, *cnet_ys = nullptr, *cnet_logeta = nullptr;
     // warm start of the Uzawa solve (option "warm_start"): 0 the previous
-    // solution as it is (the reference); 1 extrapolated in time from the last
-    // two solutions; 2 the same plus the learned correction (chip_set_warm_net)
+    // solution as it is (the reference); 1 extrapolated along the sequence
+    // of solves from the last two; 2 the same plus the learned correction
+    // (chip_set_warm_net)
     int warm_start = 0;
     int hist = 0;                 // solutions kept since the option was set
-    double t_now = 0.0;           // model time, the sum of the PG timesteps (FP64)
-    double t_sol[2] = {0.0, 0.0}; // times of the last two solves
-    double *Uprev = nullptr, *Pprev = nullptr;  // the solution before the last
-    float *Th0 = nullptr, *Th1 = nullptr;       // T at the last two solves
+    double *Uprev = nullptr, *Pprev = nullptr;    // the two solutions before the current one
+    double *Uprev2 = nullptr, *Pprev2 = nullptr;
+    double *D2u = nullptr, *D2p = nullptr;        // their second differences
+    float *Th0 = nullptr, *Th1 = nullptr;         // T at the last two solves
+    float *wn_d2pn = nullptr;
     // learned correction of the extrapolated guess (chip_set_warm_net): the
     // U-Net's BF16 weights and FP32 biases, its input [nno][16] and output
     // [nno][16], its workspace, and the finest-level scratch of the features
@@ -156,7 +158,7 @@ struct chip_ctx {
     uint16_t *rec_x = nullptr;
     float *rec_y = nullptr;
     bool feat_pending = false;   // this solve's features were formed; its target follows the solve
-    double wscale[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // s_u, s_p, s_dT, s_theta, s_eu, s_ep, r, formed
+    double wscale[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // s_u, s_p, s_dT, s_2u, s_2p, (unused), hist, formed
     bool print_convergence = false;  // per-iteration lines on stderr, as E->control.print_convergence
     hipGraphExec_t mg_graph = nullptr;
     chip::Prof prof;

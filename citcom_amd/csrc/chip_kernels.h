@@ -91,7 +91,8 @@ void scale_half(hipStream_t s, const Level &L);
 void add_f32(hipStream_t s, double *y, const float *x, long n);   // y += x
 void copy(hipStream_t s, double *dst, const double *src, long n);
 void axpy_scalar(hipStream_t s, double *y, const double *x, double a, long n);   // y += a x
-void extrapolate(hipStream_t s, double *x, double *xprev, double r, long n);   // x += r (x - xprev); xprev = old x
+// x += r (x - h1); d2 (optional) = x - 2 h1 + h2; h2 = h1; h1 = old x
+void extrapolate(hipStream_t s, double *x, double *h1, double *h2, double r, double *d2, long n);
 void sub(hipStream_t s, double *r, const double *a, const double *b, long n);    // r = a - b
 void sub(hipStream_t s, float *r, const float *a, const float *b, long n);
 // vel += alpha del, alpha = sc[1]/sc[0]; if res: res -= alpha AU
@@ -124,11 +125,12 @@ void heat_flux(hipStream_t s, chip_ctx *c, const float *T, const float *V);
 void fcopy(hipStream_t s, float *dst, const float *src, long n);
 
 // ---- the learned warm start's inputs and outputs (finest level)
-void warm_dT(hipStream_t s, const float *T, const float *Th0, const float *Th1, float r, long n, float *dT);
+void warm_dT(hipStream_t s, const float *T, const float *Th0, const float *Th1, long n, float *dT);
 void e2n(hipStream_t s, const Level &L, const double *Pa, const double *Pb, float *out);
 void warm_features(hipStream_t s, const Level &L, const float *T, const float *dT, const float *logeta,
-                   const double *U, const double *Uprev, const float *pn, const float *dpn, float inv_dT, float inv_u,
-                   float inv_p, uint16_t *x);
+                   const double *U, const double *Uprev, const double *D2, const float *pn, const float *dpn,
+                   const float *d2pn, float inv_dT, float inv_u, float inv_p, float inv_2u, float inv_2p,
+                   uint16_t *x);
 void warm_apply(hipStream_t s, const Level &L, const float *out, double s_eu, double s_ep, double *U, double *P);
 void warm_target(hipStream_t s, const Level &L, const double *U, const double *Ue, const float *dpn, double inv_eu,
                  double inv_ep, float *y);

@@ -1306,17 +1306,21 @@ __global__ void k_axpy(double *y, const double *x, double a, long n) {
 }
 void axpy_scalar(hipStream_t s, double *y, const double *x, double a, long n) { hipLaunchKernelGGL(k_axpy, dim3(nblk(n)), dim3(TPB), 0, s, y, x, a, n); }
 
-// the warm start's extrapolation in time: x <- x + r (x - xprev), then
-// xprev <- the old x (r = 0: xprev <- x, x unchanged)
-__global__ void k_extrapolate(double *x, double *xprev, double r, long n) {
+// the warm start's extrapolation along the sequence of solves: with x the
+// last solution and h1, h2 the two before it, x <- x + r (x - h1) (r = 1
+// once two solves are kept, 0 before); d2 (optional) <- x - 2 h1 + h2, the
+// second difference; then h2 <- h1, h1 <- the old x
+__global__ void k_extrapolate(double *x, double *h1, double *h2, double r, double *d2, long n) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double xn = x[i];
-    x[i] = xn + r * (xn - xprev[i]);
-    xprev[i] = xn;
+    const double xn = x[i], a = h1[i];
+    x[i] = xn + r * (xn - a);
+    if (d2) d2[i] = (xn - a) - (a - h2[i]);
+    h2[i] = a;
+    h1[i] = xn;
 }
-void extrapolate(hipStream_t s, double *x, double *xprev, double r, long n) {
-    hipLaunchKernelGGL(k_extrapolate, dim3(nblk(n)), dim3(TPB), 0, s, x, xprev, r, n);
+void extrapolate(hipStream_t s, double *x, double *h1, double *h2, double r, double *d2, long n) {
+    hipLaunchKernelGGL(k_extrapolate, dim3(nblk(n)), dim3(TPB), 0, s, x, h1, h2, r, d2, n);
 }
 
 template <typename V>
@@ -2383,15 +2387,20 @@ template int cnet_apply<float>(hipStream_t, const chip_ctx *, const Level &, con
 
 namespace chip {
 
-// dT = T - (Th0 + r (Th0 - Th1)): the temperature's departure from its own
-// extrapolation in time over the last two solves
-__global__ void k_warm_dT(const float *__restrict__ T, const float *__restrict__ Th0, const float *__restrict__ Th1,
-                          float r, long n, float *__restrict__ dT) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dT[i] = T[i] - (Th0[i] + r * (Th0[i] - Th1[i]));
+__device__ __forceinline__ uint16_t warm_bf16(float f) {
+    const unsigned u = __float_as_uint(f);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
 }
-void warm_dT(hipStream_t s, const float *T, const float *Th0, const float *Th1, float r, long n, float *dT) {
-    hipLaunchKernelGGL(k_warm_dT, dim3(nblk(n)), dim3(TPB), 0, s, T, Th0, Th1, r, n, dT);
+
+// dT = T - (2 Th0 - Th1): the temperature's departure from its own linear
+// extrapolation over the last two solves
+__global__ void k_warm_dT(const float *__restrict__ T, const float *__restrict__ Th0, const float *__restrict__ Th1,
+                          long n, float *__restrict__ dT) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dT[i] = T[i] - (2.0f * Th0[i] - Th1[i]);
+}
+void warm_dT(hipStream_t s, const float *T, const float *Th0, const float *Th1, long n, float *dT) {
+    hipLaunchKernelGGL(k_warm_dT, dim3(nblk(n)), dim3(TPB), 0, s, T, Th0, Th1, n, dT);
 }
 
 // nodal value of an element field: the mean of (Pa - Pb) over the node's
@@ -2417,20 +2426,17 @@ void e2n(hipStream_t s, const Level &L, const double *Pa, const double *Pb, floa
     hipLaunchKernelGGL(k_e2n, dim3(nblk(L.g.nno)), dim3(TPB), 0, s, L.g, Pa, Pb, out);
 }
 
-__device__ __forceinline__ uint16_t warm_bf16(float f) {
-    const unsigned u = __float_as_uint(f);
-    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
-}
-
 // the network's input, 16 bf16 channels per node: T - 1/2, dT / s_dT,
-// log10(eta) / 2.5, U / s_u (3), nodal P / s_p, (U - U_prev) / s_u (3),
-// nodal (P - P_prev) / s_p, z (node index over noz - 1), zeros.  U, P are
-// the extrapolated guess, U_prev, P_prev the last solution.
+// log10(eta) / 2.5, the extrapolated U / s_u (3) and nodal P / s_p, the
+// last step's increments U - U_prev / s_u (3) and nodal P - P_prev / s_p,
+// the second differences of U / s_2u (3) and of the nodal P / s_2p, z
+// (node index over noz - 1).  U_prev here is the last solution.
 __global__ void k_warm_features(Geo g, const float *__restrict__ T, const float *__restrict__ dT,
                                 const float *__restrict__ logeta, const double *__restrict__ U,
-                                const double *__restrict__ Uprev, const float *__restrict__ pn,
-                                const float *__restrict__ dpn, float inv_dT, float inv_u, float inv_p,
-                                uint16_t *__restrict__ x) {
+                                const double *__restrict__ Uprev, const double *__restrict__ D2,
+                                const float *__restrict__ pn, const float *__restrict__ dpn,
+                                const float *__restrict__ d2pn, float inv_dT, float inv_u, float inv_p,
+                                float inv_2u, float inv_2p, uint16_t *__restrict__ x) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= g.nno) return;
     float f[16];
@@ -2438,13 +2444,15 @@ __global__ void k_warm_features(Geo g, const float *__restrict__ T, const float 
     f[1] = dT[n] * inv_dT;
     f[2] = logeta[n] * 0.4f;
     for (int d = 0; d < 3; d++) {
-        f[3 + d] = (float)U[3 * (size_t)n + d] * inv_u;
-        f[7 + d] = (float)(U[3 * (size_t)n + d] - Uprev[3 * (size_t)n + d]) * inv_u;
+        const size_t i = 3 * (size_t)n + d;
+        f[3 + d] = (float)U[i] * inv_u;
+        f[7 + d] = (float)(U[i] - Uprev[i]) * inv_u;
+        f[11 + d] = (float)D2[i] * inv_2u;
     }
     f[6] = pn[n] * inv_p;
     f[10] = dpn[n] * inv_p;
-    f[11] = (float)(n % g.noz) / (float)(g.noz - 1);
-    for (int q = 12; q < 16; q++) f[q] = 0.f;
+    f[14] = d2pn[n] * inv_2p;
+    f[15] = (float)(n % g.noz) / (float)(g.noz - 1);
     uint4 *o = reinterpret_cast<uint4 *>(x + (size_t)n * 16);
     for (int q = 0; q < 2; q++) {
         uint4 w;
@@ -2456,10 +2464,11 @@ __global__ void k_warm_features(Geo g, const float *__restrict__ T, const float 
     }
 }
 void warm_features(hipStream_t s, const Level &L, const float *T, const float *dT, const float *logeta,
-                   const double *U, const double *Uprev, const float *pn, const float *dpn, float inv_dT, float inv_u,
-                   float inv_p, uint16_t *x) {
-    hipLaunchKernelGGL(k_warm_features, dim3(nblk(L.g.nno)), dim3(TPB), 0, s, L.g, T, dT, logeta, U, Uprev, pn, dpn,
-                       inv_dT, inv_u, inv_p, x);
+                   const double *U, const double *Uprev, const double *D2, const float *pn, const float *dpn,
+                   const float *d2pn, float inv_dT, float inv_u, float inv_p, float inv_2u, float inv_2p,
+                   uint16_t *x) {
+    hipLaunchKernelGGL(k_warm_features, dim3(nblk(L.g.nno)), dim3(TPB), 0, s, L.g, T, dT, logeta, U, Uprev, D2, pn,
+                       dpn, d2pn, inv_dT, inv_u, inv_p, inv_2u, inv_2p, x);
 }
 
 // the network's correction: U += s_eu out[n][0..2] at every node; P[e] +=

@@ -264,6 +264,8 @@ int alloc_warm_start(chip_ctx *c) {
     if (c->Uprev) return 0;
     const size_t nno = (size_t)c->L[c->levmax].g.nno, nel = (size_t)c->L[c->levmax].g.nel;
     return (dev_alloc(c, &c->Uprev, 3 * nno) || dev_alloc(c, &c->Pprev, nel) || dev_alloc(c, &c->Th0, nno) ||
+            dev_alloc(c, &c->Uprev2, 3 * nno) || dev_alloc(c, &c->Pprev2, nel) || dev_alloc(c, &c->D2u, 3 * nno) ||
+            dev_alloc(c, &c->D2p, nel) || dev_alloc(c, &c->wn_d2pn, nno) ||
             dev_alloc(c, &c->Th1, nno) || dev_alloc(c, &c->wn_dT, nno) || dev_alloc(c, &c->wn_pn, nno) ||
             dev_alloc(c, &c->wn_dpn, nno) || dev_alloc(c, &c->wn_logeta, nno) || dev_alloc(c, &c->wn_x, 16 * nno) ||
             dev_alloc(c, &c->Ue, 3 * nno) || dev_alloc(c, &c->Pe, nel))

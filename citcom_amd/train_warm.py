@@ -6,7 +6,7 @@ Samples come from the solver itself: time loops of the 64^3-element
 workload (tdep64: 65^3 nodes, 5 levels) at Rayleigh numbers and viscosity
 contrasts drawn log-uniformly from [1e5, 1e7] and [1e2, 1e5], each solve
 to the workload's own criterion with the extrapolated warm start; every
-solve with two earlier ones gives one (input, target) pair (unet.py,
+solve with three earlier ones gives one (input, target) pair (unet.py,
 chip_warm_record).  Training runs on the project's BF16 MFMA kernels with
 Adam; held-out cases measure the loss against the extrapolation's
 (the mean square of the targets).  The weights, the loss history and the

@@ -273,7 +273,7 @@ def draw_cases(n: int, seed: int, ra=(1e5, 1e7), contrast=(1e2, 1e5)):
 
 def make_samples(base: str, cases, steps: int, mixed: int = 2, log=print):
     """Run each case's time loop with the extrapolated warm start (each solve
-    to the workload's own criterion) and record every solve that has two
+    to the workload's own criterion) and record every solve that has three
     earlier ones: features bf16 [N][nno][16], targets FP32 [N][nno][4], all
     on the device.  `steps` recorded samples per case."""
     import torch
@@ -291,14 +291,15 @@ def make_samples(base: str, cases, steps: int, mixed: int = 2, log=print):
         c.set_option("warm_start", 1)
         c.initialize(_deck.seeded_temperature(p, seed=seed))
         c.general_stokes_solver()
-        c.timestep()     # the second solve: the history then holds two
+        c.timestep()     # the history then holds three solves, the
+        c.timestep()     # least the network's input needs
         its = []
         for s in range(steps):
             c.warm_record(feats[k].data_ptr(), targs[k].data_ptr())
             r = c.timestep()
             sc = c.warm_scales()
             its.append(r["stokes"]["iterations"])
-            info.append({"case": ci, "rayleigh": ra, "contrast": contrast, "step": s + 2,
+            info.append({"case": ci, "rayleigh": ra, "contrast": contrast, "step": s + 3,
                          "iterations": r["stokes"]["iterations"], "scales": sc[:7].tolist()})
             k += 1
         c.warm_record(None, None)

@@ -189,10 +189,10 @@ long chip_device_bytes(chip_ctx *c);
  * "warm_start": 0 (default) starts each Uzawa solve from the previous
  * solution as the last solve left it, as the reference does
  * (Stokes_flow_Incomp.c:354-366 start from E->U, E->P); 1 starts it from
- * the last two solutions extrapolated in time, U + r (U - U_prev) and
- * P + r (P - P_prev) with r = dt_now / dt_prev (the PG timesteps since the
- * two solves), once two solves have been kept; 2 adds the learned U-Net's
- * correction of that extrapolation (chip_set_warm_net).  Setting the option
+ * the last two solutions extrapolated along the sequence of solves,
+ * 2 U_n - U_(n-1) and 2 P_n - P_(n-1), once two solves have been kept; 2
+ * adds the learned U-Net's correction of that extrapolation
+ * (chip_set_warm_net) once three have been kept.  Setting the option
  * starts a new history.  The solve still iterates to the same criterion.
  * "print_convergence": 1 prints one line per Uzawa iteration and per inner
  * solve on stderr (generate_log_message, Stokes_flow_Incomp.c:501-526;
@@ -221,28 +221,29 @@ int chip_coarse_net_state(chip_ctx *c, int *on, int *fallbacks);
 
 /* ---- learned warm start (no counterpart in the reference) ----------------
  * The Uzawa solve's initial guess with the option "warm_start" = 2: the last
- * two solutions extrapolated in time, corrected by a U-Net (BF16 MFMA
- * convolutions, FP32 accumulation; cl_unet_forward of citcom_learn.h) on the
- * finest level's nodes.  Its input per node, 16 bf16 channels: T - 1/2,
- * dT / s_dT with dT = T - (T1 + r (T1 - T2)) (T1, T2 the temperatures of the
- * last two solves), log10(eta) / 2.5 (Gauss-point mean of the adjacent
- * elements, averaged), the extrapolated U / s_u (3), its nodal P / s_p
- * (mean of the adjacent elements), the extrapolation's increments of U and
- * P over the same scales (4), z, zeros; s_u, s_p, s_dT the RMS values.  Its
- * output channels 0..2 times s_u s_dT / s_theta are added to U at every
- * node and channel 3 times s_p s_dT / s_theta, averaged over an element's
- * eight nodes, to its P; s_theta = RMS(buoyancy) / Ra.  The solve then runs
- * to its own criterion, so accuracy is not traded.  Weights: bf16
+ * two solutions extrapolated along the sequence of solves, corrected by a
+ * U-Net (BF16 MFMA convolutions, FP32 accumulation; cl_unet_forward of
+ * citcom_learn.h) on the finest level's nodes.  Its input per node, 16 bf16
+ * channels: T - 1/2; dT / s_dT with dT = T - (2 T_n - T_(n-1)) (T_n,
+ * T_(n-1) the temperatures of the last two solves); log10(eta) / 2.5
+ * (Gauss-point mean of the adjacent elements, averaged); the extrapolated
+ * U / s_u (3) and its nodal P / s_p (mean of the adjacent elements); the
+ * last increments U_n - U_(n-1) / s_u (3) and of the nodal P / s_p; the
+ * second differences U_n - 2 U_(n-1) + U_(n-2) / s_2u (3) and of the nodal
+ * P / s_2p; z (node index over noz - 1); s_* the RMS values.  Its output
+ * channels 0..2 times s_2u are added to U at every node and channel 3 times
+ * s_2p, averaged over an element's eight nodes, to its P.  The solve then
+ * runs to its own criterion, so accuracy is not traded.  Weights: bf16
  * [594432] in layer order ([co][27][ci] each), biases FP32 [528]. */
 int chip_set_warm_net(chip_ctx *c, const uint16_t *weights, long n_weights, const float *bias, long n_bias);
-/* Training samples: with the warm start on, every solve that has two earlier
- * ones writes the network's input (bf16 [nno][16]) to features and, after
- * the solve, the target (FP32 [nno][4]: (U - U_extrapolated) / (s_u s_dT /
- * s_theta) per node, the nodal mean of (P - P_extrapolated) / (s_p s_dT /
- * s_theta)) to targets; device pointers, both NULL to stop. */
+/* Training samples: with the warm start on, every solve that has three
+ * earlier ones writes the network's input (bf16 [nno][16]) to features and, after
+ * the solve, the target (FP32 [nno][4]: (U - U_extrapolated) / s_2u per
+ * node, the nodal mean of (P - P_extrapolated) / s_2p) to targets; device
+ * pointers, both NULL to stop.  Samples start once three solves are kept. */
 int chip_warm_record(chip_ctx *c, void *features, void *targets);
-/* the last formed input's scales: s_u, s_p, s_dT, s_theta, output scale of
- * U, output scale of P, r, 1 (0 before any) */
+/* the last formed input's scales: s_u, s_p, s_dT, s_2u, s_2p, 0, the
+ * number of solves kept before this one, 1 (0 before any input formed) */
 int chip_warm_scales(chip_ctx *c, double *out8);
 
 #ifdef __cplusplus
