@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "chip_kernels.h"
 #include "../../include/citcom_learn.h"
@@ -949,6 +950,30 @@ int chip_vrms(chip_ctx *c, double *vrms_out) {
     vrms(c->stream, c, c->V, c->dots + D_TMP);
     if (sync_dots(c)) return -1;
     *vrms_out = std::sqrt(c->h_dots[D_TMP] / c->h_dots[D_TMP + 1]);
+    return 0;
+}
+
+int chip_averages(chip_ctx *c, float *T_ave, float *vrms_ave, float *visc_ave) {
+    CHIP_NEED_CTX(c);
+    const Geo &g = c->L[c->levmax].g;
+    if (!c->Have) {
+        CHIP_CHECK(hipMalloc(&c->Have, 3 * (size_t)g.noz * sizeof(float)));
+        c->allocs.push_back(c->Have);
+        CHIP_CHECK(hipMalloc(&c->ave_scratch, (size_t)g.nno * sizeof(float)));
+        c->allocs.push_back(c->ave_scratch);
+        c->dev_bytes += (3L * g.noz + g.nno) * (long)sizeof(float);
+    }
+    averages(c->stream, c, c->Have, c->ave_scratch);
+    CHIP_CHECK(hipGetLastError());
+    std::vector<float> h(3 * (size_t)g.noz);
+    CHIP_CHECK(hipMemcpyAsync(h.data(), c->Have, h.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    CHIP_CHECK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < g.noz; k++) {
+        if (T_ave) T_ave[k] = h[k];
+        // Have.vrms[i] = sqrt(Have.vrms[i]) on a float (Process_velocity.c:219-220)
+        if (vrms_ave) vrms_ave[k] = (float)std::sqrt((double)h[g.noz + k]);
+        if (visc_ave) visc_ave[k] = h[2 * g.noz + k];
+    }
     return 0;
 }
 

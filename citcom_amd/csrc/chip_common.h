@@ -104,6 +104,8 @@ struct chip_ctx {
     int *mat = nullptr;      // [nel] material group (1-based ids)
     float *H = nullptr;      // [noz] horizontal average
     float *hf = nullptr;     // [nno] nodal heat flux
+    float *Have = nullptr;   // [3 noz] layer averages of T, |V|^2, viscosity (chip_averages)
+    float *ave_scratch = nullptr;  // [nno]
     // Uzawa vectors
     double *r0 = nullptr, *r1 = nullptr, *r2 = nullptr, *z0 = nullptr, *z1 = nullptr;
     double *s1 = nullptr, *s2 = nullptr, *Ah = nullptr, *u1 = nullptr;

@@ -122,6 +122,8 @@ void thermal_buoyancy(hipStream_t s, const chip_ctx *c, const float *T, float *b
 void vrms(hipStream_t s, chip_ctx *c, const float *V, double *out2);
 void assemble_forces(hipStream_t s, const chip_ctx *c, const float *buoy, double *F);
 void heat_flux(hipStream_t s, chip_ctx *c, const float *T, const float *V);
+// averages (Process_velocity.c:179-224): Have[0..noz) T, [noz..2noz) |V|^2, [2noz..3noz) nodal viscosity
+void averages(hipStream_t s, chip_ctx *c, float *Have, float *scratch);
 void fcopy(hipStream_t s, float *dst, const float *src, long n);
 
 // ---- the learned warm start's inputs and outputs (finest level)

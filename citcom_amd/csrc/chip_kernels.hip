@@ -2025,6 +2025,27 @@ __global__ void __launch_bounds__(TPB) k_horiz_ave(Geo g, const float *__restric
     if (threadIdx.x == 0 && v[1] != 0.0) H[kz] = (float)(v[0] / v[1]);
 }
 
+// averages (Process_velocity.c:179-224): per-layer horizontal averages of
+// T, of |V|^2 (square-rooted on the host) and of the nodal viscosity
+// (visc_from_gint_to_nodes of the finest EVI); |V|^2 in FP32 as the
+// reference forms it
+__global__ void k_vsq(long nno, const float *__restrict__ V, float *__restrict__ out) {
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nno) return;
+    const float a = V[n], b = V[nno + n], c = V[2 * nno + n];
+    out[n] = a * a + b * b + c * c;
+}
+
+void averages(hipStream_t s, chip_ctx *c, float *Have /* [3 noz] */, float *scratch /* [nno] */) {
+    Level &L = c->L[c->levmax];
+    hipLaunchKernelGGL(k_horiz_ave, dim3(L.g.noz), dim3(TPB), 0, s, L.g, c->xc_d, c->yc_d, c->T, Have);
+    hipLaunchKernelGGL(k_vsq, dim3(nblk(L.g.nno)), dim3(TPB), 0, s, (long)L.g.nno, c->V, scratch);
+    hipLaunchKernelGGL(k_horiz_ave, dim3(L.g.noz), dim3(TPB), 0, s, L.g, c->xc_d, c->yc_d, scratch, Have + L.g.noz);
+    hipLaunchKernelGGL(k_visc_to_nodes, dim3(nblk(L.g.nno)), dim3(TPB), 0, s, L.g, L.EVI, L.TWW, L.MASS, scratch);
+    hipLaunchKernelGGL(k_horiz_ave, dim3(L.g.noz), dim3(TPB), 0, s, L.g, c->xc_d, c->yc_d, scratch,
+                       Have + 2 * L.g.noz);
+}
+
 __global__ void k_sub_layer(Geo g, const float *__restrict__ H, float *b) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= g.nno) return;

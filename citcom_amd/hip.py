@@ -112,6 +112,7 @@ def lib():
         L.chip_set_option.argtypes = [P, C.c_char_p, C.c_int]
         L.chip_set_coarse_net.argtypes = [P, C.c_int, P, P, P]
         L.chip_coarse_net_state.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.chip_averages.argtypes = [P, P, P, P]
         L.chip_set_warm_net.argtypes = [P, P, C.c_long, P, C.c_long]
         L.chip_warm_record.argtypes = [P, P, P]
         L.chip_warm_scales.argtypes = [P, P]
@@ -251,6 +252,15 @@ class Citcom:
         v = C.c_double()
         _check(lib().chip_vrms(self.h, C.byref(v)), "vrms")
         return v.value
+
+    def averages(self) -> dict:
+        """layer profiles of averages (Process_velocity.c:179-224): Have.T,
+        Have.vrms, Have.Vi, each [noz] (chip_averages)"""
+        noz = self.p.dims()[2]
+        out = {k: np.zeros(noz, np.float32) for k in ("T", "vrms", "Vi")}
+        _check(lib().chip_averages(self.h, out["T"].ctypes.data, out["vrms"].ctypes.data, out["Vi"].ctypes.data),
+               "averages")
+        return out
 
     def heat_flux(self) -> tuple[float, float]:
         nu = (C.c_float * 2)()

@@ -157,6 +157,15 @@ int chip_heat_flux(chip_ctx *c, float *nu);
  * no routine for it */
 int chip_vrms(chip_ctx *c, double *vrms);
 
+/* averages (Process_velocity.c:179-224) and the temperature profile of
+ * heat_flux (Process_buoyancy.c:93): per node layer k = 0..noz-1 the
+ * horizontal averages (return_horiz_ave, Global_operations.c:133-251) of
+ * T (Have.T), of |V|^2 square-rooted (Have.vrms) and of the nodal
+ * viscosity of the finest EVI (visc_from_gint_to_nodes, Nodal_mesh.c:583-615;
+ * Have.Vi), from the device T, V and EVI.  Host arrays of noz floats; any
+ * may be NULL. */
+int chip_averages(chip_ctx *c, float *T_ave, float *vrms_ave, float *visc_ave);
+
 /* One timestep of the reference's time loop (Citcom.c:111-161) with every
  * stage on the device: PG_timestep, heat_flux, then general_stokes_solver
  * (Drive_solvers.c:45-165: velocities_conform_bcs, assemble_forces, viscosity

@@ -13,14 +13,16 @@
  *       strip_bcs_from_residual on every level (inputs and outputs dumped).
  *   ref_harness <deck> <outdir> run <nsteps>
  *       the time loop of Citcom.c:97-161 for nsteps steps; per-step scalars
- *       to steps.txt and final T, Tdot, U, P, V, Nu to outdir.
+ *       to steps.txt and final T, Tdot, U, P, V, Nu and the layer profiles
+ *       of averages (Have.T, Have.vrms, Have.Vi) to outdir.
  *   ref_harness <deck> <outdir> time
  *       a bounded timing sample of the reference on the deck (CPU baseline):
  *       the operator build of one step, one multi_grid call and one
  *       PG_timestep, printed as "build <s> mg <s> pg <s>".
  *
- * Every array is written raw (native endianness) to <outdir>/<name>; the
- * 1-based reference arrays are written from index 1 unless noted.
+ * Every array is written raw (native endianness) to <outdir>/<name> (under
+ * MPI with several ranks, each rank's local arrays to <outdir>/<name>.r<rank>);
+ * the 1-based reference arrays are written from index 1 unless noted.
  */
 #include <math.h>
 #include <mpi.h>
@@ -35,12 +37,18 @@
 extern int Emergency_stop;
 
 static const char *g_out;
+static int g_rank = 0, g_nranks = 1;
 
+/* under MPI with several ranks every rank writes its own local arrays, the
+ * file name suffixed with .r<rank> */
 static void dump(const char *name, const void *p, size_t bytes)
 {
 	char path[1024];
 	FILE *fp;
-	snprintf(path, sizeof path, "%s/%s", g_out, name);
+	if(g_nranks > 1)
+		snprintf(path, sizeof path, "%s/%s.r%d", g_out, name, g_rank);
+	else
+		snprintf(path, sizeof path, "%s/%s", g_out, name);
 	fp = fopen(path, "wb");
 	if(!fp)
 	{
@@ -254,7 +262,8 @@ static void run_steps(struct All_variables *E, int nsteps)
 	char path[1024];
 	FILE *fp;
 	snprintf(path, sizeof path, "%s/steps.txt", g_out);
-	fp = fopen(path, "w");
+	/* the scalars are global (horizontal sums over ranks): rank 0 writes them */
+	fp = fopen(g_rank == 0 ? path : "/dev/null", "w");
 
 	general_stokes_solver(E);
 	process_temp_field(E, E->monitor.solution_cycles);
@@ -281,6 +290,13 @@ static void run_steps(struct All_variables *E, int nsteps)
 	}
 	fclose(fp);
 	dump_fields(E, "final");
+	/* the layer profiles of the .ave output after the last step: Have.T from
+	 * heat_flux (Process_buoyancy.c:93), Have.vrms and Have.Vi from averages
+	 * (Process_velocity.c:179-224) */
+	averages(E);
+	dump("final_aveT.f32", E->Have.T + 1, E->lmesh.noz * sizeof(float));
+	dump("final_avevrms.f32", E->Have.vrms + 1, E->lmesh.noz * sizeof(float));
+	dump("final_aveVi.f32", E->Have.Vi + 1, E->lmesh.noz * sizeof(float));
 }
 
 /* the reference's own routines, timed with its own clock (CPU_time0,
@@ -330,6 +346,8 @@ int main(int argc, char **argv)
 	MPI_Init(&argc, &argv);
 	MPI_Comm_rank(MPI_COMM_WORLD, &(E.parallel.me));
 	MPI_Comm_size(MPI_COMM_WORLD, &(E.parallel.nproc));
+	g_rank = E.parallel.me;
+	g_nranks = E.parallel.nproc;
 	E.monitor.solution_cycles = 0;
 	E.monitor.elapsed_time = 0;
 	E.advection.timestep = 0;

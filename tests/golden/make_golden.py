@@ -21,6 +21,9 @@ citcom_amd.deck and stores what the tests compare against:
   the case-1a deck under MPI (4 ranks), 20000 steps to its steady state:
   final T and V (assembled from the per-rank .temp/.velo files), Nu_top,
   Nu_bot and time from the top and bottom ranks' .ave files.
+* ``<deck>_ave.npz``     the layer profiles of the .ave output (Have.T,
+  Have.vrms, Have.Vi: heat_flux and averages) after 3 steps, with the final
+  T and V they were computed from.
 * ``busse2_tic.npz``     the initial temperature the reference ships with its
   example deck (examples/Busse1993/case2-tic.temp.0.0), as read by the
   restart reader.
@@ -160,6 +163,18 @@ def main(names):
         raise SystemExit(f"{HARNESS} missing: make -C oracle ref")
     with tempfile.TemporaryDirectory() as tmp:
         for name in names:
+            if name.endswith("_ave"):
+                # the layer profiles of averages after 3 steps (and the
+                # per-step scalars of the same run)
+                base = name[:-4]
+                d = run_harness(base, tmp, "run", "3")
+                fx = {k: refdump.load(d, f"final_ave{k}.f32") for k in ("T", "vrms", "Vi")}
+                for k in ("T", "V1", "V2", "V3"):
+                    fx[f"final_{k}"] = refdump.load(d, f"final_{k}.f32")
+                fx["steps"] = np.loadtxt(os.path.join(d, "steps.txt"), ndmin=2)
+                np.savez_compressed(os.path.join(HERE, f"{base.replace('@', '_at_')}_ave.npz"), **fx)
+                print("wrote", name)
+                continue
             if name == "busse1a_steady":
                 np.savez_compressed(os.path.join(HERE, "busse1a_steady.npz"), **steady_fixture(tmp))
                 print("wrote", name)

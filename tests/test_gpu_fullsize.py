@@ -1,7 +1,8 @@
 """The measured workloads at full size, where no CPU run of the reference or
-the restatement finishes inside a test: tdep128 (128x128x64 elements, 6
-levels, 1.08 M nodes) and layer256 (256x256x128 elements, 7 levels, 8.5 M
-nodes).  Checked through properties that hold at any size:
+the restatement finishes inside a test: tdep64 (64^3 elements, 5 levels,
+275 k nodes), tdep128 (128x128x64 elements, 6 levels, 1.08 M nodes) and
+layer256 (256x256x128 elements, 7 levels, 8.5 M nodes).  Checked through
+properties that hold at any size:
 
 - the stiffness of every level is symmetric (u.Kv = v.Ku on stripped random
   vectors, to the rounding of its FP32 coefficients) and positive (u.Ku > 0);
@@ -36,7 +37,7 @@ def start(name, mixed=0, piterations=None):
     return p, c
 
 
-@pytest.mark.parametrize("name", ["tdep128", "layer256"])
+@pytest.mark.parametrize("name", ["tdep64", "tdep128", "layer256"])
 def test_fullsize_operator_properties(name):
     say("deck", name, "operators")
     p, c = start(name)
@@ -67,7 +68,7 @@ def momentum_residual(c):
     return float(np.linalg.norm(r) / np.linalg.norm(F))
 
 
-@pytest.mark.parametrize("name,piterations", [("tdep128", None), ("layer256", 10)])
+@pytest.mark.parametrize("name,piterations", [("tdep64", None), ("tdep128", None), ("layer256", 10)])
 def test_fullsize_forms_agree(name, piterations):
     out = {}
     for mixed in (0, 1, 2):
@@ -91,3 +92,30 @@ def test_fullsize_forms_agree(name, piterations):
         eu, ep = rel_l2(out[mixed][0], out[0][0]), rel_l2(out[mixed][1], out[0][1])
         say(f"{name}: mixed={mixed} against plain: U {eu:.2e}, P {ep:.2e}")
         assert eu < bounds[mixed][0] and ep < bounds[mixed][1]
+
+
+def test_tdep64_trajectory_forms():
+    """The 64^3-element workload (Ra 1e6, contrast 1e4, seeded noise): 20
+    steps of the plain FP64 form and of the FP32 refinement form, each on its
+    own trajectory from the same state; dt and Nu_top per step and the final
+    fields against each other (the per-field error of the FP32 cycles)"""
+    runs = {}
+    for mixed in (0, 1):
+        p, c = start("tdep64", mixed)
+        c.general_stokes_solver()
+        rec = [c.timestep() for _ in range(20)]
+        runs[mixed] = {"dt": np.array([r["pg"]["timestep"] for r in rec]),
+                       "nu": np.array([r["Nu_top"] for r in rec]),
+                       "its": [r["stokes"]["iterations"] for r in rec],
+                       "U": c.download("U"), "P": c.download("P"), "T": c.download("T"), "vrms": c.vrms()}
+        c.close()
+    a, b = runs[0], runs[1]
+    e_dt = float(np.abs(b["dt"] / a["dt"] - 1).max())
+    e_nu = float(np.abs(b["nu"] / a["nu"] - 1).max())
+    eu, et = rel_l2(b["U"], a["U"]), rel_l2(b["T"], a["T"])
+    ep = rel_l2(b["P"] - b["P"].mean(), a["P"] - a["P"].mean())
+    say(f"tdep64 20 steps, FP32 refinement against plain FP64: dt {e_dt:.1e}, Nu_top {e_nu:.1e}, U {eu:.1e}, "
+        f"P {ep:.1e}, T {et:.1e}, vrms {abs(b['vrms'] / a['vrms'] - 1):.1e}; Uzawa its plain {a['its']}, "
+        f"mixed {b['its']}")
+    assert e_dt < 1e-5 and e_nu < 1e-5
+    assert et < 1e-6 and eu < 1e-5 and ep < 1e-3

@@ -10,7 +10,7 @@ tolerance (relative L2 < 1e-6 on the fields).
 import numpy as np
 import pytest
 
-from conftest import golden, problem, rel_l2
+from conftest import golden, initial_temperature, problem, rel_l2
 import refdump
 
 pytestmark = pytest.mark.gpu
@@ -319,3 +319,38 @@ def test_unsupported_problem_is_refused():
         Citcom(p)
     with pytest.raises(ChipError, match="mixed must be 0, 1 or 2"):
         Citcom(problem("tiny_tdep"), mixed=3)
+
+
+@pytest.mark.parametrize("name", ["tiny_tdep", "iso16@1e-8"])
+def test_averages_against_reference(name):
+    """the .ave layer profiles (Have.T from heat_flux, Have.vrms and Have.Vi
+    from averages, Process_velocity.c:179-224) on the device: from the
+    reference's own final state (its T and V uploaded, the viscosity formed
+    from T), against the profiles the reference computed from it after the
+    same 3-step run"""
+    from citcom_amd.hip import Citcom
+    p = problem(name)
+    tag = name.replace("@", "_at_")
+    ref = golden(f"{tag}_ave.npz")
+    c = Citcom(p)
+    c.upload("T", ref["final_T"])
+    c.upload("V", np.concatenate([ref["final_V1"], ref["final_V2"], ref["final_V3"]]))
+    c.get_system_viscosity()
+    got = c.averages()
+    for k in ("T", "vrms", "Vi"):
+        e = rel_l2(got[k], ref[k])
+        say(f"{name}: Have.{k} against the reference: {e:.1e}")
+        assert e < 1e-6, (k, e)
+    # the same profiles at the end of the GPU's own 3-step run
+    c.close()
+    c = Citcom(p)
+    c.initialize(initial_temperature(name, p))
+    c.general_stokes_solver()
+    for _ in range(3):
+        c.timestep()
+    got = c.averages()
+    for k in ("T", "vrms", "Vi"):
+        e = rel_l2(got[k], ref[k])
+        say(f"{name}: Have.{k} after the GPU's own 3 steps: {e:.1e}")
+        assert e < (1e-6 if "@" in name else 1e-5), (k, e)
+    c.close()
