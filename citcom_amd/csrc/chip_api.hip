@@ -918,7 +918,12 @@ int chip_solve_del2_u(chip_ctx *c, const double *F, double *d0, double acc, int 
 
 int chip_solve_constrained_flow_iterative(chip_ctx *c, chip_stokes_stats *st) {
     CHIP_NEED_CTX(c);
-    return uzawa(c, st);
+    if (c->warm_start) {   // the warm start's guess replaces the uploaded U, P
+        if (warm_start_guess(c)) return -1;
+        conform_velocity(S(c), c->L[c->levmax], c->U);
+    }
+    if (uzawa(c, st)) return -1;
+    return warm_finish(c);
 }
 
 int chip_PG_timestep(chip_ctx *c, chip_pg_stats *st) {

@@ -5,18 +5,34 @@
  * Linked with -Wl,--wrap=solve_constrained_flow_iterative
  *             -Wl,--wrap=PG_timestep
  *             -Wl,--wrap=construct_stiffness_B_matrix
+ *             -Wl,--wrap=get_system_viscosity
+ *             -Wl,--wrap=assemble_forces
  * the driver's call of solve_constrained_flow_iterative (Drive_solvers.c:130)
  * and the PG_timestep pointer installed as E->next_buoyancy_field
  * (Convection.c:93, called at Citcom.c:125) reach the functions below, which
  * hand E's arrays to the GPU through the C ABI of citcom_hip.h and write the
  * results back where the reference writes them.  The driver's host rebuild of
  * the operators (Drive_solvers.c:126), which only the replaced solver reads
- * after the first step, runs once; the device rebuilds them from the
- * uploaded viscosity.  __real_* stay linked.
+ * after the first step, runs once; the device rebuilds them.  Once the
+ * device holds the temperature (after the first PG_timestep), the viscosity
+ * (get_system_viscosity, Drive_solvers.c:124) and the buoyancy forces
+ * (assemble_forces, Drive_solvers.c:119) are formed on the device from it;
+ * the viscosity is copied back for the host's readers (topography,
+ * Topo_gravity.c:401; averages, Process_velocity.c:189), the forces stay on
+ * the device (only the solver reads E->F).  __real_* stay linked and run
+ * where the device cannot stand in (before the device holds the
+ * temperature, for another array than the finest-level viscosity, or for
+ * viscosity laws the device does not form).  averages itself
+ * (Process_velocity.c:179-224) cannot be redirected: its caller,
+ * process_new_velocity, is in the same translation unit, and --wrap only
+ * redirects calls between objects; it reads the V and viscosity written
+ * back here (chip_averages computes the same profiles on the device).
  *
  * CITCOM_HIP_FORM=mixed (or mixed16) in the environment selects the mixed
  * precision form (chip_problem.mixed = 1, 2); the default is the reference's
- * arithmetic.
+ * arithmetic.  CITCOM_HIP_WARM=extrap starts each Uzawa solve from the last
+ * two solutions extrapolated along the sequence of solves (chip option
+ * "warm_start" = 1); the default is the reference's start.
  *
  * Compiled against the reference's own headers with the reference's -D flags
  * (struct All_variables must have the driver's layout, global_defs.h).
@@ -37,6 +53,9 @@ double CPU_time0(void);	/* Profiling.c */
 static chip_ctx *ctx;
 static float *vpack;	/* V[d*nno + n] staging for the ABI */
 static int stokes_calls;
+static int t_on_device;	/* the device T is E->T (after a PG_timestep) */
+static int evi_on_device;	/* the device formed this step's viscosity */
+static int f_on_device;	/* the device formed this step's forces */
 
 static void fail(struct All_variables *E, const char *what)
 {
@@ -147,6 +166,16 @@ static void setup(struct All_variables *E)
 	}
 
 	CHECK(chip_create(&p, &ctx));
+	{
+		const char *warm = getenv("CITCOM_HIP_WARM");
+		if(warm && strcmp(warm, "extrap") == 0)
+			CHECK(chip_set_option(ctx, "warm_start", 1));
+		else if(warm && *warm && strcmp(warm, "previous") != 0)
+		{
+			fprintf(stderr, "citcom_hip: CITCOM_HIP_WARM=%s (previous or extrap)\n", warm);
+			parallel_process_termination();
+		}
+	}
 	free(xc);
 	free(yc);
 	free(zc);
@@ -183,11 +212,15 @@ void __wrap_solve_constrained_flow_iterative(struct All_variables *E)
 	 * construct_stiffness_B_matrix does on the host */
 	if(stokes_calls == 0 || E->viscosity.update_allowed)
 	{
-		CHECK(chip_upload(ctx, "EVI", -1, E->EVI[E->mesh.levmax] + 1, 8L * nel));
+		if(!evi_on_device)
+			CHECK(chip_upload(ctx, "EVI", -1, E->EVI[E->mesh.levmax] + 1, 8L * nel));
 		CHECK(chip_construct_stiffness_B_matrix(ctx));
 	}
+	evi_on_device = 0;
 	stokes_calls++;
-	CHECK(chip_upload(ctx, "F", 0, E->F, neq));
+	if(!f_on_device)
+		CHECK(chip_upload(ctx, "F", 0, E->F, neq));
+	f_on_device = 0;
 	CHECK(chip_upload(ctx, "U", 0, E->U, neq));
 	CHECK(chip_upload(ctx, "P", 0, E->P + 1, npno));
 	CHECK(chip_solve_constrained_flow_iterative(ctx, &st));
@@ -231,6 +264,7 @@ void __wrap_PG_timestep(struct All_variables *E)
 	CHECK(chip_download(ctx, "T", 0, E->T + 1, nno));
 	CHECK(chip_download(ctx, "Tdot", 0, E->Tdot + 1, nno));
 	CHECK(chip_download(ctx, "buoyancy", 0, E->buoyancy + 1, nno));
+	t_on_device = 1;
 
 	E->advection.timestep = st.timestep;
 	E->advection.dt_reduced = 1.0f;
@@ -241,4 +275,38 @@ void __wrap_PG_timestep(struct All_variables *E)
 	E->monitor.elapsed_time += E->advection.timestep;
 	E->advection.last_sub_iterations = 1;
 	E->control.keep_going = (E->monitor.solution_cycles < E->advection.max_timesteps) ? 1 : 0;
+}
+
+void __real_get_system_viscosity(struct All_variables *E, int propogate, float *evisc, float *visc);
+void __real_assemble_forces(struct All_variables *E, int penalty);
+
+/* Viscosity_structures.c:369-443: the finest level's Gauss-point viscosity
+ * from the device temperature (visc_from_T / visc_from_mat and the MIN/MAX
+ * clip, as chip_get_system_viscosity), copied back into evisc for the
+ * host's other readers */
+void __wrap_get_system_viscosity(struct All_variables *E, int propogate, float *evisc, float *visc)
+{
+	if(!ctx || !t_on_device || evisc != E->EVI[E->mesh.levmax] || E->viscosity.SMOOTH || E->viscosity.SDEPV ||
+	   E->viscosity.CDEPV || E->viscosity.BDEPV || E->control.KERNEL != 1)
+	{
+		__real_get_system_viscosity(E, propogate, evisc, visc);
+		return;
+	}
+	CHECK(chip_get_system_viscosity(ctx));
+	CHECK(chip_download(ctx, "EVI", -1, evisc + 1, 8L * E->lmesh.nel));
+	evi_on_device = 1;
+}
+
+/* Element_calculations.c:74-125: the buoyancy forces from the device
+ * buoyancy (the PG step left it there); E->F is read only by the solver,
+ * which finds them on the device */
+void __wrap_assemble_forces(struct All_variables *E, int penalty)
+{
+	if(!ctx || !t_on_device)
+	{
+		__real_assemble_forces(E, penalty);
+		return;
+	}
+	CHECK(chip_assemble_forces(ctx));
+	f_on_device = 1;
 }

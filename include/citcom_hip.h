@@ -133,7 +133,9 @@ int chip_apply(chip_ctx *c, const char *op, int lev, const double *in, const dou
 
 /* ---- drop-in #1: solve_constrained_flow_iterative ------------------------ */
 /* Stokes_flow_Incomp.c:50-90: Uzawa CG (solve_Ahat_p_fhat :295-497) on the
- * device U, P with right-hand side F, then v_from_vector (:530-552) into V. */
+ * device U, P with right-hand side F, then v_from_vector (:530-552) into V.
+ * With the option "warm_start" the device U, P are first replaced by the
+ * warm start's guess (and the velocity boundary values imposed again). */
 int chip_solve_constrained_flow_iterative(chip_ctx *c, chip_stokes_stats *st);
 /* solve_del2_u (General_matrix_functions.c:368-520) on host arrays, for
  * verification: d0 = K^-1 F to max(acc, accuracy*|F|) */
