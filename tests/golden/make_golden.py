@@ -24,6 +24,10 @@ citcom_amd.deck and stores what the tests compare against:
 * ``<deck>_ave.npz``     the layer profiles of the .ave output (Have.T,
   Have.vrms, Have.Vi: heat_flux and averages) after 3 steps, with the final
   T and V they were computed from.
+* ``tdep64_at_1e-9_run.npz``  the 64^3-element workload (5 levels, Ra 1e6,
+  contrast 1e4, seeded temperature) at accuracy 1e-9 under MPI (8 ranks):
+  the step-0 solve and the state after step 1, on every other node and
+  element, and the per-step scalars.
 * ``busse2_tic.npz``     the initial temperature the reference ships with its
   example deck (examples/Busse1993/case2-tic.temp.0.0), as read by the
   restart reader.
@@ -158,11 +162,71 @@ def read_steady(out: str, nsteps: int, nproc) -> dict:
             "time": np.float64(top.time), "nu": np.array([top.nu_top, bot.nu_bot])}
 
 
+def assemble_rank_dumps(d: str, tag: str, p, nproc) -> dict:
+    """Global U (FP64 [3 nno]), P (FP64 [nel]) and T (FP32 [nno]) from the
+    per-rank dumps of a harness run under MPI (<name>.r<rank>): nodes by
+    rank_blocks (shared face nodes must agree), elements by the same blocks
+    without their upper faces."""
+    from citcom_amd import outputs
+    nox, noy, noz = p.dims()
+    ranks = nproc[0] * nproc[1] * nproc[2]
+    T = outputs.assemble([refdump.load(d, f"{tag}_T.f32.r{r}").reshape(-1, 1) for r in range(ranks)],
+                         p.dims(), nproc)[:, 0]
+    U = outputs.assemble([refdump.load(d, f"{tag}_U.f64.r{r}").reshape(-1, 3) for r in range(ranks)],
+                         p.dims(), nproc).reshape(-1)
+    P = np.full((noy - 1, nox - 1, noz - 1), np.nan)
+    for me, sy, sx, sz in deck.rank_blocks(p.dims(), nproc):
+        blk = P[sy.start:sy.stop - 1, sx.start:sx.stop - 1, sz.start:sz.stop - 1]
+        blk[...] = refdump.load(d, f"{tag}_P.f64.r{me}").reshape(blk.shape)
+    assert not np.isnan(P).any()
+    return {"T": T, "U": U, "P": P.reshape(-1)}
+
+
+def tdep64_fixture(d: str, nproc=(2, 4, 1)) -> dict:
+    """The 64^3-element workload at accuracy 1e-9: the step-0 solve and step 1
+    of the reference (harness under MPI).  Stored on every other node and
+    element (each axis) in FP32 to stay small: U [33^3][3], P [32^3], and T
+    after step 1; the per-step (time, dt, Nu_top, Nu_bot)."""
+    p = deck.problem_from_deck({k: str(v) for k, v in deck.workload("tdep64@1e-9").items()})
+    nox, noy, noz = p.dims()
+    out = {"steps": np.loadtxt(os.path.join(d, "steps.txt"), ndmin=2)}
+    for tag in ("step0", "final"):
+        f = assemble_rank_dumps(d, tag, p, nproc)
+        out[f"{tag}_U2"] = f["U"].reshape(noy, nox, noz, 3)[::2, ::2, ::2].astype(np.float32).reshape(-1)
+        out[f"{tag}_P2"] = f["P"].reshape(noy - 1, nox - 1, noz - 1)[::2, ::2, ::2].astype(np.float32).reshape(-1)
+        if tag == "final":
+            out["final_T2"] = f["T"].reshape(noy, nox, noz)[::2, ::2, ::2].astype(np.float32).reshape(-1)
+    return out
+
+
+def run_tdep64_mpi(tmp: str, nproc=(2, 4, 1), piterations: int = 3000) -> str:
+    keys = deck.workload("tdep64@1e-9")
+    p = deck.problem_from_deck({k: str(v) for k, v in keys.items()})
+    deck.write_restart_temperature(os.path.join(tmp, "seed"), deck.seeded_temperature(p, seed=1), p.dims(), nproc)
+    keys.update({"datafile": os.path.join(tmp, "run"), "restart": 1, "restart_timesteps": 0,
+                 "oldfile": os.path.join(tmp, "seed"), "piterations": piterations,
+                 "nprocx": nproc[0], "nprocy": nproc[1], "nprocz": nproc[2]})
+    path = os.path.join(tmp, "tdep64.input")
+    deck.write_deck(path, keys)
+    out = os.path.join(tmp, "dump")
+    ranks = nproc[0] * nproc[1] * nproc[2]
+    subprocess.run([MPIEXEC, "-launcher", "fork", "-n", str(ranks), HARNESS, path, out, "run", "1"], check=True,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, cwd=tmp)
+    return out
+
+
 def main(names):
     if not os.path.exists(HARNESS):
         raise SystemExit(f"{HARNESS} missing: make -C oracle ref")
     with tempfile.TemporaryDirectory() as tmp:
         for name in names:
+            if name == "tdep64_at_1e-9":
+                # 8 MPI ranks, about 15 minutes on 8 cores (2258 + 2709 Uzawa
+                # iterations); TDEP64_DUMP names an existing dump directory
+                d = os.environ.get("TDEP64_DUMP") or run_tdep64_mpi(tmp)
+                np.savez_compressed(os.path.join(HERE, "tdep64_at_1e-9_run.npz"), **tdep64_fixture(d))
+                print("wrote", name)
+                continue
             if name.endswith("_ave"):
                 # the layer profiles of averages after 3 steps (and the
                 # per-step scalars of the same run)

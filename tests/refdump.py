@@ -3,6 +3,7 @@ conversion to the 0-based layout of the oracle and the HIP path."""
 from __future__ import annotations
 
 import os
+import re
 
 import numpy as np
 
@@ -11,7 +12,8 @@ _SUFFIX = {"f32": np.float32, "f64": np.float64, "i32": np.int32, "u32": np.uint
 
 def load(dirpath: str, name: str, lev: int | None = None) -> np.ndarray:
     fn = name if lev is None else f"{name}_lev{lev}"
-    dt = _SUFFIX[name.rsplit(".", 1)[1]]
+    base = re.sub(r"\.r\d+$", "", name)   # per-rank dumps of a run under MPI: <name>.r<rank>
+    dt = _SUFFIX[base.rsplit(".", 1)[1]]
     return np.fromfile(os.path.join(dirpath, fn), dtype=dt)
 
 
