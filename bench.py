@@ -8,17 +8,22 @@ multigrid velocity solve to accuracy 1e-6).
 
 Default workload: 128x128x64 elements in a 2x2x1 box, 6 multigrid levels,
 Ra = 1e7, eta = exp(-ln(1e4) T) clipped to [1e-4, 1], seeded initial
-temperature.  The step-0 Stokes solve and the warmup steps are untimed.
-The measured form is the mixed one (FP32 multigrid cycles inside FP64
-iterative refinement, to the same 1e-6 criterion); the plain FP64 form is
-timed beside it for the ratio.
+temperature.  Every form runs its own time loop from the same seeded state:
+the step-0 Stokes solve and `warmup` steps untimed, then `steps` timed
+steps, the same window for every form.  The value is the FP32+learned form
+(mixed16+learned: FP32 multigrid cycles inside FP64 iterative refinement,
+the fine levels' stiffness stored in FP16, and each Uzawa solve started
+from the learned U-Net's correction of the last two solutions' extrapolation,
+citcom_amd/weights/warm_unet.npz); the plain FP64 form (the reference
+method, previous-solution start) and the precision-only form are timed
+beside it on the same window, every solve to the same criterion (dp/p and
+dv/v below 1e-6, or the piterations cap, counted per form).
 
 Prints one JSON line: s/timestep (lower is better), the roofline of the
 dominant kernel (the 8-colour Gauss-Seidel pass) from HIP events on the
-library's stream, and a CPU baseline: the unmodified reference program under
-MPI on up to 16 host cores (or, where MPI or that build is absent, the
-reference's objects on one core, else the C restatement) timed on a bounded
-sample.
+library's stream, and a CPU baseline: one complete timestep of the
+unmodified reference program under MPI on up to 16 host cores (or, where MPI
+or that build is absent, an estimate from the C restatement on one core).
 """
 from __future__ import annotations
 
@@ -337,6 +342,45 @@ def same_state_solves(c, form):
     return out
 
 
+PEAK_BF16_TFLOPS = 2500.0   # MI355X dense BF16 MFMA
+
+
+def unet_mfma(p, reps: int = 10):
+    """MFMA throughput of the warm start's U-Net forward pass at the
+    workload's finest grid (one volume, as the time loop runs it): the
+    convolutions' multiply-adds (2 x voxels x 27 x ci x co per layer) over
+    the pass's time, HIP events on the stream the kernels run on"""
+    import torch
+
+    from citcom_amd import unet
+    from citcom_amd.learn import _p
+    L = unet._lib()
+    dims = p.dims()
+    nv = [int(np.prod(unet.level_dims(dims, l))) for l in range(4)]
+    level_of = [0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0, 0]
+    flops = sum(2.0 * nv[level_of[k]] * 27 * ci * co for k, (ci, co) in enumerate(unet.LAYERS))
+    w, b, _ = unet.load()
+    wd = torch.from_numpy(w.view(np.int16)).cuda()
+    bd = torch.from_numpy(b).cuda()
+    x = torch.zeros((1, nv[0], 16), dtype=torch.bfloat16, device="cuda").normal_()
+    out = torch.empty((1, nv[0], 16), dtype=torch.float32, device="cuda")
+    ws = torch.empty(L.cl_unet_workspace(1, *dims), dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    for _ in range(2):
+        L.cl_unet_forward(_p(x), 1, *dims, _p(wd), _p(bd), _p(out), _p(ws), stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        if L.cl_unet_forward(_p(x), 1, *dims, _p(wd), _p(bd), _p(out), _p(ws), stream):
+            raise RuntimeError("cl_unet_forward failed")
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    tf = flops / (ms * 1e-3) / 1e12
+    return {"forward_ms": ms, "gflop": flops / 1e9, "tflops": tf, "frac_of_bf16_dense_peak": tf / PEAK_BF16_TFLOPS,
+            "grid": list(dims)}
+
+
 HEADLINE = "mixed16+learned"
 
 
@@ -348,7 +392,7 @@ def main():
     ap.add_argument("--config", default="tdep128")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--form", choices=tuple(FORMS), default=HEADLINE, help="the form whose time is the value")
-    ap.add_argument("--compare", default="plain,mixed16",
+    ap.add_argument("--compare", default="plain,mixed16,mixed16+extrap",
                     help="forms timed beside it on the same window (comma-separated; '' for none)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="launch multigrid kernels one by one")
@@ -433,12 +477,29 @@ def main():
         # far apart they are after the same number of steps
         forms[args.form]["T_rel_diff_vs_plain_last"] = float(
             np.linalg.norm(main_run["T_final"] - r["T_final"]) / np.linalg.norm(r["T_final"]))
+    learned = None
+    if FORMS[args.form][1] == 2 or any(FORMS[f][1] == 2 for f in others):
+        from citcom_amd import unet
+        meta = unet.load()[2]
+        learned = {"network": f"U-Net, {meta['params']} parameters, BF16 MFMA convolutions (FP32 accumulation)",
+                   "trained_on": f"{meta['samples']} samples at {meta['dims']} nodes ({meta['base']} trajectories, "
+                                 f"Ra and contrast drawn from [1e5, 1e7] x [1e2, 1e5])",
+                   "data_s": meta["data_s"], "train_s": meta["train_s"], "epochs": meta["epochs"],
+                   "holdout_error_extrapolation": meta["holdout_extrapolation_score"],
+                   "holdout_error_network": meta["history"][-1].get("holdout"),
+                   "cost": "trained once, offline (python -m citcom_amd.train_warm); not in the timed window",
+                   "mfma": unet_mfma(p)}
+        log(f"U-Net forward at {p.dims()}: {learned['mfma']['forward_ms']:.2f} ms, "
+            f"{learned['mfma']['tflops']:.1f} TFLOP/s ({100 * learned['mfma']['frac_of_bf16_dense_peak']:.2f} % "
+            f"of the BF16 dense peak)")
     config = {"workload": f"{args.config}: {nox - 1}x{noy - 1}x{noz - 1} elements, {p.levels} MG levels, "
                           f"Ra={p.rayleigh:g}, {visc}, accuracy={p.accuracy:g}, piterations={p.p_iterations}",
               "form": args.form, "window": f"steps {args.warmup + 1}..{args.warmup + args.steps} of each form's own "
                                            f"trajectory from the same seeded state (step-0 solve and "
                                            f"{args.warmup} warmup steps untimed), every form on the same window",
               "forms": forms, "same_state_check": main_run["same_state"]}
+    if learned is not None:
+        config["learned_warm_start"] = learned
     for f, r in others.items():
         config[f"speedup_vs_{f}"] = r["s_per_step"] / s_per_step
         log(f"{f}: {r['s_per_step']:.3f} s/step, {r['uzawa_iterations_per_step']:.1f} Uzawa iterations; "

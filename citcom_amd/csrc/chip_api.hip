@@ -287,8 +287,60 @@ int multi_grid(chip_ctx *c, double *F, double *residual) {
     return 0;
 }
 
+// the captured multigrid graph holds the coarsest level's smoother choice
+int drop_mg_graph(chip_ctx *c) {
+    if (c->mg_graph) {
+        CHIP_CHECK(hipStreamSynchronize(S(c)));
+        CHIP_CHECK(hipGraphExecDestroy(c->mg_graph));
+        c->mg_graph = nullptr;
+    }
+    return 0;
+}
+
+// the multigrid loop of solve_del2_u: d0 += corrections until the residual
+// reaches acc; *residual in: r0, out: the last residual
+int mg_loop(chip_ctx *c, double *d0, const double *F, double acc, double r0, int *count, double *residual) {
+    const long neq = 3L * c->L[c->levmax].g.nno;
+    double res = r0, prev = r0;
+    int cnt = 0;
+    while (res > acc) {
+        if (multi_grid(c, c->sr, &res)) return -1;
+        if (!c->p.mixed) axpy_scalar(S(c), d0, c->L[c->levmax].vel, 1.0, neq);
+        cnt++;
+        // the plain multigrid is the fallback of the learned coarse
+        // correction: a call that does not reduce the residual (or gives a
+        // non-finite one) switches the correction off for the rest of the
+        // run and restarts this solve with the coarsest level's smoother
+        if (c->cnet_on && !(res < prev)) {
+            c->cnet_on = false;
+            c->cnet_fallbacks++;
+            if (drop_mg_graph(c)) return -1;
+            if (c->print_convergence)
+                std::fprintf(stderr, " *  learned coarse correction off: residual %.3e after %.3e\n", res, prev);
+            copy(S(c), c->sr, F, neq);
+            fill(S(c), d0, 0.0, neq);
+            res = prev = r0;
+            cnt = 0;
+            continue;
+        }
+        prev = res;
+        if (cnt > 10000) {
+            set_error("solve_del2_u: multigrid does not converge");
+            return -1;
+        }
+    }
+    *count = cnt;
+    *residual = res;
+    return 0;
+}
+
 // solve_del2_u (General_matrix_functions.c:368-520, multigrid branch):
-// d0 = K^-1 F; F is not modified
+// d0 = K^-1 F; F is not modified.  The first solve after the learned coarse
+// correction is switched on is a calibration: it runs with the coarsest
+// level's smoother, then again with the correction, and the correction
+// stays on only if its mean residual reduction per multigrid call is the
+// stronger; otherwise the plain solve's result is kept and the correction
+// switched off (so the run continues exactly as the plain one would).
 int solve_del2_u(chip_ctx *c, double *d0, const double *F, double acc, int *valid, int *count) {
     if (c->p.mixed && (d0 != c->u1 || F != c->Ah)) {
         set_error("solve_del2_u: the mixed form solves K u1 = Ah only");
@@ -301,39 +353,37 @@ int solve_del2_u(chip_ctx *c, double *d0, const double *F, double acc, int *vali
     dots(S(c), c, (int)neq, 1, c->sr, c->sr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, c->dots + D_TMP);
     if (sync_dots(c)) return -1;
     const double r0 = std::sqrt(c->h_dots[D_TMP] / neq);
-    double residual = r0;
     acc = std::max(acc, r0 * c->p.accuracy);
-    *valid = (residual < acc) ? 0 : 1;
+    *valid = (r0 < acc) ? 0 : 1;
+    double residual = r0;
     int cnt = 0;
-    double prev = residual;
-    while (residual > acc) {
-        if (multi_grid(c, c->sr, &residual)) return -1;
-        if (!c->p.mixed) axpy_scalar(S(c), d0, c->L[c->levmax].vel, 1.0, neq);
-        cnt++;
-        // the plain multigrid is the fallback of the learned coarse
-        // correction: a call that does not reduce the residual (or gives a
-        // non-finite one) switches the correction off for the rest of the
-        // run and restarts this solve with the coarsest level's smoother
-        if (c->cnet_on && !(residual < prev)) {
-            CHIP_CHECK(hipStreamSynchronize(S(c)));
+    if (c->cnet_on && !c->cnet_calibrated && r0 > acc) {
+        c->cnet_calibrated = true;
+        // plain first, its result kept in the finest level's scratch
+        c->cnet_on = false;
+        if (drop_mg_graph(c) || mg_loop(c, d0, F, acc, r0, &cnt, &residual)) return -1;
+        const double f_plain = std::pow(residual / r0, 1.0 / std::max(cnt, 1));
+        const int cnt_plain = cnt;
+        double *keep = c->cnet_keep;
+        copy(S(c), keep, d0, neq);
+        // then with the correction
+        c->cnet_on = true;
+        copy(S(c), c->sr, F, neq);
+        fill(S(c), d0, 0.0, neq);
+        if (drop_mg_graph(c) || mg_loop(c, d0, F, acc, r0, &cnt, &residual)) return -1;
+        const double f_net = std::pow(residual / r0, 1.0 / std::max(cnt, 1));
+        if (c->cnet_on && !(f_net < f_plain)) {   // not the stronger: back to the plain run's result
             c->cnet_on = false;
             c->cnet_fallbacks++;
-            if (c->mg_graph) {
-                CHIP_CHECK(hipGraphExecDestroy(c->mg_graph));
-                c->mg_graph = nullptr;
-            }
-            if (c->print_convergence)
-                std::fprintf(stderr, " *  learned coarse correction off: residual %.3e after %.3e\n", residual, prev);
-            copy(S(c), c->sr, F, neq);
-            fill(S(c), d0, 0.0, neq);
-            residual = prev = r0;
-            continue;
+            if (drop_mg_graph(c)) return -1;
+            copy(S(c), d0, keep, neq);
+            cnt = cnt_plain;
         }
-        prev = residual;
-        if (cnt > 10000) {
-            set_error("solve_del2_u: multigrid does not converge");
-            return -1;
-        }
+        if (c->print_convergence)
+            std::fprintf(stderr, " *  learned coarse correction calibration: reduction per multigrid call %.3e "
+                                 "(plain %.3e): %s\n", f_net, f_plain, c->cnet_on ? "kept" : "switched off");
+    } else if (mg_loop(c, d0, F, acc, r0, &cnt, &residual)) {
+        return -1;
     }
     if (c->print_convergence)
         std::fprintf(stderr, " *  residual (%03d) = %.3e from %.3e to %.3e\n", cnt, residual, r0, acc);
@@ -1138,6 +1188,7 @@ int chip_set_option(chip_ctx *c, const char *name, int value) {
         set_error("chip_set_option: coarse_net needs chip_set_coarse_net first");
         return -1;
     }
+    if (n == "coarse_net" && value) c->cnet_calibrated = false;   // the next solve calibrates it
     if (n == "small_levels" || n == "gs_wg" || n == "coarse_net") {
         (n == "small_levels" ? c->small_levels : n == "gs_wg" ? c->gs_wg : c->cnet_on) = value != 0;
         if (c->mg_graph) {   // the captured kernel sequence changes

@@ -133,8 +133,10 @@ struct chip_ctx {
     bool cnet_on = false;
     bool cnet_failed = false;   // a launch of the correction failed while enqueueing
     int cnet_fallbacks = 0;     // inner solves restarted without it (no residual reduction)
+    bool cnet_calibrated = false;  // the first solve with it compared it with the plain multigrid
     uint16_t *cnet_w = nullptr;
     float *cnet_b = nullptr, *cnet_ys = nullptr, *cnet_logeta = nullptr;
+    double *cnet_keep = nullptr;   // [3 nno] the calibration's plain result
     // warm start of the Uzawa solve (option "warm_start"): 0 the previous
     // solution as it is (the reference); 1 extrapolated along the sequence
     // of solves from the last two; 2 the same plus the learned correction

@@ -255,7 +255,7 @@ int alloc_coarse_net(chip_ctx *c) {
     const size_t nno = (size_t)c->L[c->levmin].g.nno;
     if (dev_alloc(c, &c->cnet_w, (size_t)CNET_MAX_LAYERS * 16 * 27 * 16) ||
         dev_alloc(c, &c->cnet_b, (size_t)CNET_MAX_LAYERS * 16) || dev_alloc(c, &c->cnet_ys, 3) ||
-        dev_alloc(c, &c->cnet_logeta, nno))
+        dev_alloc(c, &c->cnet_logeta, nno) || dev_alloc(c, &c->cnet_keep, 3 * (size_t)c->L[c->levmax].g.nno))
         return -1;
     return 0;
 }

@@ -223,11 +223,16 @@ int chip_set_option(chip_ctx *c, const char *name, int value);
  * device; the inner solve keeps iterating to its own residual criterion. */
 int chip_set_coarse_net(chip_ctx *c, int layers, const uint16_t *weights, const float *bias,
                         const float *out_scale);
-/* The plain multigrid is its fallback: an inner solve (solve_del2_u) whose
- * multigrid call with the correction does not reduce the residual switches
- * the correction off for the rest of the context's life and restarts with
- * the coarsest level's smoother.  on: whether it is still in use;
- * fallbacks: how many times that happened. */
+/* The plain multigrid is its fallback.  The first inner solve
+ * (solve_del2_u) after the option "coarse_net" is set to 1 runs with the
+ * coarsest level's smoother and again with the correction; the correction
+ * stays on only if its mean residual reduction per multigrid call is the
+ * stronger, else the plain solve's result is kept and the correction is
+ * switched off.  Later, an inner solve whose multigrid call with the
+ * correction does not reduce the residual switches it off for the rest of
+ * the context's life and restarts with the coarsest level's smoother.  on:
+ * whether it is still in use; fallbacks: how many times it was switched
+ * off. */
 int chip_coarse_net_state(chip_ctx *c, int *on, int *fallbacks);
 
 /* ---- learned warm start (no counterpart in the reference) ----------------

@@ -354,3 +354,17 @@ def test_averages_against_reference(name):
         say(f"{name}: Have.{k} after the GPU's own 3 steps: {e:.1e}")
         assert e < (1e-6 if "@" in name else 1e-5), (k, e)
     c.close()
+
+
+def test_apply_refuses_short_inputs():
+    """chip_apply copies a fixed number of doubles from its host inputs: the
+    Python mirror refuses missing or short arrays before the call"""
+    from citcom_amd.hip import ChipError, Citcom
+    p = problem("tiny")
+    c = Citcom(p)
+    n3 = 3 * c.nno()
+    for op, kw in (("gs0", {}), ("gs1", {"x": np.zeros(n3)}), ("matvec", {"x": np.zeros(n3 - 1)}),
+                   ("grad", {"x": np.zeros(c.nel() + 1)}), ("coarse", {"x2": np.zeros(5)})):
+        with pytest.raises(ChipError, match="needs"):
+            c.apply(op, c.levmax if op != "coarse" else 0, **kw)
+    c.close()
