@@ -317,6 +317,87 @@ __global__ void __launch_bounds__(256) k_matvec(Geo g, const KT *__restrict__ K,
     }
 }
 
+// matvec of large levels with three wavefronts per 64-node tile: wavefront
+// p loads y-plane p's 81 coefficients (coalesced 256-byte runs over the
+// tile) and 27 vector entries and forms its 27 block terms (the expression
+// of row_product_t); planes 1 and 2 hand their terms to wavefront 0 through
+// LDS, which adds all 27 terms in row_product_t's sequential order, so the
+// result has the same bits as k_matvec's while three planes' loads are in
+// flight at once.
+template <typename V, typename KT>
+__global__ void __launch_bounds__(192) k_matvec3(Geo g, const KT *__restrict__ K, const unsigned *__restrict__ NODE,
+                                                 const V *__restrict__ u, V *__restrict__ Au,
+                                                 const V *__restrict__ rhs, V *__restrict__ res,
+                                                 const V *__restrict__ BIs) {
+    __shared__ V Ts[2][27][64];   // [plane - 1][block * 3 + row][lane]
+    const int lane = threadIdx.x & 63, P = threadIdx.x >> 6;
+    const int q = xcd_tile(blockIdx.x, gridDim.x) * 64 + lane;
+    const int col = colour_valid(g, q);
+    int k = 0, i = 0, j = 0;
+    if (col >= 0) colour_kij(g, col, q, k, i, j);
+    const int self = nid(g, k, i, j);
+    V T[27];
+    if (col >= 0) {
+        const KT *kp = K + ktile(q);
+        const int dy = P - 1;
+        const bool oy = (unsigned)(j + dy) < (unsigned)g.noy;
+        float kv[81];
+        V uv[27];
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) {
+            const bool ox = oy && (unsigned)(i + dx) < (unsigned)g.nox;
+#pragma unroll
+            for (int dz = -1; dz <= 1; dz++) {
+                const int b = (dz + 1) + 3 * (dx + 1);
+                const bool ok = ox && (unsigned)(k + dz) < (unsigned)g.noz;
+                const int m = ok ? self + dz + g.noz * (dx + g.nox * dy) : self;
+                uv[3 * b] = u[3 * m];
+                uv[3 * b + 1] = u[3 * m + 1];
+                uv[3 * b + 2] = u[3 * m + 2];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 81; c++) kv[c] = (float)kp[(81 * P + c) * 64];
+#pragma unroll
+        for (int b = 0; b < 9; b++)
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+                T[3 * b + r] = kv[9 * b + 3 * r] * uv[3 * b] + kv[9 * b + 3 * r + 1] * uv[3 * b + 1] +
+                               kv[9 * b + 3 * r + 2] * uv[3 * b + 2];
+        if (P > 0)
+#pragma unroll
+            for (int t = 0; t < 27; t++) Ts[P - 1][t][lane] = T[t];
+    }
+    __syncthreads();
+    if (P != 0 || col < 0) return;
+    V out[3] = {V(0), V(0), V(0)};
+#pragma unroll
+    for (int b = 0; b < 9; b++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) out[r] += T[3 * b + r];
+#pragma unroll
+    for (int pl = 0; pl < 2; pl++)
+#pragma unroll
+        for (int b = 0; b < 9; b++)
+#pragma unroll
+            for (int r = 0; r < 3; r++) out[r] += Ts[pl][3 * b + r][lane];
+    const int n = self;
+    if (BIs) {  // rows stored scaled by BI: K d = (BI K) d / BI
+        out[0] = out[0] / BIs[3 * n];
+        out[1] = out[1] / BIs[3 * n + 1];
+        out[2] = out[2] / BIs[3 * n + 2];
+    }
+    strip3(NODE[n], out);
+    Au[3 * n] = out[0];
+    Au[3 * n + 1] = out[1];
+    Au[3 * n + 2] = out[2];
+    if (res) {
+        res[3 * n] = rhs[3 * n] - out[0];
+        res[3 * n + 1] = rhs[3 * n + 1] - out[1];
+        res[3 * n + 2] = rhs[3 * n + 2] - out[2];
+    }
+}
+
 // The same product with all three planes' coefficients and vector entries
 // loaded before the first addition (one memory round trip instead of
 // three; the sums keep row_product's order).  For small levels, whose
@@ -400,27 +481,27 @@ void matvec(hipStream_t s, const Level &L, const V *u, V *Au, const V *rhs, V *r
             hipLaunchKernelGGL(kern, dim3(nblk(L.g.pitch, 64)), dim3(64), 0, s, L.g, K, L.NODE, u, Au, rhs, res);
         return;
     }
+    const int tiles = L.g.pitch / 64;   // the colour index space is whole 64-node tiles
     if constexpr (sizeof(V) == 4) {
         if (L.Kh) {
             const _Float16 *Kh = (const _Float16 *)L.Kh;
             const V *BIs = (const V *)L.BIf;
-            auto kern = k_matvec<V, _Float16>;
+            auto kern = k_matvec3<V, _Float16>;
             if (e0)
-                hipExtLaunchKernelGGL(kern, dim3(nblk(L.g.pitch)), dim3(256), 0, s, e0, e1, 0, L.g, Kh, L.NODE, u, Au,
-                                      rhs, res, BIs);
+                hipExtLaunchKernelGGL(kern, dim3(tiles), dim3(192), 0, s, e0, e1, 0, L.g, Kh, L.NODE, u, Au, rhs, res,
+                                      BIs);
             else
-                hipLaunchKernelGGL(kern, dim3(nblk(L.g.pitch)), dim3(256), 0, s, L.g, Kh, L.NODE, u, Au, rhs, res, BIs);
+                hipLaunchKernelGGL(kern, dim3(tiles), dim3(192), 0, s, L.g, Kh, L.NODE, u, Au, rhs, res, BIs);
             return;
         }
     }
     const V *none = nullptr;
     const float *K = L.K;
-    auto kern = k_matvec<V, float>;
+    auto kern = k_matvec3<V, float>;
     if (e0)
-        hipExtLaunchKernelGGL(kern, dim3(nblk(L.g.pitch)), dim3(256), 0, s, e0, e1, 0, L.g, K, L.NODE, u, Au, rhs, res,
-                              none);
+        hipExtLaunchKernelGGL(kern, dim3(tiles), dim3(192), 0, s, e0, e1, 0, L.g, K, L.NODE, u, Au, rhs, res, none);
     else
-        hipLaunchKernelGGL(kern, dim3(nblk(L.g.pitch)), dim3(256), 0, s, L.g, K, L.NODE, u, Au, rhs, res, none);
+        hipLaunchKernelGGL(kern, dim3(tiles), dim3(192), 0, s, L.g, K, L.NODE, u, Au, rhs, res, none);
 }
 template void matvec<double>(hipStream_t, const Level &, const double *, double *, const double *, double *,
                              hipEvent_t, hipEvent_t);

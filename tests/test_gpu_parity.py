@@ -368,3 +368,29 @@ def test_apply_refuses_short_inputs():
         with pytest.raises(ChipError, match="needs"):
             c.apply(op, c.levmax if op != "coarse" else 0, **kw)
     c.close()
+
+
+def test_large_level_operators_bitwise(pyoracle):
+    """the kernels of levels above 20000 nodes (the three-wavefront matvec,
+    the colour passes) against the restatement, bit for bit, on the 64^3
+    workload's two finest levels (274625 and 35937 nodes)"""
+    from citcom_amd import deck
+    from citcom_amd.hip import Citcom
+    p = problem("tdep64")
+    T = deck.seeded_temperature(p, seed=1)
+    o = pyoracle.Oracle(p, smoother=1)
+    o.set_viscosity(o.visc_from_T(T))
+    c = Citcom(p)
+    c.upload("T", T)
+    c.get_system_viscosity()
+    c.construct_stiffness_B_matrix()
+    rng = np.random.default_rng(5)
+    for lev in (p.levmax, p.levmax - 1):
+        n3 = 3 * c.nno(lev)
+        u = o.strip(lev, rng.standard_normal(n3))
+        assert np.array_equal(c.apply("matvec", lev, u), o.matvec(lev, u)), lev
+        F = o.strip(lev, rng.standard_normal(n3))
+        d, Ad = c.apply("gs0", lev, None, F, sweeps=1)
+        do, Ado = o.gauss_seidel(lev, np.zeros(n3), F, 1, 0)
+        assert np.array_equal(d, do) and np.array_equal(Ad, Ado), lev
+    c.close()
