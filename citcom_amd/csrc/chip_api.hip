@@ -569,12 +569,11 @@ int warm_start_guess(chip_ctx *c) {
     c->wscale[6] = c->hist;
     if (!net && !rec) return 0;
     // features: the temperature's departure from its own extrapolation, the
-    // pressures on the node grid, log10 viscosity (EVI is this step's), and
-    // their scales
+    // nodal pressures, log10 viscosity (EVI is this step's), and their scales
     warm_dT(q, c->T, c->Th0, c->Th1, nno, c->wn_dT);
-    e2c(q, L, c->P, nullptr, c->wn_pn);
-    e2c(q, L, c->P, c->Pprev, c->wn_dpn);
-    e2c(q, L, c->D2p, nullptr, c->wn_d2pn);
+    e2n(q, L, c->P, nullptr, c->wn_pn);
+    e2n(q, L, c->P, c->Pprev, c->wn_dpn);
+    e2n(q, L, c->D2p, nullptr, c->wn_d2pn);
     cnet_logeta(q, L, c->wn_logeta);
     dots(q, c, (int)neq, 2, (const double *)c->U, (const double *)c->U, (const double *)c->D2u,
          (const double *)c->D2u, nullptr, nullptr, nullptr, nullptr, c->dots + D_WS);
@@ -617,7 +616,7 @@ int warm_finish(chip_ctx *c) {
     hipStream_t q = S(c);
     const Level &L = c->L[c->levmax];
     if (c->feat_pending) {
-        e2c(q, L, c->P, c->Pe, c->wn_dpn);
+        e2n(q, L, c->P, c->Pe, c->wn_dpn);
         warm_target(q, L, c->U, c->Ue, c->wn_dpn, 1.0 / c->wscale[3], 1.0 / c->wscale[4], c->rec_y);
         c->feat_pending = false;
     }

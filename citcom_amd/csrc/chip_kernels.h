@@ -129,8 +129,6 @@ void fcopy(hipStream_t s, float *dst, const float *src, long n);
 // ---- the learned warm start's inputs and outputs (finest level)
 void warm_dT(hipStream_t s, const float *T, const float *Th0, const float *Th1, long n, float *dT);
 void e2n(hipStream_t s, const Level &L, const double *Pa, const double *Pb, float *out);
-// element values (Pa - Pb) on their lowest corner nodes, 0 on the upper faces' nodes
-void e2c(hipStream_t s, const Level &L, const double *Pa, const double *Pb, float *out);
 void warm_features(hipStream_t s, const Level &L, const float *T, const float *dT, const float *logeta,
                    const double *U, const double *Uprev, const double *D2, const float *pn, const float *dpn,
                    const float *d2pn, float inv_dT, float inv_u, float inv_p, float inv_2u, float inv_2p,

@@ -2528,30 +2528,11 @@ void e2n(hipStream_t s, const Level &L, const double *Pa, const double *Pb, floa
     hipLaunchKernelGGL(k_e2n, dim3(nblk(L.g.nno)), dim3(TPB), 0, s, L.g, Pa, Pb, out);
 }
 
-// an element field on the node grid without averaging: node (k, i, j) holds
-// the value (Pa - Pb) of element (ez, ex, ey) = (k, i, j), the element it is
-// the lowest corner of; 0 on the nodes of the three upper faces
-__global__ void k_e2c(Geo g, const double *__restrict__ Pa, const double *__restrict__ Pb, float *__restrict__ out) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= g.nno) return;
-    const int k = n % g.noz, i = (n / g.noz) % g.nox, j = n / (g.noz * g.nox);
-    float v = 0.f;
-    if (k < g.elz && i < g.elx && j < g.ely) {
-        const size_t e = (size_t)k + (size_t)g.elz * (i + (size_t)g.elx * j);
-        v = (float)(Pb ? Pa[e] - Pb[e] : Pa[e]);
-    }
-    out[n] = v;
-}
-void e2c(hipStream_t s, const Level &L, const double *Pa, const double *Pb, float *out) {
-    hipLaunchKernelGGL(k_e2c, dim3(nblk(L.g.nno)), dim3(TPB), 0, s, L.g, Pa, Pb, out);
-}
-
 // the network's input, 16 bf16 channels per node: T - 1/2, dT / s_dT,
-// log10(eta) / 2.5, the extrapolated U / s_u (3) and P / s_p, the last
-// step's increments U - U_prev / s_u (3) and P - P_prev / s_p, the second
-// differences of U / s_2u (3) and of P / s_2p, z (node index over noz - 1).
-// U_prev here is the last solution; the pressures (element values) sit on
-// the elements' lowest corner nodes (k_e2c).
+// log10(eta) / 2.5, the extrapolated U / s_u (3) and nodal P / s_p, the
+// last step's increments U - U_prev / s_u (3) and nodal P - P_prev / s_p,
+// the second differences of U / s_2u (3) and of the nodal P / s_2p, z
+// (node index over noz - 1).  U_prev here is the last solution.
 __global__ void k_warm_features(Geo g, const float *__restrict__ T, const float *__restrict__ dT,
                                 const float *__restrict__ logeta, const double *__restrict__ U,
                                 const double *__restrict__ Uprev, const double *__restrict__ D2,
@@ -2593,14 +2574,19 @@ void warm_features(hipStream_t s, const Level &L, const float *T, const float *d
 }
 
 // the network's correction: U += s_eu out[n][0..2] at every node; P[e] +=
-// s_ep out[n][3] at the element's lowest corner node n
+// s_ep (mean of out[.][3] over the element's eight nodes)
 __global__ void k_warm_apply(Geo g, const float *__restrict__ out, double s_eu, double s_ep, double *__restrict__ U,
                              double *__restrict__ P) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < 3L * g.nno) U[i] += s_eu * (double)out[(i / 3) * 16 + i % 3];
     if (i < g.nel) {
         const int ez = (int)(i % g.elz), ex = (int)((i / g.elz) % g.elx), ey = (int)(i / ((long)g.elz * g.elx));
-        P[i] += s_ep * (double)out[((size_t)ez + (size_t)g.noz * (ex + (size_t)g.nox * ey)) * 16 + 3];
+        double m = 0.0;
+        for (int q = 0; q < 8; q++) {
+            const int k = ez + (q & 1), ii = ex + ((q >> 1) & 1), j = ey + (q >> 2);
+            m += (double)out[((size_t)k + (size_t)g.noz * (ii + (size_t)g.nox * j)) * 16 + 3];
+        }
+        P[i] += s_ep * 0.125 * m;
     }
 }
 void warm_apply(hipStream_t s, const Level &L, const float *out, double s_eu, double s_ep, double *U, double *P) {
@@ -2608,8 +2594,7 @@ void warm_apply(hipStream_t s, const Level &L, const float *out, double s_eu, do
 }
 
 // the training target of one solve: (U - Ue) / s_eu per node (3 channels)
-// and (P - Pe) / s_ep on the elements' lowest corner nodes (dpn, k_e2c), Ue,
-// Pe the extrapolated guess
+// and the nodal mean of (P - Pe) / s_ep, Ue, Pe the extrapolated guess
 __global__ void k_warm_target(Geo g, const double *__restrict__ U, const double *__restrict__ Ue,
                               const float *__restrict__ dpn, double inv_eu, double inv_ep, float *__restrict__ y) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;

@@ -243,21 +243,19 @@ int chip_coarse_net_state(chip_ctx *c, int *on, int *fallbacks);
  * channels: T - 1/2; dT / s_dT with dT = T - (2 T_n - T_(n-1)) (T_n,
  * T_(n-1) the temperatures of the last two solves); log10(eta) / 2.5
  * (Gauss-point mean of the adjacent elements, averaged); the extrapolated
- * U / s_u (3) and P / s_p; the last increments U_n - U_(n-1) / s_u (3) and
- * of P / s_p; the second differences U_n - 2 U_(n-1) + U_(n-2) / s_2u (3)
- * and of P / s_2p; z (node index over noz - 1); s_* the RMS values.  An
- * element's pressure sits on its lowest corner node (0 on the nodes of the
- * upper faces).  Its output channels 0..2 times s_2u are added to U at every
- * node and channel 3 times s_2p, read at an element's lowest corner node, to
- * its P.  The solve then
+ * U / s_u (3) and its nodal P / s_p (mean of the adjacent elements); the
+ * last increments U_n - U_(n-1) / s_u (3) and of the nodal P / s_p; the
+ * second differences U_n - 2 U_(n-1) + U_(n-2) / s_2u (3) and of the nodal
+ * P / s_2p; z (node index over noz - 1); s_* the RMS values.  Its output
+ * channels 0..2 times s_2u are added to U at every node and channel 3 times
+ * s_2p, averaged over an element's eight nodes, to its P.  The solve then
  * runs to its own criterion, so accuracy is not traded.  Weights: bf16
  * [594432] in layer order ([co][27][ci] each), biases FP32 [528]. */
 int chip_set_warm_net(chip_ctx *c, const uint16_t *weights, long n_weights, const float *bias, long n_bias);
 /* Training samples: with the warm start on, every solve that has three
  * earlier ones writes the network's input (bf16 [nno][16]) to features and, after
  * the solve, the target (FP32 [nno][4]: (U - U_extrapolated) / s_2u per
- * node, (P - P_extrapolated) / s_2p of the element whose lowest corner the
- * node is, 0 on the upper faces) to targets; device
+ * node, the nodal mean of (P - P_extrapolated) / s_2p) to targets; device
  * pointers, both NULL to stop.  Samples start once three solves are kept. */
 int chip_warm_record(chip_ctx *c, void *features, void *targets);
 /* the last formed input's scales: s_u, s_p, s_dT, s_2u, s_2p, 0, the
