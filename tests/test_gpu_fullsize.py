@@ -127,9 +127,18 @@ def test_tdep64_against_reference():
     """The 64^3-element workload against the reference run at accuracy 1e-9
     (tests/golden/tdep64_at_1e-9_run.npz: the reference under MPI, 8 ranks;
     2258 and 2709 Uzawa iterations): the step-0 solve and step 1 (dt, Nu,
-    the fields after it), with both codes converged well below the 1e-6
-    field bar.  Fields on every other node and element, as the fixture keeps
-    them; the pressure up to its free-slip constant."""
+    the fields after it).  Fields on every other node and element, as the
+    fixture keeps them; the pressure up to its free-slip constant.
+
+    On 8 ranks the reference's Gauss-Seidel sweeps each subdomain and
+    exchanges the boundary sums (General_matrix_functions.c:1273, :1336), so
+    its iterates differ from the one-domain sweep, and the relative-increment
+    criterion (Stokes_flow_Incomp.c:442-443) stops the two loops at different
+    points of the slowly converging pressure modes.  Measured: this build's
+    step-0 pressure at 1e-9 is 2.2e-6 from its own solve run to 1e-11, and
+    the reference's is 7.9e-6 from that solve (8.7e-6 from ours at 1e-9);
+    the velocity agrees to 4.4e-8.  Hence the pressure bound of 2e-5 at
+    step 0 against 1e-6 for the velocity."""
     from citcom_amd import deck
     from citcom_amd.hip import Citcom
     g = golden("tdep64_at_1e-9_run.npz")
@@ -151,16 +160,24 @@ def test_tdep64_against_reference():
     eu, ep = rel_l2(u2, g["step0_U2"]), pdiff(p2, g["step0_P2"])
     say(f"tdep64 step 0 at 1e-9: {st['iterations']} Uzawa iterations (reference 2258), U {eu:.1e}, P {ep:.1e} "
         f"(P with its constant {rel_l2(p2, g['step0_P2']):.1e})")
-    assert eu < 1e-6 and ep < 1e-6
+    assert eu < 1e-6 and ep < 2e-5
     r = c.timestep()
+    # the harness records heat_flux after the step's Stokes solve
+    # (ref_harness.c, run_steps); timestep() returns the one the time loop
+    # computes before it (process_temp_field, Citcom.c:129), from the
+    # previous velocity
+    nu = c.heat_flux()
     ref = g["steps"][1]
     u2, p2 = sub(c.download("U"), c.download("P"))
     t2 = c.download("T").reshape(noy, nox, noz)[::2, ::2, ::2].reshape(-1)
-    e = {"dt": abs(r["pg"]["timestep"] / ref[2] - 1), "Nu_top": abs(r["Nu_top"] / ref[3] - 1),
-         "Nu_bot": abs(r["Nu_bot"] / ref[4] - 1), "U": rel_l2(u2, g["final_U2"]), "P": pdiff(p2, g["final_P2"]),
+    e = {"dt": abs(r["pg"]["timestep"] / ref[2] - 1), "Nu_top": abs(nu[0] / ref[3] - 1),
+         "Nu_bot": abs(nu[1] / ref[4] - 1), "U": rel_l2(u2, g["final_U2"]), "P": pdiff(p2, g["final_P2"]),
          "T": rel_l2(t2, g["final_T2"])}
     say(f"tdep64 step 1 at 1e-9: {r['stokes']['iterations']} Uzawa iterations (reference 2709); " +
         ", ".join(f"{k} {v:.1e}" for k, v in e.items()))
     c.close()
-    assert e["dt"] < 1e-6 and e["T"] < 1e-6 and e["U"] < 1e-6 and e["P"] < 1e-6
+    # measured: dt 5.5e-11, T 7.7e-10, U 9.6e-8, P 1.7e-6 (the step-0
+    # pressure spread carried into step 1's start), Nu_top 4e-9, Nu_bot 3.3e-6
+    # (its advective part is large: |Vz| reaches 1e4 in the bottom layers)
+    assert e["dt"] < 1e-6 and e["T"] < 1e-6 and e["U"] < 1e-6 and e["P"] < 1e-5
     assert e["Nu_top"] < 1e-5 and e["Nu_bot"] < 1e-5
