@@ -34,6 +34,7 @@ enum : int {
     D_INIT = 14,   // [14] r1.r1, [15] V.V
     D_WS = 16,     // [16..19] warm-start feature scales
     D_HF = 24,     // [24..27] heat-flux sums
+    D_XST = 28,    // [28] status of the one-XCD smoother launches (nonzero: one stopped early)
 };
 
 inline hipStream_t S(const chip_ctx *c) { return c->stream; }
@@ -160,6 +161,11 @@ void do_gs(chip_ctx *c, int lev, V *d, const V *F, V *Ad, int sweeps, int guess,
         }
     }
     const V *BI = vecs<V>(L).BI;
+    if (c->small_levels && c->xcd_gs && sweeps > 0 && gs_xcd_ok(L)) {
+        ProfGuard pg(c, 4, gs_bytes<V>(L) * L.g.nno * sweeps + mv_bytes<V>(L) * L.g.nno, lev);
+        gs_xcd(S(c), L, BI, d, F, Ad, rhs, res, sweeps, guess, c->dots + D_XST, pg.a, pg.b);
+        return;
+    }
     if (c->small_levels && c->gs_wg && gs_wg_ok<V>(L)) {
         ProfGuard pg(c, 3, gs_bytes<V>(L) * L.g.nno * sweeps + mv_bytes<V>(L) * L.g.nno, lev);
         gs_wg(S(c), L, BI, d, F, Ad, rhs, res, sweeps, guess, pg.a, pg.b);
@@ -256,6 +262,21 @@ void enqueue_multi_grid(chip_ctx *c, double *F) {
          nullptr, c->dots + D_TMP);
 }
 
+// a k_gs_xcd launch that stopped early leaves a nonzero status in
+// dots[D_XST] (read by the last sync_dots): reported as an error, after the
+// status and the levels' barrier counters are cleared for later launches
+int xcd_status(chip_ctx *c) {
+    const double st = c->h_dots[D_XST];
+    if (st == 0.0) return 0;
+    (void)hipMemsetAsync(c->dots + D_XST, 0, sizeof(double), c->stream);
+    for (int lev = c->levmin; lev <= c->levmax; lev++)
+        if (c->L[lev].xbar) (void)hipMemsetAsync(c->L[lev].xbar, 0, sizeof(XcdBar), c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    set_error(st == 2.0 ? "the one-XCD smoother's workgroups were not on one XCD (set option xcd_gs = 0)"
+                        : "a barrier of the one-XCD smoother passed its poll limit (set option xcd_gs = 0)");
+    return -1;
+}
+
 // F is always c->sr here, so one captured graph serves every call
 int multi_grid(chip_ctx *c, double *F, double *residual) {
     if (c->use_graphs && !c->prof.kernel && F == c->sr) {
@@ -282,7 +303,7 @@ int multi_grid(chip_ctx *c, double *F, double *residual) {
             return -1;
         }
     }
-    if (sync_dots(c)) return -1;
+    if (sync_dots(c) || xcd_status(c)) return -1;
     *residual = std::sqrt(c->h_dots[D_TMP] / (3.0 * c->L[c->levmax].g.nno));
     return 0;
 }
@@ -885,6 +906,7 @@ int chip_apply(chip_ctx *c, const char *op, int lev, const double *in, const dou
         do_gs(c, lev, a, b, d, arg, guess);
         CHIP_CHECK(down(out, a, nv));
         CHIP_CHECK(down(out2, d, nv));
+        if (sync_dots(c) || xcd_status(c)) return -1;
     } else if (o == "affine0" || o == "affine1") {
         if (!L.Ma) {
             set_error("affine: the level has no affine smoother form");
@@ -1054,7 +1076,7 @@ int chip_profile_enable(chip_ctx *c, const char *kernel, int on) {
         c->prof.kernel = 0;
         return 0;
     }
-    c->prof.kernel = (k == "gs_colour") ? 1 : (k == "matvec") ? 2 : (k == "gs_wg") ? 3 : 0;
+    c->prof.kernel = (k == "gs_colour") ? 1 : (k == "matvec") ? 2 : (k == "gs_wg") ? 3 : (k == "gs_xcd") ? 4 : 0;
     c->prof.launches = 0;
     c->prof.total_ms = c->prof.total_bytes = 0.0;
     if (!c->prof.kernel) {
@@ -1188,8 +1210,26 @@ int chip_set_option(chip_ctx *c, const char *name, int value) {
         return -1;
     }
     if (n == "coarse_net" && value) c->cnet_calibrated = false;   // the next solve calibrates it
-    if (n == "small_levels" || n == "gs_wg" || n == "coarse_net") {
-        (n == "small_levels" ? c->small_levels : n == "gs_wg" ? c->gs_wg : c->cnet_on) = value != 0;
+    if (n == "xcd_gs" && value) {   // relies on round-robin dispatch to the XCDs: checked here
+        for (int lev = c->levmin; lev <= c->levmax; lev++) {
+            if (!gs_xcd_ok(c->L[lev])) continue;
+            bool same = false;
+            if (xcd_colocated(c->stream, xcd_groups(c->L[lev]), c->xcd_ids, &same)) {
+                set_error("chip_set_option: XCD dispatch check failed");
+                return -1;
+            }
+            if (!same) {
+                set_error("chip_set_option: xcd_gs needs the workgroups of a launch dispatched round-robin to the "
+                          "XCDs, which this device does not do");
+                return -1;
+            }
+        }
+    }
+    if (n == "small_levels" || n == "gs_wg" || n == "coarse_net" || n == "xcd_gs") {
+        (n == "small_levels" ? c->small_levels
+         : n == "gs_wg"      ? c->gs_wg
+         : n == "xcd_gs"     ? c->xcd_gs
+                             : c->cnet_on) = value != 0;
         if (c->mg_graph) {   // the captured kernel sequence changes
             CHIP_CHECK(hipStreamSynchronize(c->stream));
             CHIP_CHECK(hipGraphExecDestroy(c->mg_graph));

@@ -41,6 +41,14 @@ struct Geo {
     int cz[8], cx[8]; // nodes of colour c along z and x
 };
 
+// arrival counter and XCD mask of a level's one-XCD smoother launches
+// (k_gs_xcd); zeroed once at set-up, never reset by the launches
+struct XcdBar {
+    unsigned long long count;
+    unsigned mask;
+    unsigned pad;
+};
+
 struct Level {
     Geo g;
     // operator
@@ -73,6 +81,8 @@ struct Level {
     // FP32 copies for the mixed-precision cycle
     float *velf = nullptr, *resf = nullptr, *flf = nullptr, *delf = nullptr, *AUf = nullptr;
     float *BIf = nullptr;
+    // one-XCD smoother state (levels gs_xcd_ok accepts)
+    XcdBar *xbar = nullptr;
 };
 
 struct Prof {
@@ -125,6 +135,8 @@ struct chip_ctx {
     bool use_graphs = true;  // replay multi_grid as a captured HIP graph
     bool small_levels = true;  // single-workgroup smoother on small levels
     bool gs_wg = true;         // ... including levels whose stiffness is read through L2
+    bool xcd_gs = false;       // whole smoother calls of mid-size levels on one XCD (k_gs_xcd)
+    unsigned *xcd_ids = nullptr;   // [16] set-up check of the XCD dispatch
     // learned coarse-level correction (chip_set_coarse_net): a 3-D CNN on the
     // coarsest level in place of its smoother call; BF16 weights
     // [layers][16][27][16], FP32 biases [layers][16], output scales [3],

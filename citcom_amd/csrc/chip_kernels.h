@@ -44,6 +44,19 @@ bool gs_wg_ok(const Level &L);
 template <typename V>
 void gs_wg(hipStream_t s, const Level &L, const V *BI, V *d, const V *F, V *Ad, const V *rhs, V *res, int sweeps,
            int guess, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+// a whole gauss_seidel call as one launch of a few workgroups on one XCD,
+// colour passes separated by barriers through that XCD's L2 (k_gs_xcd), then
+// Ad = K d (and res = rhs - Ad); a nonzero *status after the launch means it
+// stopped early (2: the workgroups were not on one XCD, 1: a barrier's poll
+// limit) and its results are not valid
+bool gs_xcd_ok(const Level &L);
+int xcd_groups(const Level &L);
+template <typename V>
+void gs_xcd(hipStream_t s, const Level &L, const V *BI, V *d, const V *F, V *Ad, const V *rhs, V *res, int sweeps,
+            int guess, double *status, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+// set-up check: do the active blocks of an 8 nwg-block launch share one XCD
+// (four launches)?  ids_dev holds 16 unsigned
+int xcd_colocated(hipStream_t s, int nwg, unsigned *ids_dev, bool *same);
 // level above the coarsest: its smoother call as d = Ma din + Mb F
 bool affine_ok(const Level &L);
 void affine_build(hipStream_t s, const Level &L, int sweeps);

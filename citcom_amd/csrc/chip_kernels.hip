@@ -233,16 +233,26 @@ __device__ __forceinline__ double lane_value(double v, int lane) {
     return __hiloint2double(hi, lo);
 }
 
+// a vector entry: a plain load, or (COHERENT) one that reads the copy in L2
+// rather than the CU's L1, for values other workgroups of the same kernel
+// have written (k_gs_xcd)
+template <bool COHERENT, typename V>
+__device__ __forceinline__ V vload(const V *__restrict__ p) {
+    if constexpr (COHERENT)
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        return *p;
+}
+
 // Plane P = dy+1 of the three rows of node (k, i, j), coefficients kp (its
-// tile column), into P[r]; arithmetic in the vector type V
-template <int P, typename V, typename KT>
-__device__ __forceinline__ void plane_rows(const Geo &g, const KT *__restrict__ kp, int k, int i, int j, int self,
-                                           const V *__restrict__ u, V out[3]) {
+// tile column), into P[r]; arithmetic in the vector type V.
+// The three parts of plane_rows: the neighbours' vector entries (and which
+// neighbours exist), the plane's 81 coefficients, and the tree sums
+template <int P, typename V, bool COHERENT>
+__device__ __forceinline__ void plane_gather(const Geo &g, int k, int i, int j, int self, const V *__restrict__ u,
+                                             V uv[27], bool okb[9]) {
     constexpr int DY = P - 1;
     const bool oy = (unsigned)(j + DY) < (unsigned)g.noy;
-    float kv[81];
-    V uv[27];
-    bool okb[9];
 #pragma unroll
     for (int dx = -1; dx <= 1; dx++) {
         const bool ox = oy && (unsigned)(i + dx) < (unsigned)g.nox;
@@ -252,13 +262,21 @@ __device__ __forceinline__ void plane_rows(const Geo &g, const KT *__restrict__ 
             const bool ok = ox && (unsigned)(k + dz) < (unsigned)g.noz;
             const int m = ok ? self + dz + g.noz * (dx + g.nox * DY) : self;
             okb[b] = ok;
-            uv[3 * b] = u[3 * m];
-            uv[3 * b + 1] = u[3 * m + 1];
-            uv[3 * b + 2] = u[3 * m + 2];
+            uv[3 * b] = vload<COHERENT>(u + 3 * m);
+            uv[3 * b + 1] = vload<COHERENT>(u + 3 * m + 1);
+            uv[3 * b + 2] = vload<COHERENT>(u + 3 * m + 2);
         }
     }
+}
+
+template <int P, typename KT>
+__device__ __forceinline__ void plane_coefs(const KT *__restrict__ kp, float kv[81]) {
 #pragma unroll
     for (int c = 0; c < 81; c++) kv[c] = (float)kp[(81 * P + c) * 64];
+}
+
+template <typename V>
+__device__ __forceinline__ void plane_sums(const float kv[81], const V uv[27], const bool okb[9], V out[3]) {
     PairSum16<V> ps[3];
 #define CHIP_TERM(B, R)                                                                                            \
     (okb[B] ? kv[9 * (B) + 3 * (R)] * uv[3 * (B)] + kv[9 * (B) + 3 * (R) + 1] * uv[3 * (B) + 1] +                  \
@@ -283,6 +301,17 @@ __device__ __forceinline__ void plane_rows(const Geo &g, const KT *__restrict__ 
         ps[r].template push<15>(V(0));
         out[r] = ps[r].total;
     }
+}
+
+template <int P, typename V, typename KT>
+__device__ __forceinline__ void plane_rows(const Geo &g, const KT *__restrict__ kp, int k, int i, int j, int self,
+                                           const V *__restrict__ u, V out[3]) {
+    float kv[81];
+    V uv[27];
+    bool okb[9];
+    plane_gather<P, V, false>(g, k, i, j, self, u, uv, okb);
+    plane_coefs<P, KT>(kp, kv);
+    plane_sums<V>(kv, uv, okb, out);
 }
 
 // matvec: n_assemble_del2_u (Element_calculations.c:552-621) with strip;
@@ -735,6 +764,11 @@ bool affine_ok(const Level &L) { return 3L * L.g.nno <= AFFINE_MAX_DOF; }
 constexpr int WG_SLOTS = 4;
 constexpr int WG_TPB = WG_SLOTS * 3 * 64;
 constexpr int WG_MAX_NNO = 1024;
+// largest level of the one-XCD smoother (k_gs_xcd, below): its FP32
+// stiffness (972 B a node) stays within about one XCD's 4 MB L2; polls per
+// barrier before a launch gives up
+constexpr int XCD_MAX_NNO = 6000;
+constexpr unsigned XCD_SPIN = 1u << 20;
 template <typename V>
 __global__ void __launch_bounds__(WG_TPB) k_gs_wg(Geo g, const float *__restrict__ K,
                                                   const unsigned *__restrict__ NODE, const V *__restrict__ BI, V *d,
@@ -886,6 +920,210 @@ template void gs_wg<double>(hipStream_t, const Level &, const double *, double *
                             const double *, double *, int, int, hipEvent_t, hipEvent_t);
 template void gs_wg<float>(hipStream_t, const Level &, const float *, float *, const float *, float *, const float *,
                            float *, int, int, hipEvent_t, hipEvent_t);
+
+// ---------------------------------------------------------------------------
+// A mid-size level (a few thousand nodes, a few MB of stiffness) runs a
+// whole gauss_seidel call as one launch of nwg workgroups that share one XCD.
+// Workgroups are dispatched to the XCDs round-robin, so blocks 8m (m = 0 ..
+// nwg-1) sit on one XCD and share its L2; the grid's other blocks return at
+// once.  Each colour pass is k_gs_colour's (same trees, same update
+// expression; workgroup m takes the colour's tiles m, m + nwg, ...), and a
+// barrier through that L2 stands in for each kernel boundary: every wave
+// waits until its stores are acknowledged, then one thread per workgroup
+// adds one to a 64-bit arrival counter and polls it (relaxed device-scope
+// atomics).  Vector entries that other workgroups write are read with
+// device-scope loads, which take the L2 copy rather than the CU's L1; the
+// stiffness, F and BI are read-only and stay cached.  The counter is never
+// reset: every launch adds a multiple of nwg, so a launch's first arrival
+// returns base + (arrivals before it) with base a multiple of nwg.  The first
+// barrier also checks that the workgroups share an XCD (the XCC_ID hardware
+// register); if they do not, or a poll runs past its limit, the launch stops
+// early and writes a nonzero status (2: not on one XCD, 1: poll limit) for
+// the host, which reports the multigrid call as failed.  Then Ad = K d in
+// row_product's order (and res = rhs - Ad), as after the per-colour passes.
+
+__device__ __forceinline__ unsigned xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | 20); }
+
+// polls until the counter reaches target; false past the poll limit
+__device__ __forceinline__ bool xcd_wait(unsigned long long *cnt, unsigned long long target) {
+    for (unsigned s = 0; __hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; s++)
+        if (s >= XCD_SPIN) return false;
+    return true;
+}
+
+template <typename V>
+__global__ void __launch_bounds__(192) k_gs_xcd(Geo g, const float *__restrict__ K, const unsigned *__restrict__ NODE,
+                                                const V *__restrict__ BI, V *d, const V *__restrict__ F,
+                                                V *__restrict__ Ad, const V *__restrict__ rhs, V *__restrict__ res,
+                                                int sweeps, int guess, XcdBar *bar, double *status) {
+    if (blockIdx.x & 7) return;
+    const int me = blockIdx.x >> 3, nwg = gridDim.x >> 3;
+    __shared__ V Ps[3][3][64];   // [plane][row][node]
+    __shared__ unsigned long long base_s;
+    __shared__ int stop_s;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (!guess)
+        for (int idx = me * 192 + tid; idx < 3 * g.nno; idx += nwg * 192) d[idx] = V(0);
+    // first barrier: the launch's base and the one-XCD check
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (tid == 0) {
+        const unsigned seen = __hip_atomic_fetch_or(&bar->mask, 1u << xcc_id(), __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+        (void)seen;
+        __builtin_amdgcn_s_waitcnt(0);   // the mask bit lands before the arrival
+        const unsigned long long old =
+            __hip_atomic_fetch_add(&bar->count, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long base = old - old % (unsigned long long)nwg;
+        base_s = base;
+        int stop = xcd_wait(&bar->count, base + nwg) ? 0 : 1;
+        if (!stop) {
+            const unsigned m = __hip_atomic_load(&bar->mask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (m & (m - 1)) stop = 2;
+        }
+        stop_s = stop;
+    }
+    __syncthreads();
+    if (stop_s) {
+        if (tid == 0) *status = (double)stop_s;
+        return;
+    }
+    unsigned long long target = base_s + nwg;
+    for (int cyc = 0; cyc < sweeps; cyc++)
+        for (int c = 0; c < 8; c++) {
+            const int tiles = (g.cn[c] + 63) >> 6;
+            for (int t = me; t < tiles; t += nwg) {   // uniform over the workgroup
+                const int q = g.cs[c] + t * 64 + lane;
+                const bool act = q < g.cs[c] + g.cn[c];
+                int k = 0, i = 0, j = 0;
+                if (act) colour_kij(g, c, q, k, i, j);
+                const int self = nid(g, k, i, j);
+                V Fr = V(0), Br = V(0), dr = V(0);
+                if (act) {
+                    Fr = F[3 * self + w];
+                    Br = BI[3 * self + w];
+                    dr = vload<true>(d + 3 * self + w);
+                    const float *kp = K + ktile(q);
+                    float kv[81];
+                    V uv[27], out[3];
+                    bool okb[9];
+                    if (w == 0) {
+                        plane_gather<0, V, true>(g, k, i, j, self, d, uv, okb);
+                        plane_coefs<0, float>(kp, kv);
+                    } else if (w == 1) {
+                        plane_gather<1, V, true>(g, k, i, j, self, d, uv, okb);
+                        plane_coefs<1, float>(kp, kv);
+                    } else {
+                        plane_gather<2, V, true>(g, k, i, j, self, d, uv, okb);
+                        plane_coefs<2, float>(kp, kv);
+                    }
+                    plane_sums<V>(kv, uv, okb, out);
+                    Ps[w][0][lane] = out[0];
+                    Ps[w][1][lane] = out[1];
+                    Ps[w][2][lane] = out[2];
+                }
+                __syncthreads();
+                if (act) {
+                    const int r = w;
+                    const V o = (Ps[0][r][lane] + Ps[1][r][lane]) + Ps[2][r][lane];
+                    const float tt = (float)((Fr - o) * Br);
+                    d[3 * self + r] = dr + tt;
+                }
+                __syncthreads();
+            }
+            // every wave's updates are acknowledged before the arrival
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            target += nwg;
+            if (tid == 0) {
+                __hip_atomic_fetch_add(&bar->count, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                stop_s = xcd_wait(&bar->count, target) ? 0 : 1;
+            }
+            __syncthreads();
+            if (stop_s) {
+                if (tid == 0) *status = 1.0;
+                return;
+            }
+        }
+    // every workgroup has read the mask (after the first barrier) and passed
+    // the last barrier: clear it for the next launch
+    if (me == 0 && tid == 0) __hip_atomic_store(&bar->mask, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int q = me * 192 + tid; q < g.pitch; q += nwg * 192) {
+        const int col = colour_valid(g, q);
+        if (col < 0) continue;
+        int k, i, j;
+        colour_kij(g, col, q, k, i, j);
+        const float *kp = K + ktile(q);
+        V out[3];
+        row_product_t<V>(
+            g, [kp](int ci) { return kp[ci * 64]; }, k, i, j, [d](int idx) { return vload<true>(d + idx); }, out);
+        const int n = nid(g, k, i, j);
+        strip3(NODE[n], out);
+        Ad[3 * n] = out[0];
+        Ad[3 * n + 1] = out[1];
+        Ad[3 * n + 2] = out[2];
+        if (res) {
+            res[3 * n] = rhs[3 * n] - out[0];
+            res[3 * n + 1] = rhs[3 * n + 1] - out[1];
+            res[3 * n + 2] = rhs[3 * n + 2] - out[2];
+        }
+    }
+}
+
+// workgroups of a level's k_gs_xcd launch: one per tile of its largest
+// colour (at most 16, which gs_xcd_ok checks)
+int xcd_groups(const Level &L) {
+    int t = 1;
+    for (int c = 0; c < 8; c++) t = std::max(t, (L.g.cn[c] + 63) / 64);
+    return std::min(t, 16);
+}
+
+bool gs_xcd_ok(const Level &L) {
+    int t = 1;
+    for (int c = 0; c < 8; c++) t = std::max(t, (L.g.cn[c] + 63) / 64);
+    return !L.Kh && L.xbar && L.g.nno > WG_MAX_NNO && L.g.nno <= XCD_MAX_NNO && t <= 16;
+}
+
+template <typename V>
+void gs_xcd(hipStream_t s, const Level &L, const V *BI, V *d, const V *F, V *Ad, const V *rhs, V *res, int sweeps,
+            int guess, double *status, hipEvent_t e0, hipEvent_t e1) {
+    auto kern = k_gs_xcd<V>;
+    const dim3 grid(8 * xcd_groups(L));
+    if (e0)
+        hipExtLaunchKernelGGL(kern, grid, dim3(192), 0, s, e0, e1, 0, L.g, (const float *)L.K, L.NODE, BI, d, F, Ad,
+                              rhs, res, sweeps, guess, L.xbar, status);
+    else
+        hipLaunchKernelGGL(kern, grid, dim3(192), 0, s, L.g, (const float *)L.K, L.NODE, BI, d, F, Ad, rhs, res,
+                           sweeps, guess, L.xbar, status);
+}
+template void gs_xcd<double>(hipStream_t, const Level &, const double *, double *, const double *, double *,
+                             const double *, double *, int, int, double *, hipEvent_t, hipEvent_t);
+template void gs_xcd<float>(hipStream_t, const Level &, const float *, float *, const float *, float *, const float *,
+                            float *, int, int, double *, hipEvent_t, hipEvent_t);
+
+// The XCC_ID of the active blocks of a k_gs_xcd-shaped launch (8 nwg
+// blocks, every eighth one active), for the set-up check that they share
+// one XCD.
+__global__ void k_xcd_ids(unsigned *ids) {
+    if ((blockIdx.x & 7) == 0 && threadIdx.x == 0) ids[blockIdx.x >> 3] = xcc_id();
+}
+
+int xcd_colocated(hipStream_t s, int nwg, unsigned *ids_dev, bool *same) {
+    *same = false;
+    if (nwg < 1 || nwg > 16) return -1;
+    unsigned h[16];
+    for (int rep = 0; rep < 4; rep++) {
+        hipLaunchKernelGGL(k_xcd_ids, dim3(8 * nwg), dim3(64), 0, s, ids_dev);
+        if (hipGetLastError() != hipSuccess) return -1;
+        if (hipMemcpyAsync(h, ids_dev, nwg * sizeof(unsigned), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return -1;
+        for (int m = 1; m < nwg; m++)
+            if (h[m] != h[0]) return 0;
+    }
+    *same = true;
+    return 0;
+}
 
 // ---------------------------------------------------------------------------
 // Coarsest level as a matrix.  The gauss_seidel call on the coarsest level of

@@ -425,6 +425,7 @@ int build_mesh(chip_ctx *c) {
                 dev_alloc(c, &L.Kh, (size_t)KROW * g.pitch))
                 return -1;
         }
+        if (dev_alloc(c, &L.xbar, 1)) return -1;
         // isoviscous until a viscosity is set
         std::vector<float> one((size_t)g.nel * 8, 1.0f);
         CHIP_CHECK(hipMemcpy(L.EVI, one.data(), one.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -455,7 +456,8 @@ int build_mesh(chip_ctx *c) {
         dev_alloc(c, &c->z0, nel) || dev_alloc(c, &c->z1, nel) || dev_alloc(c, &c->s1, nel) ||
         dev_alloc(c, &c->s2, nel) || dev_alloc(c, &c->Ah, neq) || dev_alloc(c, &c->u1, neq) ||
         dev_alloc(c, &c->sr, neq) || dev_alloc(c, &c->sd, neq) || dev_alloc(c, &c->partial, 4 * 1024 + 64) ||
-        dev_alloc(c, &c->counter, 4) || dev_alloc(c, &c->dots, 32) || dev_alloc(c, &c->minmax, 4))
+        dev_alloc(c, &c->counter, 4) || dev_alloc(c, &c->dots, 32) || dev_alloc(c, &c->minmax, 4) ||
+        dev_alloc(c, &c->xcd_ids, 16))
         return -1;
     CHIP_CHECK(hipHostMalloc((void **)&c->h_dots, 32 * sizeof(double), hipHostMallocDefault));
     CHIP_CHECK(hipHostMalloc((void **)&c->h_minmax, 4 * sizeof(unsigned), hipHostMallocDefault));

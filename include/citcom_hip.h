@@ -177,8 +177,9 @@ int chip_timestep(chip_ctx *c, chip_pg_stats *pg, chip_stokes_stats *st, float *
 int chip_general_stokes_solver(chip_ctx *c, chip_stokes_stats *st);
 
 /* ---- measurement ---------------------------------------------------------- */
-/* kernel: "gs_colour", "matvec" or "gs_wg" (the one-workgroup smoother call
- * of small levels).  While enabled, every launch of that kernel
+/* kernel: "gs_colour", "matvec", "gs_wg" (the one-workgroup smoother call
+ * of small levels) or "gs_xcd" (the one-XCD smoother call of mid-size
+ * levels).  While enabled, every launch of that kernel
  * is bracketed by HIP events on the work stream; read returns the number of
  * launches, their summed duration and the algorithmic bytes they moved. */
 int chip_profile_enable(chip_ctx *c, const char *kernel, int on);
@@ -193,6 +194,12 @@ long chip_device_bytes(chip_ctx *c);
  * "gs_wg": 1 (default) also runs a whole gauss_seidel call in one workgroup
  * on levels of at most 1024 nodes whose stiffness does not fit in LDS;
  * 0 launches one kernel per colour pass there; results are identical.
+ * "xcd_gs": 1 runs a whole gauss_seidel call on levels of 1025 to 6000
+ * nodes as one launch of a few workgroups on one XCD, with barriers through
+ * that XCD's L2 between the colour passes; 0 (default) launches one kernel
+ * per colour pass there; results are identical.  Setting 1 first checks
+ * that the device dispatches workgroups to its XCDs round-robin, and is
+ * refused where it does not.
  * "profile_level": >= 0 restricts chip_profile_enable to that level's
  * launches, -1 (default) profiles every level.
  * "coarse_net": 1 replaces the coarsest level's smoother call in multigrid

@@ -309,6 +309,63 @@ def test_one_workgroup_smoother_same_bits(name, mixed):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("mixed", [0, 1, 2])
+@pytest.mark.parametrize("name", ["tdep128", "tdep64"])
+def test_one_xcd_smoother_same_bits(name, mixed):
+    """A whole gauss_seidel call as one launch of a few workgroups on one XCD
+    (k_gs_xcd: tdep128's 2601-node level, tdep64's 4913-node level; colour
+    passes separated by barriers through that XCD's L2) gives the per-colour
+    launches' iterate bit for bit, in the FP64, FP32 and FP16-stiffness
+    forms: the step-0 solve (Uzawa capped at 20 iterations) and two
+    timesteps agree exactly with the kernel on and off."""
+    from citcom_amd import deck
+    from citcom_amd.hip import Citcom
+    runs = []
+    for on in (1, 0):
+        p = problem(name)
+        p.p_iterations = 20
+        c = Citcom(p, mixed=mixed)
+        c.set_option("xcd_gs", on)
+        c.initialize(deck.seeded_temperature(p, seed=5))
+        st = c.general_stokes_solver()
+        for _ in range(2):
+            r = c.timestep(with_nu=False)
+        runs.append((c.download("U"), c.download("P"), c.download("T"), st["mg_cycles"], r["stokes"]["mg_cycles"]))
+        c.close()
+    say(name, "mixed", mixed, "one-XCD smoother on/off: MG calls", runs[0][3:], runs[1][3:])
+    for a, b in zip(runs[0], runs[1]):
+        assert np.array_equal(a, b)
+
+
+def test_one_xcd_smoother_calls():
+    """Single smoother calls on tdep128's 2601-node level through the
+    one-XCD kernel, with and without an initial guess and with 3 and 2
+    sweeps in turn (the launches' barrier counter carries over from call to
+    call): d and K d equal the per-colour launches' bit for bit."""
+    from citcom_amd import deck
+    from citcom_amd.hip import Citcom
+    p = problem("tdep128")
+    c = Citcom(p)
+    c.initialize(deck.seeded_temperature(p, seed=2))
+    c.get_system_viscosity()
+    c.construct_stiffness_B_matrix()
+    lev = 2
+    assert c.nno(lev) == 2601
+    rng = np.random.default_rng(4)
+    n3 = 3 * c.nno(lev)
+    for call in range(6):
+        F = c.apply("strip", lev, rng.standard_normal(n3))
+        d0 = c.apply("strip", lev, rng.standard_normal(n3))
+        op, sweeps = ("gs1", 3) if call % 2 else ("gs0", 2 + call % 3)
+        out = []
+        for on in (1, 0):
+            c.set_option("xcd_gs", on)
+            out.append(c.apply(op, lev, d0 if op == "gs1" else None, F, sweeps=sweeps))
+        assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1]), (call, op, sweeps)
+    say("one-XCD smoother calls on the 2601-node level: 6 calls bit for bit")
+    c.close()
+
+
 def test_unsupported_problem_is_refused():
     """Viscosity laws other than rheol 0, 3 and 11, and unknown precision
     forms, are refused at setup, not computed as something else."""
