@@ -215,11 +215,12 @@ FORMS = {
     "mixed16": (2, 0, "as mixed, the two finest levels' stiffness rows scaled by BI and stored in FP16 for the cycles "
                       "(FP32 arithmetic; the FP64 residual keeps the reference's FP32 stiffness), previous-solution "
                       "start"),
-    "mixed16+extrap": (2, 1, "as mixed16, each Uzawa solve started from the last two solutions extrapolated along "
-                             "the sequence of solves, 2 x_n - x_(n-1) (no network)"),
+    "mixed16+extrap": (2, 1, "as mixed16, each Uzawa solve started from the last solutions extrapolated along the "
+                             "sequence of solves: 2 U_n - U_(n-1), P_n + 3/4 (P_n - P_(n-1)) + 1/4 (P_(n-1) - "
+                             "P_(n-2)) (no network)"),
     "mixed16+learned": (2, 2, "as mixed16+extrap, the extrapolation corrected by the learned U-Net (BF16 MFMA, "
                               "citcom_amd/weights/warm_unet.npz) before each Uzawa solve"),
-    "plain+extrap": (0, 1, "the plain FP64 form started from the extrapolation 2 x_n - x_(n-1) (no network)"),
+    "plain+extrap": (0, 1, "the plain FP64 form started from the extrapolation of mixed16+extrap (no network)"),
 }
 
 
@@ -302,8 +303,9 @@ def same_state_solves(c, form):
     """Two more steps of this form's trajectory; the second one's solve is
     repeated from the same state (same temperature, same previous solutions)
     with other initial guesses: the form's own, the plain extrapolation
-    2 x_n - x_(n-1), and the previous solution as it is (the reference's
-    start), in the same arithmetic.  The form's own solve is the first, so
+    (2 U_n - U_(n-1), P_n + 3/4 (P_n - P_(n-1)) + 1/4 (P_(n-1) - P_(n-2))),
+    and the previous solution as it is (the reference's start), in the same
+    arithmetic.  The form's own solve is the first, so
     the history it reads is intact."""
     import torch
 

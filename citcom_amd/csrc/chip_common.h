@@ -151,8 +151,8 @@ struct chip_ctx {
     double *cnet_keep = nullptr;   // [3 nno] the calibration's plain result
     // warm start of the Uzawa solve (option "warm_start"): 0 the previous
     // solution as it is (the reference); 1 extrapolated along the sequence
-    // of solves from the last two; 2 the same plus the learned correction
-    // (chip_set_warm_net)
+    // of solves from the last two or three (warm_start_guess); 2 the same
+    // plus the learned correction (chip_set_warm_net)
     int warm_start = 0;
     int hist = 0;                 // solutions kept since the option was set
     double *Uprev = nullptr, *Pprev = nullptr;    // the two solutions before the current one

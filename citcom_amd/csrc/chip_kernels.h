@@ -104,8 +104,9 @@ void scale_half(hipStream_t s, const Level &L);
 void add_f32(hipStream_t s, double *y, const float *x, long n);   // y += x
 void copy(hipStream_t s, double *dst, const double *src, long n);
 void axpy_scalar(hipStream_t s, double *y, const double *x, double a, long n);   // y += a x
-// x += r (x - h1); d2 (optional) = x - 2 h1 + h2; h2 = h1; h1 = old x
-void extrapolate(hipStream_t s, double *x, double *h1, double *h2, double r, double *d2, long n);
+// x += r1 (x - h1) + r2 (h1 - h2); d2 (optional) = x - 2 h1 + h2; h2 = h1;
+// h1 = old x
+void extrapolate(hipStream_t s, double *x, double *h1, double *h2, double r1, double r2, double *d2, long n);
 void sub(hipStream_t s, double *r, const double *a, const double *b, long n);    // r = a - b
 void sub(hipStream_t s, float *r, const float *a, const float *b, long n);
 // vel += alpha del, alpha = sc[1]/sc[0]; if res: res -= alpha AU

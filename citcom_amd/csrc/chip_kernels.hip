@@ -1626,20 +1626,20 @@ __global__ void k_axpy(double *y, const double *x, double a, long n) {
 void axpy_scalar(hipStream_t s, double *y, const double *x, double a, long n) { hipLaunchKernelGGL(k_axpy, dim3(nblk(n)), dim3(TPB), 0, s, y, x, a, n); }
 
 // the warm start's extrapolation along the sequence of solves: with x the
-// last solution and h1, h2 the two before it, x <- x + r (x - h1) (r = 1
-// once two solves are kept, 0 before); d2 (optional) <- x - 2 h1 + h2, the
-// second difference; then h2 <- h1, h1 <- the old x
-__global__ void k_extrapolate(double *x, double *h1, double *h2, double r, double *d2, long n) {
+// last solution and h1, h2 the two before it, x <- x + r1 (x - h1) +
+// r2 (h1 - h2); d2 (optional) <- x - 2 h1 + h2, the second difference; then
+// h2 <- h1, h1 <- the old x
+__global__ void k_extrapolate(double *x, double *h1, double *h2, double r1, double r2, double *d2, long n) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double xn = x[i], a = h1[i];
-    x[i] = xn + r * (xn - a);
-    if (d2) d2[i] = (xn - a) - (a - h2[i]);
+    const double xn = x[i], a = h1[i], b = h2[i];
+    x[i] = xn + r1 * (xn - a) + r2 * (a - b);
+    if (d2) d2[i] = (xn - a) - (a - b);
     h2[i] = a;
     h1[i] = xn;
 }
-void extrapolate(hipStream_t s, double *x, double *h1, double *h2, double r, double *d2, long n) {
-    hipLaunchKernelGGL(k_extrapolate, dim3(nblk(n)), dim3(TPB), 0, s, x, h1, h2, r, d2, n);
+void extrapolate(hipStream_t s, double *x, double *h1, double *h2, double r1, double r2, double *d2, long n) {
+    hipLaunchKernelGGL(k_extrapolate, dim3(nblk(n)), dim3(TPB), 0, s, x, h1, h2, r1, r2, d2, n);
 }
 
 template <typename V>

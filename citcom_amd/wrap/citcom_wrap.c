@@ -31,7 +31,7 @@
  * CITCOM_HIP_FORM=mixed (or mixed16) in the environment selects the mixed
  * precision form (chip_problem.mixed = 1, 2); the default is the reference's
  * arithmetic.  CITCOM_HIP_WARM=extrap starts each Uzawa solve from the last
- * two solutions extrapolated along the sequence of solves (chip option
+ * solutions extrapolated along the sequence of solves (chip option
  * "warm_start" = 1); the default is the reference's start.
  *
  * Compiled against the reference's own headers with the reference's -D flags

@@ -207,8 +207,10 @@ long chip_device_bytes(chip_ctx *c);
  * "warm_start": 0 (default) starts each Uzawa solve from the previous
  * solution as the last solve left it, as the reference does
  * (Stokes_flow_Incomp.c:354-366 start from E->U, E->P); 1 starts it from
- * the last two solutions extrapolated along the sequence of solves,
- * 2 U_n - U_(n-1) and 2 P_n - P_(n-1), once two solves have been kept; 2
+ * the last solutions extrapolated along the sequence of solves: the
+ * velocity 2 U_n - U_(n-1), the pressure with its increment averaged over
+ * the last two solves 3:1, P_n + 3/4 (P_n - P_(n-1)) + 1/4 (P_(n-1) -
+ * P_(n-2)) (linearly, 2 P_n - P_(n-1), while only two solves are kept); 2
  * adds the learned U-Net's correction of that extrapolation
  * (chip_set_warm_net) once three have been kept.  Setting the option
  * starts a new history.  The solve still iterates to the same criterion.
@@ -244,7 +246,8 @@ int chip_coarse_net_state(chip_ctx *c, int *on, int *fallbacks);
 
 /* ---- learned warm start (no counterpart in the reference) ----------------
  * The Uzawa solve's initial guess with the option "warm_start" = 2: the last
- * two solutions extrapolated along the sequence of solves, corrected by a
+ * solutions extrapolated along the sequence of solves (as with
+ * "warm_start" = 1), corrected by a
  * U-Net (BF16 MFMA convolutions, FP32 accumulation; cl_unet_forward of
  * citcom_learn.h) on the finest level's nodes.  Its input per node, 16 bf16
  * channels: T - 1/2; dT / s_dT with dT = T - (2 T_n - T_(n-1)) (T_n,

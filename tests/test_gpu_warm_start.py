@@ -5,8 +5,9 @@ nodes, the size the network was trained at) with a temperature seed and a
 Along one time loop run with the learned warm start, each step's Stokes solve
 is repeated from the same state (same temperature, same previous solutions)
 on a second context with the other initial guesses: the plain extrapolation
-2 x_n - x_(n-1) (the network's base, no network) and the previous solution
-as it is (the reference's start, Stokes_flow_Incomp.c:354-366).  Every
+(the network's base without the network: 2 U_n - U_(n-1), and P_n + 3/4
+(P_n - P_(n-1)) + 1/4 (P_(n-1) - P_(n-2))) and the previous solution as it
+is (the reference's start, Stokes_flow_Incomp.c:354-366).  Every
 solve runs to the same criterion (dp/p and dv/v below the accuracy, or the
 piterations cap); the test compares their Uzawa iterations and wall times,
 and the solutions they reach.
@@ -60,8 +61,9 @@ def test_learned_warm_start_gain():
         U, P = c.download("U"), c.download("P")
         sols = {}
         guesses = {"previous": (Un, Pn)}
-        if len(hist) >= 2:
-            guesses["extrapolation"] = (2 * Un - hist[-2][0], 2 * Pn - hist[-2][1])
+        if len(hist) >= 3:
+            (U1, P1), (U2, P2) = hist[-2], hist[-3]
+            guesses["extrapolation"] = (2 * Un - U1, Pn + 0.75 * (Pn - P1) + 0.25 * (P1 - P2))
         for tag, (Ug, Pg) in guesses.items():
             aux.upload("T", T)
             aux.thermal_buoyancy()
