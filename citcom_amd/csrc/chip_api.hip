@@ -125,7 +125,8 @@ void do_matvec(chip_ctx *c, int lev, const V *u, V *Au, const V *rhs = nullptr, 
 }
 
 // gauss_seidel (General_matrix_functions.c:1160-1361), 8-colour; returns Ad = K d
-// (with res: also res = rhs - Ad, formed by the final matvec)
+// (with res: also res = rhs - Ad, formed by the final matvec).  guess: 1
+// start from d, 0 from zero, 2 from zero with d already zero (no fill)
 template <typename V>
 void do_gs(chip_ctx *c, int lev, V *d, const V *F, V *Ad, int sweeps, int guess, const V *rhs = nullptr,
            V *res = nullptr);
@@ -142,7 +143,7 @@ void mg_gs(chip_ctx *c, int lev, V *d, const V *F, V *Ad, int sweeps, int guess,
         V *out;
         if constexpr (sizeof(V) == sizeof(double)) out = L.tmp;
         else out = L.tmpf;
-        affine_apply(S(c), L, guess ? (const V *)d : nullptr, F, out);
+        affine_apply(S(c), L, guess == 1 ? (const V *)d : nullptr, F, out);
         do_matvec(c, lev, (const V *)out, Ad, rhs, res);
         copy_v(S(c), d, (const V *)out, 3L * L.g.nno);
         return;
@@ -171,7 +172,7 @@ void do_gs(chip_ctx *c, int lev, V *d, const V *F, V *Ad, int sweeps, int guess,
         gs_wg(S(c), L, BI, d, F, Ad, rhs, res, sweeps, guess, pg.a, pg.b);
         return;
     }
-    if (!guess) fill(S(c), d, V(0), 3L * L.g.nno);
+    if (guess == 0) fill(S(c), d, V(0), 3L * L.g.nno);
     for (int cyc = 0; cyc < sweeps; cyc++)
         for (int col = 0; col < 8; col++) {
             ProfGuard pg(c, 1, gs_bytes<V>(L) * L.g.cn[col], lev);
@@ -220,9 +221,12 @@ void enqueue_cycles(chip_ctx *c, V *F) {
                 const LV<V> v = vecs<V>(L[dlev]);
                 const int cycles = (dlev == levmax) ? p.v_steps_high : p.down_heavy;
                 V *rhs = (dlev == lev) ? fl(dlev) : v.fl;
-                // res = rhs - AU formed by the smoother's final matvec
-                mg_gs(c, dlev, v.vel, (const V *)rhs, v.AU, cycles, dlev == lev, (const V *)rhs, v.res);
-                project(S(c), L[dlev], L[dlev - 1], (const V *)v.res, vecs<V>(L[dlev - 1]).fl);
+                // res = rhs - AU formed by the smoother's final matvec; below
+                // the top of the V-cycle the smoother starts from zero, which
+                // the restriction above it has written
+                mg_gs(c, dlev, v.vel, (const V *)rhs, v.AU, cycles, dlev == lev ? 1 : 2, (const V *)rhs, v.res);
+                project(S(c), L[dlev], L[dlev - 1], (const V *)v.res, vecs<V>(L[dlev - 1]).fl, true,
+                        vecs<V>(L[dlev - 1]).vel);
             }
             coarse_smooth(c, vecs<V>(L[levmin]).vel, (const V *)vecs<V>(L[levmin]).fl);
             for (int ulev = levmin + 1; ulev <= lev; ulev++) {
@@ -231,9 +235,15 @@ void enqueue_cycles(chip_ctx *c, V *F) {
                 const long n = 3L * L[ulev].g.nno;
                 interp(S(c), L[ulev - 1], L[ulev], (const V *)vecs<V>(L[ulev - 1]).vel, v.del);
                 mg_gs(c, ulev, v.del, (const V *)v.res, v.AU, cycles, 1);
+                V *resout = ulev == levmax ? v.res : nullptr;
+                if (line_search_ok(n)) {
+                    line_search(S(c), n, v.vel, (const V *)v.del, resout, (const V *)v.AU, (const V *)v.res,
+                                c->dots + D_MG);
+                    continue;
+                }
                 dots(S(c), c, (int)n, 2, (const V *)v.AU, (const V *)v.AU, v.AU, v.res, nullptr, nullptr, nullptr,
                      nullptr, c->dots + D_MG);
-                mg_update(S(c), v.vel, v.del, ulev == levmax ? v.res : nullptr, v.AU, c->dots + D_MG, n);
+                mg_update(S(c), v.vel, v.del, resout, v.AU, c->dots + D_MG, n);
             }
         }
     }

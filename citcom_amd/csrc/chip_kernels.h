@@ -66,8 +66,10 @@ void affine_apply(hipStream_t s, const Level &L, const V *din, const V *F, V *ou
 void coarse_build(hipStream_t s, const Level &L, int sweeps);
 template <typename V>
 void coarse_apply(hipStream_t s, const Level &L, const V *F, V *d);
+// (zero, if given, is a coarse vector set to zero in the same launch)
 template <typename V>
-void project(hipStream_t s, const Level &fine, const Level &coarse, const V *AU, V *AD, bool do_strip = true);
+void project(hipStream_t s, const Level &fine, const Level &coarse, const V *AU, V *AD, bool do_strip = true,
+             V *zero = nullptr);
 template <typename V>
 void interp(hipStream_t s, const Level &coarse, const Level &fine, const V *AD, V *AU, bool do_strip = true);
 void div(hipStream_t s, const Level &L, const double *U, double *divU);
@@ -110,6 +112,12 @@ void extrapolate(hipStream_t s, double *x, double *h1, double *h2, double r1, do
 void sub(hipStream_t s, double *r, const double *a, const double *b, long n);    // r = a - b
 void sub(hipStream_t s, float *r, const float *a, const float *b, long n);
 // vel += alpha del, alpha = sc[1]/sc[0]; if res: res -= alpha AU
+// multi_grid's line search on a short vector in one launch: sc = (AU.AU,
+// AU.res), vel += (sc[1] / sc[0]) del, resout -= the same times AU (if
+// resout); for n where line_search_ok
+bool line_search_ok(long n);
+template <typename V>
+void line_search(hipStream_t s, long n, V *vel, const V *del, V *resout, const V *AU, const V *res, double *sc);
 void mg_update(hipStream_t s, double *vel, const double *del, double *res, const double *AU, const double *sc, long n);
 void mg_update(hipStream_t s, float *vel, const float *del, float *res, const float *AU, const double *sc, long n);
 // Uzawa pieces (Stokes_flow_Incomp.c:393-464)

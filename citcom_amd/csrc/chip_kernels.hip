@@ -1270,7 +1270,8 @@ template void coarse_apply<float>(hipStream_t, const Level &, const float *, flo
 template <typename V>
 __global__ void __launch_bounds__(TPB) k_project(Geo gf, Geo gc, const float *__restrict__ TWW,
                                                  const float *__restrict__ MASS, const unsigned *__restrict__ NODE,
-                                                 const V *__restrict__ AU, V *__restrict__ AD, int do_strip) {
+                                                 const V *__restrict__ AU, V *__restrict__ AD, int do_strip,
+                                                 V *__restrict__ zero) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= gc.nno) return;
     int k, i, j;
@@ -1331,15 +1332,20 @@ __global__ void __launch_bounds__(TPB) k_project(Geo gf, Geo gc, const float *__
     AD[3 * n] = acc[0];
     AD[3 * n + 1] = acc[1];
     AD[3 * n + 2] = acc[2];
+    if (zero) {   // the coarse level's smoother starts from zero
+        zero[3 * n] = V(0);
+        zero[3 * n + 1] = V(0);
+        zero[3 * n + 2] = V(0);
+    }
 }
 
 template <typename V>
-void project(hipStream_t s, const Level &fine, const Level &coarse, const V *AU, V *AD, bool do_strip) {
+void project(hipStream_t s, const Level &fine, const Level &coarse, const V *AU, V *AD, bool do_strip, V *zero) {
     hipLaunchKernelGGL(k_project<V>, dim3(nblk(coarse.g.nno)), dim3(TPB), 0, s, fine.g, coarse.g, coarse.TWW,
-                       coarse.MASS, coarse.NODE, AU, AD, (int)do_strip);
+                       coarse.MASS, coarse.NODE, AU, AD, (int)do_strip, zero);
 }
-template void project<double>(hipStream_t, const Level &, const Level &, const double *, double *, bool);
-template void project<float>(hipStream_t, const Level &, const Level &, const float *, float *, bool);
+template void project<double>(hipStream_t, const Level &, const Level &, const double *, double *, bool, double *);
+template void project<float>(hipStream_t, const Level &, const Level &, const float *, float *, bool, float *);
 
 // interp_vector + un_inject_vector (Solver_multigrid.c:173-298, 581-634):
 // injection, then linear fills along x, z, y in the reference's order,
@@ -1581,6 +1587,53 @@ template void dots<double>(hipStream_t, chip_ctx *, int, int, const double *, co
                            const double *, const double *, const double *, const double *, const double *, double *);
 template void dots<float>(hipStream_t, chip_ctx *, int, int, const float *, const float *, const float *,
                           const float *, const float *, const float *, const float *, const float *, double *);
+
+// multi_grid's line search on a small level in one launch of one workgroup
+// (General_matrix_functions.c:626-638): sc = (AU.AU, AU.res), alpha =
+// sc[1] / sc[0], vel += alpha del and, with resout, resout -= alpha AU
+// (resout may be res: every read of res precedes the barrier).  In place of
+// dots + mg_update where the level's vectors are short enough that the two
+// launches, not the bytes, set the time: one CU streams about 100 GB/s, so
+// at 8192 entries (about 160 KB moved) the one launch takes 2 us.
+constexpr int LS_TPB = 1024;
+constexpr long LS_MAX_N = 8192;
+
+template <typename V>
+__global__ void __launch_bounds__(LS_TPB) k_line_search(long n, V *__restrict__ vel, const V *__restrict__ del,
+                                                        V *resout, const V *__restrict__ AU, const V *res,
+                                                        double *__restrict__ sc) {
+    __shared__ double sh[4 * 16];
+    __shared__ double alpha_s;
+    double v[2] = {0.0, 0.0};
+    for (long i = threadIdx.x; i < n; i += LS_TPB) {
+        const double a = (double)AU[i];
+        v[0] += a * a;
+        v[1] += a * (double)res[i];
+    }
+    block_sum<2>(v, sh);
+    if (threadIdx.x == 0) {
+        sc[0] = v[0];
+        sc[1] = v[1];
+        alpha_s = v[1] / v[0];
+    }
+    __syncthreads();
+    const V alpha = (V)alpha_s;
+    for (long i = threadIdx.x; i < n; i += LS_TPB) {
+        vel[i] += alpha * del[i];
+        if (resout) resout[i] -= alpha * AU[i];
+    }
+}
+
+bool line_search_ok(long n) { return n <= LS_MAX_N; }
+
+template <typename V>
+void line_search(hipStream_t s, long n, V *vel, const V *del, V *resout, const V *AU, const V *res, double *sc) {
+    hipLaunchKernelGGL(k_line_search<V>, dim3(1), dim3(LS_TPB), 0, s, n, vel, del, resout, AU, res, sc);
+}
+template void line_search<double>(hipStream_t, long, double *, const double *, double *, const double *,
+                                  const double *, double *);
+template void line_search<float>(hipStream_t, long, float *, const float *, float *, const float *, const float *,
+                                 double *);
 
 // ---------------------------------------------------------------------------
 // elementwise
