@@ -118,10 +118,11 @@ template <>
 LV<float> vecs<float>(const Level &L) { return {L.velf, L.resf, L.flf, L.delf, L.AUf, L.BIf}; }
 
 template <typename V>
-void do_matvec(chip_ctx *c, int lev, const V *u, V *Au, const V *rhs = nullptr, V *res = nullptr) {
+void do_matvec(chip_ctx *c, int lev, const V *u, V *Au, const V *rhs = nullptr, V *res = nullptr,
+               V *ucopy = nullptr) {
     const Level &L = c->L[lev];
     ProfGuard pg(c, 2, mv_bytes<V>(L) * L.g.nno, lev);
-    matvec(S(c), L, u, Au, rhs, res, pg.a, pg.b);
+    matvec(S(c), L, u, Au, rhs, res, pg.a, pg.b, ucopy);
 }
 
 // gauss_seidel (General_matrix_functions.c:1160-1361), 8-colour; returns Ad = K d
@@ -130,9 +131,6 @@ void do_matvec(chip_ctx *c, int lev, const V *u, V *Au, const V *rhs = nullptr, 
 template <typename V>
 void do_gs(chip_ctx *c, int lev, V *d, const V *F, V *Ad, int sweeps, int guess, const V *rhs = nullptr,
            V *res = nullptr);
-
-inline void copy_v(hipStream_t s, double *d, const double *a, long n) { copy(s, d, a, n); }
-inline void copy_v(hipStream_t s, float *d, const float *a, long n) { fcopy(s, d, a, n); }
 
 // a smoother call of multi_grid: the affine form where the level has one
 template <typename V>
@@ -144,8 +142,7 @@ void mg_gs(chip_ctx *c, int lev, V *d, const V *F, V *Ad, int sweeps, int guess,
         if constexpr (sizeof(V) == sizeof(double)) out = L.tmp;
         else out = L.tmpf;
         affine_apply(S(c), L, guess == 1 ? (const V *)d : nullptr, F, out);
-        do_matvec(c, lev, (const V *)out, Ad, rhs, res);
-        copy_v(S(c), d, (const V *)out, 3L * L.g.nno);
+        do_matvec(c, lev, (const V *)out, Ad, rhs, res, d);   // and d = out
         return;
     }
     do_gs(c, lev, d, F, Ad, sweeps, guess, rhs, res);

@@ -75,6 +75,7 @@ struct Level {
     // ([3nno][3nno] each, row-major), for affine_sweeps sweeps; scratch
     // vectors for the result in both precisions
     double *Ma = nullptr, *Mb = nullptr;
+    float *Maf = nullptr, *Mbf = nullptr;   // FP32 copies for the FP32 cycles (mixed forms)
     int affine_sweeps = 0;
     double *tmp = nullptr;
     float *tmpf = nullptr;

@@ -26,7 +26,7 @@ void build_bpi(hipStream_t s, const Level &L, int precond);
 // e0/e1 the launch records the kernel's own start and end time in them.
 template <typename V>
 void matvec(hipStream_t s, const Level &L, const V *u, V *Au, const V *rhs, V *res, hipEvent_t e0 = nullptr,
-            hipEvent_t e1 = nullptr);
+            hipEvent_t e1 = nullptr, V *ucopy = nullptr);
 // one colour pass of the node-block Gauss-Seidel (BI in the vector type)
 template <typename V>
 void gs_colour(hipStream_t s, const Level &L, const V *BI, int colour, V *d, const V *F, hipEvent_t e0 = nullptr,

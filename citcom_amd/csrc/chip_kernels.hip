@@ -478,7 +478,7 @@ template <typename V>
 __global__ void __launch_bounds__(64) k_matvec_small(Geo g, const float *__restrict__ K,
                                                      const unsigned *__restrict__ NODE, const V *__restrict__ u,
                                                      V *__restrict__ Au, const V *__restrict__ rhs,
-                                                     V *__restrict__ res) {
+                                                     V *__restrict__ res, V *__restrict__ ucopy) {
     const int q = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int col = colour_valid(g, q);
     if (col < 0) return;
@@ -496,19 +496,32 @@ __global__ void __launch_bounds__(64) k_matvec_small(Geo g, const float *__restr
         res[3 * n + 1] = rhs[3 * n + 1] - out[1];
         res[3 * n + 2] = rhs[3 * n + 2] - out[2];
     }
+    if (ucopy) {   // u copied to ucopy in the same launch
+        ucopy[3 * n] = u[3 * n];
+        ucopy[3 * n + 1] = u[3 * n + 1];
+        ucopy[3 * n + 2] = u[3 * n + 2];
+    }
 }
 
 template <typename V>
-void matvec(hipStream_t s, const Level &L, const V *u, V *Au, const V *rhs, V *res, hipEvent_t e0, hipEvent_t e1) {
+void matvec(hipStream_t s, const Level &L, const V *u, V *Au, const V *rhs, V *res, hipEvent_t e0, hipEvent_t e1,
+            V *ucopy) {
     if (!L.Kh && L.g.nno <= MV_SMALL_NNO) {
         auto kern = k_matvec_small<V>;
         const float *K = L.K;
         if (e0)
             hipExtLaunchKernelGGL(kern, dim3(nblk(L.g.pitch, 64)), dim3(64), 0, s, e0, e1, 0, L.g, K, L.NODE, u, Au,
-                                  rhs, res);
+                                  rhs, res, ucopy);
         else
-            hipLaunchKernelGGL(kern, dim3(nblk(L.g.pitch, 64)), dim3(64), 0, s, L.g, K, L.NODE, u, Au, rhs, res);
+            hipLaunchKernelGGL(kern, dim3(nblk(L.g.pitch, 64)), dim3(64), 0, s, L.g, K, L.NODE, u, Au, rhs, res,
+                               ucopy);
         return;
+    }
+    if (ucopy) {   // large levels: a copy launch of its own
+        if constexpr (sizeof(V) == 8)
+            copy(s, (double *)ucopy, (const double *)u, 3L * L.g.nno);
+        else
+            fcopy(s, (float *)ucopy, (const float *)u, 3L * L.g.nno);
     }
     const int tiles = L.g.pitch / 64;   // the colour index space is whole 64-node tiles
     if constexpr (sizeof(V) == 4) {
@@ -533,9 +546,9 @@ void matvec(hipStream_t s, const Level &L, const V *u, V *Au, const V *rhs, V *r
         hipLaunchKernelGGL(kern, dim3(tiles), dim3(192), 0, s, L.g, K, L.NODE, u, Au, rhs, res, none);
 }
 template void matvec<double>(hipStream_t, const Level &, const double *, double *, const double *, double *,
-                             hipEvent_t, hipEvent_t);
+                             hipEvent_t, hipEvent_t, double *);
 template void matvec<float>(hipStream_t, const Level &, const float *, float *, const float *, float *, hipEvent_t,
-                            hipEvent_t);
+                            hipEvent_t, float *);
 
 // Gauss-Seidel: gauss_seidel (General_matrix_functions.c:1160-1361) as eight
 // colour passes; the node-block update t = (F - (K d)_node) * BI, rounded to
@@ -1204,14 +1217,18 @@ __global__ void __launch_bounds__(192) k_gs_unit(Geo g, const float *__restrict_
 void affine_build(hipStream_t s, const Level &L, int sweeps) {
     const int n3 = 3 * L.g.nno;
     hipLaunchKernelGGL(k_gs_unit, dim3(2 * n3), dim3(192), (size_t)n3 * 8, s, L.g, L.K, L.BI, L.Ma, L.Mb, sweeps);
+    if (L.Maf) {   // the FP32 cycles read the maps rounded to FP32: half the bytes
+        to_f32(s, L.Maf, L.Ma, (long)n3 * n3);
+        to_f32(s, L.Mbf, L.Mb, (long)n3 * n3);
+    }
 }
 
 // out = Ma din + Mb F (din may be null: zero guess).  One wavefront per row
 // i: lane t sums the Ma terms j = t, t+64, ... in order, then the Mb terms
 // likewise, into S[t]; the S[t] meet in a pairwise tree (S[q] += S[q+w],
 // w = 1 .. 32) formed by a butterfly over the wavefront.
-template <typename V>
-__global__ void __launch_bounds__(256) k_affine_apply(const double *__restrict__ Ma, const double *__restrict__ Mb,
+template <typename V, typename M>
+__global__ void __launch_bounds__(256) k_affine_apply(const M *__restrict__ Ma, const M *__restrict__ Mb,
                                                       const V *__restrict__ din, const V *__restrict__ F,
                                                       V *__restrict__ out, int n3) {
     const int t = threadIdx.x & 63, i = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -1219,9 +1236,9 @@ __global__ void __launch_bounds__(256) k_affine_apply(const double *__restrict__
     if (i < n3) {   // unrolled so that the loads of several terms are in flight; the sum keeps its order
         if (din)
 #pragma unroll 8
-            for (int j = t; j < n3; j += 64) S = S + Ma[(size_t)i * n3 + j] * (double)din[j];
+            for (int j = t; j < n3; j += 64) S = S + (double)Ma[(size_t)i * n3 + j] * (double)din[j];
 #pragma unroll 8
-        for (int j = t; j < n3; j += 64) S = S + Mb[(size_t)i * n3 + j] * (double)F[j];
+        for (int j = t; j < n3; j += 64) S = S + (double)Mb[(size_t)i * n3 + j] * (double)F[j];
     }
 #pragma unroll
     for (int w = 1; w < 64; w *= 2) S = S + __shfl_xor(S, w, 64);
@@ -1231,7 +1248,12 @@ __global__ void __launch_bounds__(256) k_affine_apply(const double *__restrict__
 template <typename V>
 void affine_apply(hipStream_t s, const Level &L, const V *din, const V *F, V *out) {
     const int n3 = 3 * L.g.nno;
-    hipLaunchKernelGGL(k_affine_apply<V>, dim3((n3 + 3) / 4), dim3(256), 0, s, L.Ma, L.Mb, din, F, out, n3);
+    if (sizeof(V) == 4 && L.Maf)
+        hipLaunchKernelGGL((k_affine_apply<V, float>), dim3((n3 + 3) / 4), dim3(256), 0, s, (const float *)L.Maf,
+                           (const float *)L.Mbf, din, F, out, n3);
+    else
+        hipLaunchKernelGGL((k_affine_apply<V, double>), dim3((n3 + 3) / 4), dim3(256), 0, s, (const double *)L.Ma,
+                           (const double *)L.Mb, din, F, out, n3);
 }
 template void affine_apply<double>(hipStream_t, const Level &, const double *, const double *, double *);
 template void affine_apply<float>(hipStream_t, const Level &, const float *, const float *, float *);

@@ -440,6 +440,7 @@ int build_mesh(chip_ctx *c) {
             if (affine_ok(A)) {
                 const size_t n3 = 3 * (size_t)A.g.nno;
                 if (dev_alloc(c, &A.Ma, n3 * n3) || dev_alloc(c, &A.Mb, n3 * n3) || dev_alloc(c, &A.tmp, n3) ||
+                    (p.mixed && (dev_alloc(c, &A.Maf, n3 * n3) || dev_alloc(c, &A.Mbf, n3 * n3))) ||
                     dev_alloc(c, &A.tmpf, n3))
                     return -1;
                 A.affine_sweeps = p.down_heavy;
