@@ -282,18 +282,22 @@ def run_form(p, T, form, steps, warmup, graphs, profile=False, same_state=False)
     if same_state:
         out["same_state"] = same_state_solves(c, form)
     if profile:
-        c.profile("gs_colour")
-        c.timestep()
-        out["gs"] = c.profile_read()
+        # colour passes level by level (one more step each), per multigrid
+        # call of that step: the FP16-stiffness form runs two template
+        # instantiations of the colour pass on different levels
+        out["gs_levels"] = {}
+        for lev in range(p.levels):
+            c.set_option("profile_level", lev)
+            c.profile("gs_colour")
+            r = c.timestep()
+            n, ms, b = c.profile_read()
+            mg = max(r["stokes"]["mg_cycles"], 1)
+            if n:
+                out["gs_levels"][lev] = {"launches": n / mg, "ms": ms / mg, "bytes": b / mg}
+        c.set_option("profile_level", -1)
         c.profile("matvec")
         c.timestep()
         out["mv"] = c.profile_read()
-        # the finest level alone: its colour passes are the bandwidth-bound ones
-        c.set_option("profile_level", p.levmax)
-        c.profile("gs_colour")
-        c.timestep()
-        out["gs_finest"] = c.profile_read()
-        c.set_option("profile_level", -1)
         c.profile(None)
     c.close()
     return out
@@ -445,21 +449,49 @@ def main():
     log(f"device streams: copy {stream['copy_gbs']:.0f} GB/s, read {stream['read_gbs']:.0f} GB/s")
     del src, dst
 
-    n_gs, ms_gs, bytes_gs = main_run["gs"]
     n_mv, ms_mv, bytes_mv = main_run["mv"]
-    n_f, ms_f, bytes_f = main_run["gs_finest"]
-    ach_f = (bytes_f / max(n_f, 1)) / ((ms_f / max(n_f, 1)) * 1e-3) / 1e9
-    log(f"finest-level colour passes: {n_f} launches, {ms_f / max(n_f, 1) * 1e3:.1f} us each, {ach_f:.0f} GB/s")
-    avg_ms = ms_gs / max(n_gs, 1)
-    achieved = (bytes_gs / max(n_gs, 1)) / (avg_ms * 1e-3) / 1e9
-    log(f"gs_colour: {n_gs} launches, {ms_gs:.1f} ms total, {achieved:.0f} GB/s; "
-        f"matvec: {n_mv} launches, {ms_mv:.1f} ms, {(bytes_mv / max(ms_mv, 1e-9)) / 1e6:.0f} GB/s")
     s_per_step = main_run["s_per_step"]
     mixed = FORMS[args.form][0]
-    # <vector type, stiffness type>; the FP16 variant runs on the two finest
-    # levels and the FP32 one below, both averaged here
-    kernel = {0: "k_gs_colour<double, float>", 1: "k_gs_colour<float, float>",
-              2: "k_gs_colour<float, _Float16> (two finest levels) and <float, float>"}[mixed]
+    levels = main_run["gs_levels"]
+
+    def passes(levs, name):
+        """colour passes of these levels, per multigrid call: launches, time,
+        algorithmic bytes, the rate they reach, and their counter traffic"""
+        n = sum(levels[lv]["launches"] for lv in levs)
+        ms = sum(levels[lv]["ms"] for lv in levs)
+        b = sum(levels[lv]["bytes"] for lv in levs)
+        ach = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        return {"kernel": name, "levels": sorted(levs), "launches_per_mg_call": n, "ms_per_mg_call": ms,
+                "avg_launch_ms": ms / max(n, 1e-9), "algorithmic_bytes_per_launch": b / max(n, 1e-9),
+                "achieved": ach, "frac": ach / PEAK_HBM_GBS}
+
+    # <vector type, stiffness type>: the FP16-stiffness form runs
+    # k_gs_colour<float, _Float16> on its two finest levels and
+    # k_gs_colour<float, float> below; the other forms one instantiation
+    fp16 = {lv for lv in levels if mixed == 2 and lv >= p.levels - 2 and lv >= 2}
+    groups = {"k_gs_colour<float, _Float16>": fp16,
+              {0: "k_gs_colour<double, float>", 1: "k_gs_colour<float, float>",
+               2: "k_gs_colour<float, float> (levels below the FP16 ones)"}[mixed]: set(levels) - fp16}
+    per_kernel = {k: passes(v, k) for k, v in groups.items() if v}
+    if not per_kernel:
+        sys.exit("bench: no colour-pass launches to measure on this workload (every level is smoothed in one "
+                 "workgroup); use a larger --config")
+    for k, r in per_kernel.items():
+        r["traffic"] = measured_traffic(k, args.config)
+        log(f"{k}: levels {r['levels']}, {r['launches_per_mg_call']:.0f} launches and {r['ms_per_mg_call']:.3f} ms "
+            f"per multigrid call, {r['avg_launch_ms'] * 1e3:.2f} us each, {r['achieved']:.0f} GB/s")
+    # the dominant kernel: the colour pass, over all its instantiations and
+    # levels (per instantiation beside it)
+    allp = passes(set(levels), "k_gs_colour (every instantiation and level)")
+    allp["traffic"] = measured_traffic({0: "k_gs_colour<double, float>", 1: "k_gs_colour<float, float>",
+                                        2: "k_gs_colour<float, _Float16> (two finest levels) and <float, float>"}
+                                       [mixed], args.config)
+    fin = passes({p.levels - 1}, "k_gs_colour, finest level")
+    fin_kernel = [k for k, r in per_kernel.items() if p.levels - 1 in r["levels"]]
+    fin["traffic"] = measured_traffic(fin_kernel[0], args.config, "finest_level_hbm_bytes_per_launch") \
+        if fin_kernel else None
+    log(f"all colour passes: {allp['achieved']:.0f} GB/s; finest level {fin['avg_launch_ms'] * 1e3:.1f} us each, "
+        f"{fin['achieved']:.0f} GB/s; matvec: {ms_mv:.1f} ms, {(bytes_mv / max(ms_mv, 1e-9)) / 1e6:.0f} GB/s")
     visc = (f"viscosity exp(-{p.viscE[0]:.4g} T)" + (f", {p.num_mat} layers (visc0 {p.visc0[:p.num_mat]})"
                                                       if p.num_mat > 1 else "")) if p.tdepv else "isoviscous"
 
@@ -506,9 +538,6 @@ def main():
         config[f"speedup_vs_{f}"] = r["s_per_step"] / s_per_step
         log(f"{f}: {r['s_per_step']:.3f} s/step, {r['uzawa_iterations_per_step']:.1f} Uzawa iterations; "
             f"{args.form}: {s_per_step:.3f} s/step ({r['s_per_step'] / s_per_step:.2f}x)")
-    finest = {"achieved": ach_f, "frac": ach_f / PEAK_HBM_GBS, "launches": n_f,
-              "avg_launch_ms": ms_f / max(n_f, 1), "algorithmic_bytes_per_launch": bytes_f / max(n_f, 1),
-              "traffic": measured_traffic(kernel, args.config, "finest_level_hbm_bytes_per_launch")}
     out = {
         "metric": METRIC,
         "value": s_per_step,
@@ -523,9 +552,12 @@ def main():
         "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic",
         "config": config,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(kernel, args.config), "kernel": kernel,
-                     "launches": n_gs, "avg_launch_ms": avg_ms, "finest_level": finest, "device_streams": stream},
+        "roofline": {"bound": "hbm", "achieved": allp["achieved"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": allp["frac"], "traffic": allp["traffic"], "kernel": allp["kernel"],
+                     "levels": allp["levels"], "avg_launch_ms": allp["avg_launch_ms"],
+                     "launches_per_mg_call": allp["launches_per_mg_call"],
+                     "algorithmic_bytes_per_launch": allp["algorithmic_bytes_per_launch"],
+                     "per_kernel": per_kernel, "finest_level": fin, "device_streams": stream},
     }
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
