@@ -13,6 +13,9 @@
 #include <string>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include "chip_kernels.h"
 #include "../../include/citcom_learn.h"
 
@@ -205,12 +208,21 @@ void coarse_smooth(chip_ctx *c, V *vel, const V *rhs) {
 // its res
 template <typename V>
 void enqueue_cycles(chip_ctx *c, V *F) {
-    const int levmin = c->levmin, levmax = c->levmax;
+    const int levmax = c->levmax;
+    // the FP32 cycles stop at the level solved exactly, where there is one
+    const bool direct = sizeof(V) == 4 && c->direct_ok && !c->cnet_on;
+    const int levmin = direct ? c->dlev : c->levmin;
     const chip_problem &p = c->p;
     Level *L = c->L;
     auto fl = [&](int lev) { return lev == levmax ? F : vecs<V>(L[lev]).fl; };
+    auto bottom = [&](V *vel, const V *rhs) {
+        if (direct)
+            dense_apply(S(c), c->dAinv, rhs, vel, 3 * L[levmin].g.nno);
+        else
+            coarse_smooth(c, vel, rhs);
+    };
     for (int lev = levmax; lev > levmin; lev--) project(S(c), L[lev], L[lev - 1], (const V *)fl(lev), fl(lev - 1));
-    coarse_smooth(c, vecs<V>(L[levmin]).vel, (const V *)fl(levmin));
+    bottom(vecs<V>(L[levmin]).vel, (const V *)fl(levmin));
     for (int lev = levmin + 1; lev <= levmax; lev++) {
         interp(S(c), L[lev - 1], L[lev], (const V *)vecs<V>(L[lev - 1]).vel, vecs<V>(L[lev]).vel);
         for (int Vn = 1; Vn <= p.mg_cycle; Vn++) {
@@ -225,7 +237,7 @@ void enqueue_cycles(chip_ctx *c, V *F) {
                 project(S(c), L[dlev], L[dlev - 1], (const V *)v.res, vecs<V>(L[dlev - 1]).fl, true,
                         vecs<V>(L[dlev - 1]).vel);
             }
-            coarse_smooth(c, vecs<V>(L[levmin]).vel, (const V *)vecs<V>(L[levmin]).fl);
+            bottom(vecs<V>(L[levmin]).vel, (const V *)vecs<V>(L[levmin]).fl);
             for (int ulev = levmin + 1; ulev <= lev; ulev++) {
                 const LV<V> v = vecs<V>(L[ulev]);
                 const int cycles = (ulev == levmax) ? p.v_steps_high : p.up_heavy;
@@ -497,6 +509,59 @@ int uzawa(chip_ctx *c, chip_stokes_stats *st) {
     return prof_flush(c);
 }
 
+// the FP32 cycles' exact coarsest-level solve (mixed forms): after every
+// rebuild, the level's stiffness assembled dense, factored (potrf) and
+// inverted (potri) in FP64 on the work stream, the inverse kept in FP32.  A
+// factorization that fails (the matrix not positive definite in FP64)
+// leaves the cycles on the full hierarchy until the next rebuild; the
+// captured multigrid graph follows the change.  The learned coarse-level
+// correction, when on, keeps the full hierarchy (it works on its coarsest
+// level).
+int direct_factor(chip_ctx *c) {
+    const bool was = c->direct_ok;
+    c->direct_ok = false;
+    if (c->dlev >= 0 && c->direct_on) {
+        const Level &L = c->L[c->dlev];
+        const int n = 3 * L.g.nno;
+        if (!c->rb) {
+            rocblas_handle h;
+            if (rocblas_create_handle(&h) != rocblas_status_success) {
+                set_error("coarse direct solve: rocblas_create_handle failed");
+                return -1;
+            }
+            c->rb = h;
+        }
+        rocblas_handle h = (rocblas_handle)c->rb;
+        if (rocblas_set_stream(h, S(c)) != rocblas_status_success) {
+            set_error("coarse direct solve: rocblas_set_stream failed");
+            return -1;
+        }
+        dense_assemble(S(c), L, c->dA);
+        int info = 0;
+        if (rocsolver_dpotrf(h, rocblas_fill_lower, n, c->dA, n, c->dinfo) != rocblas_status_success) {
+            set_error("coarse direct solve: rocsolver_dpotrf failed to launch");
+            return -1;
+        }
+        CHIP_CHECK(hipMemcpyAsync(&info, c->dinfo, sizeof(int), hipMemcpyDeviceToHost, S(c)));
+        CHIP_CHECK(hipStreamSynchronize(S(c)));
+        if (info == 0) {
+            if (rocsolver_dpotri(h, rocblas_fill_lower, n, c->dA, n, c->dinfo) != rocblas_status_success) {
+                set_error("coarse direct solve: rocsolver_dpotri failed to launch");
+                return -1;
+            }
+            CHIP_CHECK(hipMemcpyAsync(&info, c->dinfo, sizeof(int), hipMemcpyDeviceToHost, S(c)));
+            CHIP_CHECK(hipStreamSynchronize(S(c)));
+        }
+        if (info == 0) {
+            sym_to_f32(S(c), c->dA, c->dAinv, n);
+            CHIP_CHECK(hipGetLastError());
+            c->direct_ok = true;
+        }
+    }
+    if (c->direct_ok != was && drop_mg_graph(c)) return -1;
+    return 0;
+}
+
 int rebuild(chip_ctx *c) {
     hipStream_t q = S(c);
     for (int lev = c->levmax; lev > c->levmin; lev--) project_viscosity(q, c->L[lev], c->L[lev - 1]);
@@ -512,7 +577,7 @@ int rebuild(chip_ctx *c) {
     for (int lev = c->levmin; lev <= c->levmax; lev++)
         if (c->L[lev].Kh) scale_half(q, c->L[lev]);
     CHIP_CHECK(hipGetLastError());
-    return 0;
+    return direct_factor(c);
 }
 
 // PG_timestep (Advection_diffusion.c:251-345)
@@ -742,6 +807,7 @@ int chip_create(const chip_problem *p, chip_ctx **out) {
 void chip_destroy(chip_ctx *c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->rb) (void)rocblas_destroy_handle((rocblas_handle)c->rb);
     if (c->mg_graph) (void)hipGraphExecDestroy(c->mg_graph);
     for (void *ptr : c->allocs) (void)hipFree(ptr);
     if (c->h_dots) (void)hipHostFree(c->h_dots);
@@ -882,6 +948,7 @@ int chip_get_system_viscosity(chip_ctx *c) {
 
 int chip_construct_stiffness_B_matrix(chip_ctx *c) {
     CHIP_NEED_CTX(c);
+    c->built = true;
     if (rebuild(c)) return -1;
     CHIP_CHECK(hipStreamSynchronize(c->stream));
     return 0;
@@ -932,6 +999,14 @@ int chip_apply(chip_ctx *c, const char *op, int lev, const double *in, const dou
         CHIP_CHECK(up(b, in2, nv));
         affine_apply(q, L, guess ? (const double *)a : nullptr, (const double *)b, L.tmp);
         CHIP_CHECK(down(out, L.tmp, nv));
+    } else if (o == "direct") {
+        if (lev != c->dlev || !c->direct_ok) {
+            set_error("direct: the level solved exactly by the FP32 cycles, after a rebuild");
+            return -1;
+        }
+        CHIP_CHECK(up(b, in2, nv));
+        dense_apply(q, c->dAinv, (const double *)b, a, (int)nv);
+        CHIP_CHECK(down(out, a, nv));
     } else if (o == "coarse") {
         if (lev != c->levmin || !L.Mc) {
             set_error("coarse: the coarsest level has no coarse matrix");
@@ -1225,6 +1300,12 @@ int chip_set_option(chip_ctx *c, const char *name, int value) {
         return -1;
     }
     if (n == "coarse_net" && value) c->cnet_calibrated = false;   // the next solve calibrates it
+    if (n == "coarse_direct") {   // the FP32 cycles' exact coarsest-level solve
+        c->direct_on = value != 0;
+        if (c->built && direct_factor(c)) return -1;
+        if (!c->built && !c->direct_on && drop_mg_graph(c)) return -1;
+        return 0;
+    }
     if (n == "xcd_gs" && value) {   // relies on round-robin dispatch to the XCDs: checked here
         for (int lev = c->levmin; lev <= c->levmax; lev++) {
             if (!gs_xcd_ok(c->L[lev])) continue;

@@ -19,6 +19,10 @@ namespace chip {
 
 constexpr int MAXL = 12;
 constexpr int KROW = 243;   // 3 rows x 27 neighbour nodes x 3 columns
+// largest level (unknowns) solved exactly in the FP32 cycles: its dense
+// inverse is 3 nno x 3 nno FP32 (243 MB at 8192), applied a few times per
+// multigrid call, factored in FP64 at every rebuild
+constexpr long DIRECT_MAX_DOF = 8192;
 constexpr int GNX_SZ = 216; // Shape_function_dx: vpt[3*8*8] + ppt[3*8]
 constexpr int GDA_SZ = 11;  // Shape_function_dA: vpt[9] + ppt[2]
 constexpr int ECO_SZ = 13;  // COORD: centre[4] size[4] recip_size[4] area
@@ -137,6 +141,17 @@ struct chip_ctx {
     bool small_levels = true;  // single-workgroup smoother on small levels
     bool gs_wg = true;         // ... including levels whose stiffness is read through L2
     bool xcd_gs = false;       // whole smoother calls of mid-size levels on one XCD (k_gs_xcd)
+    // exact solve on the FP32 cycles' coarsest level (mixed forms, option
+    // "coarse_direct"): the level (-1: none), its stiffness as a dense FP64
+    // matrix factored and inverted in place (rocSOLVER, column-major), the
+    // inverse in FP32 (row-major), the factorization's status
+    int dlev = -1;
+    bool direct_on = true;
+    bool direct_ok = false;
+    double *dA = nullptr;
+    float *dAinv = nullptr;
+    int *dinfo = nullptr;
+    void *rb = nullptr;        // rocblas_handle of the work stream
     unsigned *xcd_ids = nullptr;   // [16] set-up check of the XCD dispatch
     // learned coarse-level correction (chip_set_coarse_net): a 3-D CNN on the
     // coarsest level in place of its smoother call; BF16 weights

@@ -1140,6 +1140,90 @@ int xcd_colocated(hipStream_t s, int nwg, unsigned *ids_dev, bool *same) {
 
 // ---------------------------------------------------------------------------
 // Coarsest level as a matrix.  The gauss_seidel call on the coarsest level of
+// ---------------------------------------------------------------------------
+// Exact solve on the FP32 cycles' coarsest level (mixed forms): the level's
+// stiffness as a dense symmetric matrix, factored and inverted in FP64 by
+// rocSOLVER (potrf, potri) when the operators are rebuilt, applied in FP32.
+// Fixed components (the rows strip3 zeroes) get an identity row and column,
+// so the matrix is SPD and the solve leaves them at the right-hand side's
+// value, which the restriction has already zeroed.
+
+// A (column-major n x n, zeroed) <- K of the level: node q's row r, block
+// of neighbour m, column cc at A[(3n+r) + (3m+cc) n]
+__global__ void __launch_bounds__(TPB) k_dense_assemble(Geo g, const float *__restrict__ K,
+                                                        const unsigned *__restrict__ NODE, double *A) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int col = colour_valid(g, q);
+    if (col < 0) return;
+    int k, i, j;
+    colour_kij(g, col, q, k, i, j);
+    const long n3 = 3L * g.nno;
+    const int self = nid(g, k, i, j);
+    const unsigned fs = NODE[self];
+    const unsigned fixbit[3] = {VBX, VBY, VBZ};
+    const float *kp = K + ktile(q);
+    for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++)
+            for (int dz = -1; dz <= 1; dz++) {
+                if ((unsigned)(j + dy) >= (unsigned)g.noy || (unsigned)(i + dx) >= (unsigned)g.nox ||
+                    (unsigned)(k + dz) >= (unsigned)g.noz)
+                    continue;
+                const int m = self + dz + g.noz * (dx + g.nox * dy);
+                const unsigned fm = NODE[m];
+                const int b = (dz + 1) + 3 * (dx + 1), P = dy + 1;
+                for (int r = 0; r < 3; r++)
+                    for (int cc = 0; cc < 3; cc++) {
+                        const long row = 3L * self + r, cl = 3L * m + cc;
+                        double v = (double)kp[(81 * P + 9 * b + 3 * r + cc) * 64];
+                        if ((fs & fixbit[r]) || (fm & fixbit[cc])) v = (row == cl) ? 1.0 : 0.0;
+                        A[row + cl * n3] = v;
+                    }
+            }
+}
+
+// M (row-major, full) <- the inverse held in A's lower triangle, in FP32
+__global__ void __launch_bounds__(TPB) k_sym_to_f32(const double *__restrict__ A, float *__restrict__ M, int n) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)n * n) return;
+    const long r = idx / n, c = idx % n;
+    const long lo = r > c ? r : c, hi = r > c ? c : r;   // lower triangle: row >= column
+    M[idx] = (float)A[lo + hi * (long)n];
+}
+
+// out = M F, one wavefront per row, FP64 sums in a fixed order (lane t the
+// terms j = t, t+64, ..., then a butterfly over the wavefront)
+template <typename V>
+__global__ void __launch_bounds__(256) k_dense_apply(const float *__restrict__ M, const V *__restrict__ F,
+                                                     V *__restrict__ out, int n) {
+    const int t = threadIdx.x & 63, i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    double S = 0.0;
+    if (i < n) {
+        const float *row = M + (size_t)i * n;
+#pragma unroll 8
+        for (int j = t; j < n; j += 64) S = S + (double)row[j] * (double)F[j];
+    }
+#pragma unroll
+    for (int w = 1; w < 64; w *= 2) S = S + __shfl_xor(S, w, 64);
+    if (i < n && t == 0) out[i] = (V)S;
+}
+
+void dense_assemble(hipStream_t s, const Level &L, double *A) {
+    const long n3 = 3L * L.g.nno;
+    (void)hipMemsetAsync(A, 0, sizeof(double) * n3 * n3, s);
+    hipLaunchKernelGGL(k_dense_assemble, dim3(nblk(L.g.pitch)), dim3(TPB), 0, s, L.g, L.K, L.NODE, A);
+}
+
+void sym_to_f32(hipStream_t s, const double *A, float *M, int n) {
+    hipLaunchKernelGGL(k_sym_to_f32, dim3((unsigned)(((long)n * n + TPB - 1) / TPB)), dim3(TPB), 0, s, A, M, n);
+}
+
+template <typename V>
+void dense_apply(hipStream_t s, const float *M, const V *F, V *out, int n) {
+    hipLaunchKernelGGL(k_dense_apply<V>, dim3((n + 3) / 4), dim3(256), 0, s, M, F, out, n);
+}
+template void dense_apply<double>(hipStream_t, const float *, const double *, double *, int);
+template void dense_apply<float>(hipStream_t, const float *, const float *, float *, int);
+
 // multi_grid (v_steps_low sweeps from a zero guess, General_matrix_functions.c
 // :574, :613) is linear in its right-hand side apart from the FP32 rounding
 // of each increment, and its Ad is not read afterwards.  Column j of Mc is

@@ -447,6 +447,18 @@ int build_mesh(chip_ctx *c) {
             }
         }
     }
+    // the FP32 cycles' exact coarsest-level solve: the largest level of at
+    // most DIRECT_MAX_DOF unknowns with at least two levels below it and
+    // the finest above it (17x17x9 nodes at 128x128x64 elements)
+    if (p.mixed)
+        for (int lev = c->levmax - 1; lev >= c->levmin + 2; lev--)
+            if (3L * c->L[lev].g.nno <= DIRECT_MAX_DOF) {
+                const size_t n3 = 3 * (size_t)c->L[lev].g.nno;
+                if (dev_alloc(c, &c->dA, n3 * n3) || dev_alloc(c, &c->dAinv, n3 * n3) || dev_alloc(c, &c->dinfo, 1))
+                    return -1;
+                c->dlev = lev;
+                break;
+            }
     const Geo &g = c->L[c->levmax].g;
     const size_t nno = g.nno, nel = g.nel, neq = 3 * nno;
     if (dev_alloc(c, &c->U, neq) || dev_alloc(c, &c->P, nel) || dev_alloc(c, &c->F, neq) ||

@@ -83,13 +83,15 @@ def test_fullsize_forms_agree(name, piterations):
         c.close()
         assert rm < 1e-4   # measured 4.6e-6 (tdep128), 1.1e-6 (layer256)
         assert st["iterations"] == p.p_iterations or (st["dvelocity"] < p.accuracy and st["dpressure"] < p.accuracy)
-    # FP32 cycles: the same iterates to rounding (measured U 4.4e-8, P 1.3e-6
-    # at tdep128; at tdep64, where the cap also stops the velocity at dv/v
-    # 2.8e-6, U 1.2e-6, P 4.4e-5).  FP16 stiffness: a slightly different
-    # preconditioner, so at the 375-iteration cap, where no form has
-    # converged the pressure (dp/p 5e-4), it stops at another iterate: U
-    # 6.6e-6, P 7.0e-3 measured at tdep128
-    bounds = {1: (1e-5, 1e-4), 2: (1e-4, 3e-2)}
+    # FP32 cycles on the full hierarchy: the same iterates to rounding
+    # (measured at tdep64, where the cap also stops the velocity at dv/v
+    # 2.8e-6: U 1.0e-6, P 3.6e-5).  Where the FP32 cycles solve a level
+    # exactly (tdep128, layer256: the 17x17x9-node level) or the stiffness is
+    # FP16, the preconditioner differs, so at the 375-iteration cap, where
+    # no form has converged the pressure (dp/p 5e-4), the solve stops at
+    # another iterate: U 5.6e-6, P 1.5e-3 (FP32) and U 6.6e-6, P 7.0e-3
+    # (FP16) measured at tdep128
+    bounds = {1: (1e-5, 1e-4) if name == "tdep64" else (1e-4, 3e-2), 2: (1e-4, 3e-2)}
     for mixed in (1, 2):
         eu, ep = rel_l2(out[mixed][0], out[0][0]), rel_l2(out[mixed][1], out[0][1])
         say(f"{name}: mixed={mixed} against plain: U {eu:.2e}, P {ep:.2e}")
@@ -181,3 +183,42 @@ def test_tdep64_against_reference():
     # (its advective part is large: |Vz| reaches 1e4 in the bottom layers)
     assert e["dt"] < 1e-6 and e["T"] < 1e-6 and e["U"] < 1e-6 and e["P"] < 1e-5
     assert e["Nu_top"] < 1e-5 and e["Nu_bot"] < 1e-5
+
+
+def test_direct_coarse_level():
+    """The FP32 cycles' exact solve on the 17x17x9-node level of tdep128
+    (the level's stiffness factored and inverted in FP64 by rocSOLVER, the
+    inverse applied in FP32): K x = F for random right-hand sides to the
+    FP32 inverse's rounding; and the mixed forms' step-0 Stokes solve with
+    it on and off reaches the same velocity and pressure"""
+    p, c = start("tdep128", mixed=1)
+    c.get_system_viscosity()
+    c.construct_stiffness_B_matrix()
+    lev = 2
+    assert c.nno(lev) == 17 * 17 * 9
+    rng = np.random.default_rng(7)
+    worst = 0.0
+    for _ in range(3):
+        F = c.apply("strip", lev, rng.standard_normal(3 * c.nno(lev)))
+        x = c.apply("direct", lev, None, F)
+        r = c.apply("strip", lev, c.apply("matvec", lev, x) - F)
+        worst = max(worst, float(np.linalg.norm(r) / np.linalg.norm(F)))
+    say(f"direct solve on level {lev}: |K x - F| / |F| = {worst:.2e}")
+    c.close()
+    assert worst < 1e-5   # measured 7.3e-8
+    # measured: 505 against 510 multigrid calls (FP32), 515 against 523
+    # (FP16 stiffness); U 5.6e-6 and 6.8e-6, P 1.5e-3 and 6.5e-3 apart at
+    # the 375-iteration cap, where no form has converged the pressure
+    out = {}
+    for mixed in (1, 2):
+        for on in (1, 0):
+            p, c = start("tdep128", mixed)
+            c.set_option("coarse_direct", on)
+            st = c.general_stokes_solver()
+            out[(mixed, on)] = (c.download("U"), c.download("P"), st)
+            say(f"tdep128 mixed={mixed} coarse_direct={on}: {st['iterations']} Uzawa its, {st['mg_cycles']} MG calls")
+            c.close()
+        (U1, P1, s1), (U0, P0, s0) = out[(mixed, 1)], out[(mixed, 0)]
+        eu, ep = rel_l2(U1, U0), rel_l2(P1 - P1.mean(), P0 - P0.mean())
+        say(f"tdep128 mixed={mixed}: direct against the full hierarchy: U {eu:.2e}, P {ep:.2e}")
+        assert eu < 1e-4 and ep < 3e-2
