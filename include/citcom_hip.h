@@ -124,6 +124,8 @@ int chip_construct_stiffness_B_matrix(chip_ctx *c);
  *         Ma in + Mb in2
  *     "cnet" (coarsest level) the learned coarse-level correction for the
  *         right-hand side in2 (after chip_set_coarse_net)
+ *     "direct" (the level the FP32 cycles solve exactly, mixed forms, after
+ *         a rebuild) the FP32 inverse of its stiffness: out = K^-1 in2
  *     "project" project_vector lev -> lev-1 (Solver_multigrid.c:72-159)
  *     "interp"  interp_vector lev-1 -> lev (Solver_multigrid.c:173-298)
  *     "div" assemble_div_u (Element_calculations.c:691-720)
@@ -194,6 +196,14 @@ long chip_device_bytes(chip_ctx *c);
  * "gs_wg": 1 (default) also runs a whole gauss_seidel call in one workgroup
  * on levels of at most 1024 nodes whose stiffness does not fit in LDS;
  * 0 launches one kernel per colour pass there; results are identical.
+ * "coarse_direct": 1 (default) lets the FP32 cycles of the mixed forms
+ * stop at the largest level below the finest with at most 8192 unknowns
+ * and at least two levels below it (17x17x9 nodes at 128x128x64 elements)
+ * and solve it exactly: its stiffness is assembled dense, factored and
+ * inverted in FP64 by rocSOLVER (potrf, potri) at every rebuild and
+ * applied in FP32, in place of the smoother calls, transfers and maps of
+ * that level and the ones below; 0 keeps the full hierarchy.  The plain
+ * form always keeps it.  The solve still iterates to the same criterion.
  * "xcd_gs": 1 runs a whole gauss_seidel call on levels of 1025 to 6000
  * nodes as one launch of a few workgroups on one XCD, with barriers through
  * that XCD's L2 between the colour passes; 0 (default) launches one kernel
