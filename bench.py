@@ -284,7 +284,10 @@ def run_form(p, T, form, steps, warmup, graphs, profile=False, same_state=False)
     if profile:
         # colour passes level by level (one more step each), per multigrid
         # call of that step: the FP16-stiffness form runs two template
-        # instantiations of the colour pass on different levels
+        # instantiations of the colour pass on different levels.  The exact
+        # coarse level's factorization runs on the work stream here, so that
+        # the passes measured do not share the device with it
+        c.set_option("direct_async", 0)
         out["gs_levels"] = {}
         for lev in range(p.levels):
             c.set_option("profile_level", lev)

@@ -511,55 +511,102 @@ int uzawa(chip_ctx *c, chip_stokes_stats *st) {
 
 // the FP32 cycles' exact coarsest-level solve (mixed forms): after every
 // rebuild, the level's stiffness assembled dense, factored (potrf) and
-// inverted (potri) in FP64 on the work stream, the inverse kept in FP32.  A
+// inverted (potri) in FP64, the inverse kept in FP32.  The first
+// factorization runs on the work stream and is used at once; later ones run
+// on a side stream while the step's solve goes on with the previous
+// inverse (an exact solve of the previous step's stiffness, a slightly
+// weaker coarse correction), and are taken up at the next rebuild.  A
 // factorization that fails (the matrix not positive definite in FP64)
-// leaves the cycles on the full hierarchy until the next rebuild; the
-// captured multigrid graph follows the change.  The learned coarse-level
-// correction, when on, keeps the full hierarchy (it works on its coarsest
-// level).
-int direct_factor(chip_ctx *c) {
-    const bool was = c->direct_ok;
-    c->direct_ok = false;
-    if (c->dlev >= 0 && c->direct_on) {
-        const Level &L = c->L[c->dlev];
-        const int n = 3 * L.g.nno;
-        if (!c->rb) {
-            rocblas_handle h;
-            if (rocblas_create_handle(&h) != rocblas_status_success) {
-                set_error("coarse direct solve: rocblas_create_handle failed");
-                return -1;
-            }
-            c->rb = h;
-        }
-        rocblas_handle h = (rocblas_handle)c->rb;
-        if (rocblas_set_stream(h, S(c)) != rocblas_status_success) {
-            set_error("coarse direct solve: rocblas_set_stream failed");
-            return -1;
-        }
-        dense_assemble(S(c), L, c->dA);
-        int info = 0;
-        if (rocsolver_dpotrf(h, rocblas_fill_lower, n, c->dA, n, c->dinfo) != rocblas_status_success) {
-            set_error("coarse direct solve: rocsolver_dpotrf failed to launch");
-            return -1;
-        }
-        CHIP_CHECK(hipMemcpyAsync(&info, c->dinfo, sizeof(int), hipMemcpyDeviceToHost, S(c)));
-        CHIP_CHECK(hipStreamSynchronize(S(c)));
-        if (info == 0) {
-            if (rocsolver_dpotri(h, rocblas_fill_lower, n, c->dA, n, c->dinfo) != rocblas_status_success) {
-                set_error("coarse direct solve: rocsolver_dpotri failed to launch");
-                return -1;
-            }
-            CHIP_CHECK(hipMemcpyAsync(&info, c->dinfo, sizeof(int), hipMemcpyDeviceToHost, S(c)));
-            CHIP_CHECK(hipStreamSynchronize(S(c)));
-        }
-        if (info == 0) {
-            sym_to_f32(S(c), c->dA, c->dAinv, n);
-            CHIP_CHECK(hipGetLastError());
-            c->direct_ok = true;
-        }
-    }
-    if (c->direct_ok != was && drop_mg_graph(c)) return -1;
+// leaves the cycles on the full hierarchy until one succeeds; the captured
+// multigrid graph follows the change.  The learned coarse-level correction,
+// when on, keeps the full hierarchy (it works on its coarsest level).
+
+// the pending side-stream factorization: wait for it, and when both its
+// steps succeeded convert it into the FP32 inverse on the work stream.
+// *state: 0 none was pending, 1 taken, 2 it failed
+int direct_take(chip_ctx *c, int *state) {
+    *state = 0;
+    if (!c->fact_pending) return 0;
+    c->fact_pending = false;
+    int info[2] = {1, 1};
+    CHIP_CHECK(hipMemcpyAsync(info, c->dinfo, sizeof info, hipMemcpyDeviceToHost, c->side));
+    CHIP_CHECK(hipStreamSynchronize(c->side));
+    *state = 2;
+    if (info[0] != 0 || info[1] != 0) return 0;
+    CHIP_CHECK(hipStreamWaitEvent(S(c), c->ev_fact, 0));
+    sym_to_f32(S(c), c->dA, c->dAinv, 3 * c->L[c->dlev].g.nno);
+    CHIP_CHECK(hipGetLastError());
+    *state = 1;
     return 0;
+}
+
+int direct_factor(chip_ctx *c) {
+    bool was = c->direct_ok;
+    int state = 0;
+    if (direct_take(c, &state)) return -1;
+    const bool taken = state == 1;
+    if (state == 2) {   // the last one failed: factor this step's on the work stream
+        c->direct_ok = false;
+        if (drop_mg_graph(c)) return -1;
+        was = false;
+    }
+    if (c->dlev < 0 || !c->direct_on) {
+        c->direct_ok = false;
+        return (c->direct_ok != was) ? drop_mg_graph(c) : 0;
+    }
+    const Level &L = c->L[c->dlev];
+    const int n = 3 * L.g.nno;
+    if (!c->rb) {
+        rocblas_handle h;
+        if (rocblas_create_handle(&h) != rocblas_status_success) {
+            set_error("coarse direct solve: rocblas_create_handle failed");
+            return -1;
+        }
+        c->rb = h;
+        CHIP_CHECK(hipStreamCreate(&c->side));
+        CHIP_CHECK(hipEventCreateWithFlags(&c->ev_asm, hipEventDisableTiming));
+        CHIP_CHECK(hipEventCreateWithFlags(&c->ev_fact, hipEventDisableTiming));
+    }
+    rocblas_handle h = (rocblas_handle)c->rb;
+    // dA is free: the taken factorization has been converted (same stream)
+    dense_assemble(S(c), L, c->dA);
+    const bool async = c->direct_async && (was || taken);   // an inverse to go on with
+    if (async) {
+        CHIP_CHECK(hipEventRecord(c->ev_asm, S(c)));
+        CHIP_CHECK(hipStreamWaitEvent(c->side, c->ev_asm, 0));
+        if (rocblas_set_stream(h, c->side) != rocblas_status_success ||
+            rocsolver_dpotrf(h, rocblas_fill_lower, n, c->dA, n, c->dinfo) != rocblas_status_success ||
+            rocsolver_dpotri(h, rocblas_fill_lower, n, c->dA, n, c->dinfo + 1) != rocblas_status_success) {
+            set_error("coarse direct solve: rocSOLVER failed to launch");
+            return -1;
+        }
+        CHIP_CHECK(hipEventRecord(c->ev_fact, c->side));
+        c->fact_pending = true;
+        c->direct_ok = true;
+        return 0;
+    }
+    int info = 0;
+    if (rocblas_set_stream(h, S(c)) != rocblas_status_success ||
+        rocsolver_dpotrf(h, rocblas_fill_lower, n, c->dA, n, c->dinfo) != rocblas_status_success) {
+        set_error("coarse direct solve: rocsolver_dpotrf failed to launch");
+        return -1;
+    }
+    CHIP_CHECK(hipMemcpyAsync(&info, c->dinfo, sizeof(int), hipMemcpyDeviceToHost, S(c)));
+    CHIP_CHECK(hipStreamSynchronize(S(c)));
+    if (info == 0) {
+        if (rocsolver_dpotri(h, rocblas_fill_lower, n, c->dA, n, c->dinfo + 1) != rocblas_status_success) {
+            set_error("coarse direct solve: rocsolver_dpotri failed to launch");
+            return -1;
+        }
+        CHIP_CHECK(hipMemcpyAsync(&info, c->dinfo + 1, sizeof(int), hipMemcpyDeviceToHost, S(c)));
+        CHIP_CHECK(hipStreamSynchronize(S(c)));
+    }
+    c->direct_ok = info == 0;
+    if (c->direct_ok) {
+        sym_to_f32(S(c), c->dA, c->dAinv, n);
+        CHIP_CHECK(hipGetLastError());
+    }
+    return (c->direct_ok != was) ? drop_mg_graph(c) : 0;
 }
 
 int rebuild(chip_ctx *c) {
@@ -807,7 +854,11 @@ int chip_create(const chip_problem *p, chip_ctx **out) {
 void chip_destroy(chip_ctx *c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->rb) (void)rocblas_destroy_handle((rocblas_handle)c->rb);
+    if (c->ev_asm) (void)hipEventDestroy(c->ev_asm);
+    if (c->ev_fact) (void)hipEventDestroy(c->ev_fact);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->mg_graph) (void)hipGraphExecDestroy(c->mg_graph);
     for (void *ptr : c->allocs) (void)hipFree(ptr);
     if (c->h_dots) (void)hipHostFree(c->h_dots);
@@ -1300,6 +1351,10 @@ int chip_set_option(chip_ctx *c, const char *name, int value) {
         return -1;
     }
     if (n == "coarse_net" && value) c->cnet_calibrated = false;   // the next solve calibrates it
+    if (n == "direct_async") {
+        c->direct_async = value != 0;
+        return 0;
+    }
     if (n == "coarse_direct") {   // the FP32 cycles' exact coarsest-level solve
         c->direct_on = value != 0;
         if (c->built && direct_factor(c)) return -1;

@@ -150,8 +150,16 @@ struct chip_ctx {
     bool direct_ok = false;
     double *dA = nullptr;
     float *dAinv = nullptr;
-    int *dinfo = nullptr;
-    void *rb = nullptr;        // rocblas_handle of the work stream
+    int *dinfo = nullptr;      // [2] potrf's and potri's status
+    void *rb = nullptr;        // rocblas_handle
+    // after the first, each factorization runs on a side stream during the
+    // step it was started in and is taken up (converted to the FP32
+    // inverse) at the next rebuild; until then the cycles use the previous
+    // inverse
+    hipStream_t side = nullptr;
+    hipEvent_t ev_asm = nullptr, ev_fact = nullptr;
+    bool fact_pending = false;
+    bool direct_async = true;  // option "direct_async": 0 factors every time on the work stream
     unsigned *xcd_ids = nullptr;   // [16] set-up check of the XCD dispatch
     // learned coarse-level correction (chip_set_coarse_net): a 3-D CNN on the
     // coarsest level in place of its smoother call; BF16 weights

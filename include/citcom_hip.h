@@ -204,6 +204,10 @@ long chip_device_bytes(chip_ctx *c);
  * applied in FP32, in place of the smoother calls, transfers and maps of
  * that level and the ones below; 0 keeps the full hierarchy.  The plain
  * form always keeps it.  The solve still iterates to the same criterion.
+ * "direct_async": 1 (default) factors every rebuild's matrix after the
+ * first on a side stream while the step's solve goes on with the previous
+ * inverse (taken up at the next rebuild); 0 factors on the work stream and
+ * uses the new inverse at once.
  * "xcd_gs": 1 runs a whole gauss_seidel call on levels of 1025 to 6000
  * nodes as one launch of a few workgroups on one XCD, with barriers through
  * that XCD's L2 between the colour passes; 0 (default) launches one kernel
