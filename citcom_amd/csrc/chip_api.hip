@@ -512,18 +512,19 @@ int uzawa(chip_ctx *c, chip_stokes_stats *st) {
 // the FP32 cycles' exact coarsest-level solve (mixed forms): after every
 // rebuild, the level's stiffness assembled dense, factored (potrf) and
 // inverted (potri) in FP64, the inverse kept in FP32.  The first
-// factorization runs on the work stream and is used at once; later ones run
-// on a side stream while the step's solve goes on with the previous
-// inverse (an exact solve of the previous step's stiffness, a slightly
-// weaker coarse correction), and are taken up at the next rebuild.  A
-// factorization that fails (the matrix not positive definite in FP64)
-// leaves the cycles on the full hierarchy until one succeeds; the captured
-// multigrid graph follows the change.  The learned coarse-level correction,
-// when on, keeps the full hierarchy (it works on its coarsest level).
+// factorization runs on the work stream and is used at once; later ones
+// (assembly, factorization, inverse, FP32 conversion) run on a side stream
+// while the step's solve goes on with the previous inverse (an exact solve
+// of the previous step's stiffness, a slightly weaker coarse correction),
+// and are taken up at the next rebuild by one device copy.  A factorization
+// that fails (the matrix not positive definite in FP64) leaves the cycles
+// on the full hierarchy until one succeeds; the captured multigrid graph
+// follows the change.  The learned coarse-level correction, when on, keeps
+// the full hierarchy (it works on its coarsest level).
 
-// the pending side-stream factorization: wait for it, and when both its
-// steps succeeded convert it into the FP32 inverse on the work stream.
-// *state: 0 none was pending, 1 taken, 2 it failed
+// the pending side-stream factorization (already waited for by rebuild):
+// when both its steps succeeded, its FP32 inverse replaces the one the
+// cycles read.  *state: 0 none was pending, 1 taken, 2 it failed
 int direct_take(chip_ctx *c, int *state) {
     *state = 0;
     if (!c->fact_pending) return 0;
@@ -533,9 +534,8 @@ int direct_take(chip_ctx *c, int *state) {
     CHIP_CHECK(hipStreamSynchronize(c->side));
     *state = 2;
     if (info[0] != 0 || info[1] != 0) return 0;
-    CHIP_CHECK(hipStreamWaitEvent(S(c), c->ev_fact, 0));
-    sym_to_f32(S(c), c->dA, c->dAinv, 3 * c->L[c->dlev].g.nno);
-    CHIP_CHECK(hipGetLastError());
+    const size_t n = 3 * (size_t)c->L[c->dlev].g.nno;
+    CHIP_CHECK(hipMemcpyAsync(c->dAinv, c->dAstage, n * n * sizeof(float), hipMemcpyDeviceToDevice, S(c)));
     *state = 1;
     return 0;
 }
@@ -564,27 +564,31 @@ int direct_factor(chip_ctx *c) {
         }
         c->rb = h;
         CHIP_CHECK(hipStreamCreate(&c->side));
-        CHIP_CHECK(hipEventCreateWithFlags(&c->ev_asm, hipEventDisableTiming));
+        CHIP_CHECK(hipEventCreateWithFlags(&c->ev_k, hipEventDisableTiming));
         CHIP_CHECK(hipEventCreateWithFlags(&c->ev_fact, hipEventDisableTiming));
     }
     rocblas_handle h = (rocblas_handle)c->rb;
-    // dA is free: the taken factorization has been converted (same stream)
-    dense_assemble(S(c), L, c->dA);
     const bool async = c->direct_async && (was || taken);   // an inverse to go on with
     if (async) {
-        CHIP_CHECK(hipEventRecord(c->ev_asm, S(c)));
-        CHIP_CHECK(hipStreamWaitEvent(c->side, c->ev_asm, 0));
+        // the side stream reads this step's stiffness (rebuild waits for it
+        // before the next one overwrites it)
+        CHIP_CHECK(hipEventRecord(c->ev_k, S(c)));
+        CHIP_CHECK(hipStreamWaitEvent(c->side, c->ev_k, 0));
+        dense_assemble(c->side, L, c->dA);
         if (rocblas_set_stream(h, c->side) != rocblas_status_success ||
             rocsolver_dpotrf(h, rocblas_fill_lower, n, c->dA, n, c->dinfo) != rocblas_status_success ||
             rocsolver_dpotri(h, rocblas_fill_lower, n, c->dA, n, c->dinfo + 1) != rocblas_status_success) {
             set_error("coarse direct solve: rocSOLVER failed to launch");
             return -1;
         }
+        sym_to_f32(c->side, c->dA, c->dAstage, n);
+        CHIP_CHECK(hipGetLastError());
         CHIP_CHECK(hipEventRecord(c->ev_fact, c->side));
         c->fact_pending = true;
         c->direct_ok = true;
         return 0;
     }
+    dense_assemble(S(c), L, c->dA);
     int info = 0;
     if (rocblas_set_stream(h, S(c)) != rocblas_status_success ||
         rocsolver_dpotrf(h, rocblas_fill_lower, n, c->dA, n, c->dinfo) != rocblas_status_success) {
@@ -611,6 +615,9 @@ int direct_factor(chip_ctx *c) {
 
 int rebuild(chip_ctx *c) {
     hipStream_t q = S(c);
+    // a side-stream factorization still reading the stiffness finishes
+    // before the stiffness is rebuilt
+    if (c->fact_pending) CHIP_CHECK(hipStreamWaitEvent(q, c->ev_fact, 0));
     for (int lev = c->levmax; lev > c->levmin; lev--) project_viscosity(q, c->L[lev], c->L[lev - 1]);
     for (int lev = c->levmin; lev <= c->levmax; lev++) elt_g(q, c->L[lev]);
     for (int lev = c->levmax; lev >= c->levmin; lev--) node_ks(q, c->L[lev], c->p);
@@ -856,7 +863,7 @@ void chip_destroy(chip_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->rb) (void)rocblas_destroy_handle((rocblas_handle)c->rb);
-    if (c->ev_asm) (void)hipEventDestroy(c->ev_asm);
+    if (c->ev_k) (void)hipEventDestroy(c->ev_k);
     if (c->ev_fact) (void)hipEventDestroy(c->ev_fact);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->mg_graph) (void)hipGraphExecDestroy(c->mg_graph);

@@ -150,6 +150,7 @@ struct chip_ctx {
     bool direct_ok = false;
     double *dA = nullptr;
     float *dAinv = nullptr;
+    float *dAstage = nullptr;  // the side stream's FP32 inverse, copied to dAinv when taken up
     int *dinfo = nullptr;      // [2] potrf's and potri's status
     void *rb = nullptr;        // rocblas_handle
     // after the first, each factorization runs on a side stream during the
@@ -157,7 +158,7 @@ struct chip_ctx {
     // inverse) at the next rebuild; until then the cycles use the previous
     // inverse
     hipStream_t side = nullptr;
-    hipEvent_t ev_asm = nullptr, ev_fact = nullptr;
+    hipEvent_t ev_k = nullptr, ev_fact = nullptr;
     bool fact_pending = false;
     bool direct_async = true;  // option "direct_async": 0 factors every time on the work stream
     unsigned *xcd_ids = nullptr;   // [16] set-up check of the XCD dispatch

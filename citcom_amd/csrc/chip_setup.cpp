@@ -454,7 +454,8 @@ int build_mesh(chip_ctx *c) {
         for (int lev = c->levmax - 1; lev >= c->levmin + 2; lev--)
             if (3L * c->L[lev].g.nno <= DIRECT_MAX_DOF) {
                 const size_t n3 = 3 * (size_t)c->L[lev].g.nno;
-                if (dev_alloc(c, &c->dA, n3 * n3) || dev_alloc(c, &c->dAinv, n3 * n3) || dev_alloc(c, &c->dinfo, 2))
+                if (dev_alloc(c, &c->dA, n3 * n3) || dev_alloc(c, &c->dAinv, n3 * n3) ||
+                    dev_alloc(c, &c->dAstage, n3 * n3) || dev_alloc(c, &c->dinfo, 2))
                     return -1;
                 c->dlev = lev;
                 break;
