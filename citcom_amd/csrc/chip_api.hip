@@ -217,7 +217,7 @@ void enqueue_cycles(chip_ctx *c, V *F) {
     auto fl = [&](int lev) { return lev == levmax ? F : vecs<V>(L[lev]).fl; };
     auto bottom = [&](V *vel, const V *rhs) {
         if (direct)
-            dense_apply(S(c), c->dAinv, rhs, vel, 3 * L[levmin].g.nno);
+            dense_apply(S(c), c->dAh, c->dAs, rhs, vel, 3 * L[levmin].g.nno);
         else
             coarse_smooth(c, vel, rhs);
     };
@@ -511,9 +511,10 @@ int uzawa(chip_ctx *c, chip_stokes_stats *st) {
 
 // the FP32 cycles' exact coarsest-level solve (mixed forms): after every
 // rebuild, the level's stiffness assembled dense, factored (potrf) and
-// inverted (potri) in FP64, the inverse kept in FP32.  The first
+// inverted (potri) in FP64, the inverse kept as FP16 rows scaled by their
+// largest entry (to about 5e-4 of it).  The first
 // factorization runs on the work stream and is used at once; later ones
-// (assembly, factorization, inverse, FP32 conversion) run on a side stream
+// (assembly, factorization, inverse, conversion) run on a side stream
 // while the step's solve goes on with the previous inverse (an exact solve
 // of the previous step's stiffness, a slightly weaker coarse correction),
 // and are taken up at the next rebuild by one device copy.  A factorization
@@ -535,7 +536,8 @@ int direct_take(chip_ctx *c, int *state) {
     *state = 2;
     if (info[0] != 0 || info[1] != 0) return 0;
     const size_t n = 3 * (size_t)c->L[c->dlev].g.nno;
-    CHIP_CHECK(hipMemcpyAsync(c->dAinv, c->dAstage, n * n * sizeof(float), hipMemcpyDeviceToDevice, S(c)));
+    CHIP_CHECK(hipMemcpyAsync(c->dAh, c->dAh_stage, n * n * sizeof(uint16_t), hipMemcpyDeviceToDevice, S(c)));
+    CHIP_CHECK(hipMemcpyAsync(c->dAs, c->dAs_stage, n * sizeof(float), hipMemcpyDeviceToDevice, S(c)));
     *state = 1;
     return 0;
 }
@@ -581,7 +583,8 @@ int direct_factor(chip_ctx *c) {
             set_error("coarse direct solve: rocSOLVER failed to launch");
             return -1;
         }
-        sym_to_f32(c->side, c->dA, c->dAstage, n);
+        sym_to_f32(c->side, c->dA, c->dAinv, n);
+        rowscale_f16(c->side, c->dAinv, c->dAh_stage, c->dAs_stage, n);
         CHIP_CHECK(hipGetLastError());
         CHIP_CHECK(hipEventRecord(c->ev_fact, c->side));
         c->fact_pending = true;
@@ -608,6 +611,7 @@ int direct_factor(chip_ctx *c) {
     c->direct_ok = info == 0;
     if (c->direct_ok) {
         sym_to_f32(S(c), c->dA, c->dAinv, n);
+        rowscale_f16(S(c), c->dAinv, c->dAh, c->dAs, n);
         CHIP_CHECK(hipGetLastError());
     }
     return (c->direct_ok != was) ? drop_mg_graph(c) : 0;
@@ -1063,7 +1067,7 @@ int chip_apply(chip_ctx *c, const char *op, int lev, const double *in, const dou
             return -1;
         }
         CHIP_CHECK(up(b, in2, nv));
-        dense_apply(q, c->dAinv, (const double *)b, a, (int)nv);
+        dense_apply(q, c->dAh, c->dAs, (const double *)b, a, (int)nv);
         CHIP_CHECK(down(out, a, nv));
     } else if (o == "coarse") {
         if (lev != c->levmin || !L.Mc) {

@@ -144,13 +144,16 @@ struct chip_ctx {
     // exact solve on the FP32 cycles' coarsest level (mixed forms, option
     // "coarse_direct"): the level (-1: none), its stiffness as a dense FP64
     // matrix factored and inverted in place (rocSOLVER, column-major), the
-    // inverse in FP32 (row-major), the factorization's status
+    // inverse (row-major; applied in FP16 below), the factorization's status
     int dlev = -1;
     bool direct_on = true;
     bool direct_ok = false;
     double *dA = nullptr;
-    float *dAinv = nullptr;
-    float *dAstage = nullptr;  // the side stream's FP32 inverse, copied to dAinv when taken up
+    float *dAinv = nullptr;    // FP32 work buffer of the conversion
+    // the inverse the cycles apply: FP16 rows scaled by their largest entry
+    // (dAh, dAs), and the side stream's copies of both, copied in when taken up
+    uint16_t *dAh = nullptr, *dAh_stage = nullptr;
+    float *dAs = nullptr, *dAs_stage = nullptr;
     int *dinfo = nullptr;      // [2] potrf's and potri's status
     void *rb = nullptr;        // rocblas_handle
     // after the first, each factorization runs on a side stream during the

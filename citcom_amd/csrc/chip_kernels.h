@@ -65,11 +65,13 @@ void affine_apply(hipStream_t s, const Level &L, const V *din, const V *F, V *ou
 // exact coarsest-level solve of the FP32 cycles: A (column-major, n = 3 nno)
 // <- the level's stiffness with identity rows and columns at fixed
 // components (zeroed first); M (row-major FP32) <- the symmetric matrix
-// whose lower triangle A holds; out = M F
+// whose lower triangle A holds; H, sc <- M's rows scaled by their largest
+// entry, in FP16 (bits), and the scales; out = diag(sc) H F
 void dense_assemble(hipStream_t s, const Level &L, double *A);
 void sym_to_f32(hipStream_t s, const double *A, float *M, int n);
+void rowscale_f16(hipStream_t s, const float *M, uint16_t *H, float *sc, int n);
 template <typename V>
-void dense_apply(hipStream_t s, const float *M, const V *F, V *out, int n);
+void dense_apply(hipStream_t s, const uint16_t *H, const float *sc, const V *F, V *out, int n);
 // coarsest level: Mc = the v_steps_low-sweep smoother as a matrix, d = Mc F
 void coarse_build(hipStream_t s, const Level &L, int sweeps);
 template <typename V>

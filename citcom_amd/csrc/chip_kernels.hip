@@ -1143,7 +1143,8 @@ int xcd_colocated(hipStream_t s, int nwg, unsigned *ids_dev, bool *same) {
 // ---------------------------------------------------------------------------
 // Exact solve on the FP32 cycles' coarsest level (mixed forms): the level's
 // stiffness as a dense symmetric matrix, factored and inverted in FP64 by
-// rocSOLVER (potrf, potri) when the operators are rebuilt, applied in FP32.
+// rocSOLVER (potrf, potri) when the operators are rebuilt, applied as FP16
+// rows scaled by their largest entry (FP64 sums).
 // Fixed components (the rows strip3 zeroes) get an identity row and column,
 // so the matrix is SPD and the solve leaves them at the right-hand side's
 // value, which the restriction has already zeroed.
@@ -1190,21 +1191,39 @@ __global__ void __launch_bounds__(TPB) k_sym_to_f32(const double *__restrict__ A
     M[idx] = (float)A[lo + hi * (long)n];
 }
 
-// out = M F, one wavefront per row, FP64 sums in a fixed order (lane t the
-// terms j = t, t+64, ..., then a butterfly over the wavefront)
+// H, s <- M (row-major FP32) with each row scaled by its largest entry and
+// rounded to FP16 (entries within 1, half the bytes to apply), one
+// wavefront per row
+__global__ void __launch_bounds__(256) k_rowscale_f16(const float *__restrict__ M, _Float16 *__restrict__ H,
+                                                      float *__restrict__ s, int n) {
+    const int t = threadIdx.x & 63, i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    const float *row = M + (size_t)i * n;
+    float m = 0.0f;
+    for (int j = t; j < n; j += 64) m = fmaxf(m, fabsf(row[j]));
+#pragma unroll
+    for (int w = 1; w < 64; w *= 2) m = fmaxf(m, __shfl_xor(m, w, 64));
+    const float sc = m > 0.0f ? m : 1.0f;
+    for (int j = t; j < n; j += 64) H[(size_t)i * n + j] = (_Float16)(row[j] / sc);
+    if (t == 0) s[i] = sc;
+}
+
+// out = s M' F (M' the scaled FP16 rows), one wavefront per row, FP64 sums
+// in a fixed order (lane t the terms j = t, t+64, ..., then a butterfly
+// over the wavefront)
 template <typename V>
-__global__ void __launch_bounds__(256) k_dense_apply(const float *__restrict__ M, const V *__restrict__ F,
-                                                     V *__restrict__ out, int n) {
+__global__ void __launch_bounds__(256) k_dense_apply(const _Float16 *__restrict__ H, const float *__restrict__ s,
+                                                     const V *__restrict__ F, V *__restrict__ out, int n) {
     const int t = threadIdx.x & 63, i = blockIdx.x * 4 + (threadIdx.x >> 6);
     double S = 0.0;
     if (i < n) {
-        const float *row = M + (size_t)i * n;
+        const _Float16 *row = H + (size_t)i * n;
 #pragma unroll 8
-        for (int j = t; j < n; j += 64) S = S + (double)row[j] * (double)F[j];
+        for (int j = t; j < n; j += 64) S = S + (double)(float)row[j] * (double)F[j];
     }
 #pragma unroll
     for (int w = 1; w < 64; w *= 2) S = S + __shfl_xor(S, w, 64);
-    if (i < n && t == 0) out[i] = (V)S;
+    if (i < n && t == 0) out[i] = (V)(S * (double)s[i]);
 }
 
 void dense_assemble(hipStream_t s, const Level &L, double *A) {
@@ -1217,12 +1236,16 @@ void sym_to_f32(hipStream_t s, const double *A, float *M, int n) {
     hipLaunchKernelGGL(k_sym_to_f32, dim3((unsigned)(((long)n * n + TPB - 1) / TPB)), dim3(TPB), 0, s, A, M, n);
 }
 
-template <typename V>
-void dense_apply(hipStream_t s, const float *M, const V *F, V *out, int n) {
-    hipLaunchKernelGGL(k_dense_apply<V>, dim3((n + 3) / 4), dim3(256), 0, s, M, F, out, n);
+void rowscale_f16(hipStream_t s, const float *M, uint16_t *H, float *sc, int n) {
+    hipLaunchKernelGGL(k_rowscale_f16, dim3((n + 3) / 4), dim3(256), 0, s, M, (_Float16 *)H, sc, n);
 }
-template void dense_apply<double>(hipStream_t, const float *, const double *, double *, int);
-template void dense_apply<float>(hipStream_t, const float *, const float *, float *, int);
+
+template <typename V>
+void dense_apply(hipStream_t s, const uint16_t *H, const float *sc, const V *F, V *out, int n) {
+    hipLaunchKernelGGL(k_dense_apply<V>, dim3((n + 3) / 4), dim3(256), 0, s, (const _Float16 *)H, sc, F, out, n);
+}
+template void dense_apply<double>(hipStream_t, const uint16_t *, const float *, const double *, double *, int);
+template void dense_apply<float>(hipStream_t, const uint16_t *, const float *, const float *, float *, int);
 
 // multi_grid (v_steps_low sweeps from a zero guess, General_matrix_functions.c
 // :574, :613) is linear in its right-hand side apart from the FP32 rounding

@@ -455,7 +455,8 @@ int build_mesh(chip_ctx *c) {
             if (3L * c->L[lev].g.nno <= DIRECT_MAX_DOF) {
                 const size_t n3 = 3 * (size_t)c->L[lev].g.nno;
                 if (dev_alloc(c, &c->dA, n3 * n3) || dev_alloc(c, &c->dAinv, n3 * n3) ||
-                    dev_alloc(c, &c->dAstage, n3 * n3) || dev_alloc(c, &c->dinfo, 2))
+                    dev_alloc(c, &c->dAh, n3 * n3) || dev_alloc(c, &c->dAh_stage, n3 * n3) ||
+                    dev_alloc(c, &c->dAs, n3) || dev_alloc(c, &c->dAs_stage, n3) || dev_alloc(c, &c->dinfo, 2))
                     return -1;
                 c->dlev = lev;
                 break;

@@ -201,7 +201,8 @@ long chip_device_bytes(chip_ctx *c);
  * and at least two levels below it (17x17x9 nodes at 128x128x64 elements)
  * and solve it exactly: its stiffness is assembled dense, factored and
  * inverted in FP64 by rocSOLVER (potrf, potri) at every rebuild and
- * applied in FP32, in place of the smoother calls, transfers and maps of
+ * applied as FP16 rows scaled by their largest entry (FP64 sums), in
+ * place of the smoother calls, transfers and maps of
  * that level and the ones below; 0 keeps the full hierarchy.  The plain
  * form always keeps it.  The solve still iterates to the same criterion.
  * "direct_async": 1 (default) factors every rebuild's matrix after the

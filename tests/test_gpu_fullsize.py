@@ -205,7 +205,7 @@ def test_direct_coarse_level():
         worst = max(worst, float(np.linalg.norm(r) / np.linalg.norm(F)))
     say(f"direct solve on level {lev}: |K x - F| / |F| = {worst:.2e}")
     c.close()
-    assert worst < 1e-5   # measured 7.3e-8
+    assert worst < 2e-3   # measured 4.6e-4 (the inverse applied in FP16)
     # measured: 505 against 510 multigrid calls (FP32), 515 against 523
     # (FP16 stiffness); U 5.6e-6 and 6.8e-6, P 1.5e-3 and 6.5e-3 apart at
     # the 375-iteration cap, where no form has converged the pressure
