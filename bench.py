@@ -289,14 +289,15 @@ def run_form(p, T, form, steps, warmup, graphs, profile=False, same_state=False)
         # the passes measured do not share the device with it
         c.set_option("direct_async", 0)
         out["gs_levels"] = {}
-        for lev in range(p.levels):
-            c.set_option("profile_level", lev)
-            c.profile("gs_colour")
-            r = c.timestep()
-            n, ms, b = c.profile_read()
-            mg = max(r["stokes"]["mg_cycles"], 1)
-            if n:
-                out["gs_levels"][lev] = {"launches": n / mg, "ms": ms / mg, "bytes": b / mg}
+        for kind in ("gs_colour", "gs_box"):
+            for lev in range(p.levels):
+                c.set_option("profile_level", lev)
+                c.profile(kind)
+                r = c.timestep()
+                n, ms, b = c.profile_read()
+                mg = max(r["stokes"]["mg_cycles"], 1)
+                if n:
+                    out["gs_levels"][(kind, lev)] = {"launches": n / mg, "ms": ms / mg, "bytes": b / mg}
         c.set_option("profile_level", -1)
         c.profile("matvec")
         c.timestep()
@@ -457,43 +458,59 @@ def main():
     mixed = FORMS[args.form][0]
     levels = main_run["gs_levels"]
 
-    def passes(levs, name):
-        """colour passes of these levels, per multigrid call: launches, time,
-        algorithmic bytes, the rate they reach, and their counter traffic"""
-        n = sum(levels[lv]["launches"] for lv in levs)
-        ms = sum(levels[lv]["ms"] for lv in levs)
-        b = sum(levels[lv]["bytes"] for lv in levs)
+    def passes(keys, name):
+        """smoother launches (colour passes, box sweeps) of these (kind,
+        level) keys, per multigrid call: launches, time, algorithmic bytes,
+        the rate they reach, and their counter traffic"""
+        n = sum(levels[k]["launches"] for k in keys)
+        ms = sum(levels[k]["ms"] for k in keys)
+        b = sum(levels[k]["bytes"] for k in keys)
         ach = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-        return {"kernel": name, "levels": sorted(levs), "launches_per_mg_call": n, "ms_per_mg_call": ms,
-                "avg_launch_ms": ms / max(n, 1e-9), "algorithmic_bytes_per_launch": b / max(n, 1e-9),
-                "achieved": ach, "frac": ach / PEAK_HBM_GBS}
+        return {"kernel": name, "levels": sorted(lv for _, lv in keys), "launches_per_mg_call": n,
+                "ms_per_mg_call": ms, "avg_launch_ms": ms / max(n, 1e-9),
+                "algorithmic_bytes_per_launch": b / max(n, 1e-9), "achieved": ach, "frac": ach / PEAK_HBM_GBS}
 
-    # <vector type, stiffness type>: the FP16-stiffness form runs
-    # k_gs_colour<float, _Float16> on its two finest levels and
-    # k_gs_colour<float, float> below; the other forms one instantiation
-    fp16 = {lv for lv in levels if mixed == 2 and lv >= p.levels - 2 and lv >= 2}
-    groups = {"k_gs_colour<float, _Float16>": fp16,
+    # <vector type, stiffness type>: the FP16-stiffness form runs the FP16
+    # instantiations on its two finest levels and the FP32 ones below; the
+    # FP32 cycles smooth the levels between the exact level and the finest
+    # by boxes (k_gs_box, one launch per sweep), the finest by colour passes
+    fp16 = {lv for _, lv in levels if mixed == 2 and lv >= p.levels - 2 and lv >= 2}
+    col = {k for k in levels if k[0] == "gs_colour"}
+    box = {k for k in levels if k[0] == "gs_box"}
+    groups = {"k_gs_colour<float, _Float16>": {k for k in col if k[1] in fp16},
               {0: "k_gs_colour<double, float>", 1: "k_gs_colour<float, float>",
-               2: "k_gs_colour<float, float> (levels below the FP16 ones)"}[mixed]: set(levels) - fp16}
+               2: "k_gs_colour<float, float> (levels below the FP16 ones)"}[mixed]:
+                  {k for k in col if k[1] not in fp16},
+              "k_gs_box<_Float16>": {k for k in box if k[1] in fp16},
+              "k_gs_box<float>": {k for k in box if k[1] not in fp16}}
     per_kernel = {k: passes(v, k) for k, v in groups.items() if v}
-    if not per_kernel:
+    if not col:
         sys.exit("bench: no colour-pass launches to measure on this workload (every level is smoothed in one "
                  "workgroup); use a larger --config")
     for k, r in per_kernel.items():
-        r["traffic"] = measured_traffic(k, args.config)
+        # the counter summary's per-instantiation figures cover the level
+        # sets of a run without boxes; with boxes the colour passes are the
+        # finest level's alone
+        r["traffic"] = (measured_traffic(k, args.config, "finest_level_hbm_bytes_per_launch")
+                        if box and r["levels"] == [p.levels - 1] else measured_traffic(k, args.config))
         log(f"{k}: levels {r['levels']}, {r['launches_per_mg_call']:.0f} launches and {r['ms_per_mg_call']:.3f} ms "
             f"per multigrid call, {r['avg_launch_ms'] * 1e3:.2f} us each, {r['achieved']:.0f} GB/s")
-    # the dominant kernel: the colour pass, over all its instantiations and
-    # levels (per instantiation beside it)
-    allp = passes(set(levels), "k_gs_colour (every instantiation and level)")
-    allp["traffic"] = measured_traffic({0: "k_gs_colour<double, float>", 1: "k_gs_colour<float, float>",
-                                        2: "k_gs_colour<float, _Float16> (two finest levels) and <float, float>"}
-                                       [mixed], args.config)
-    fin = passes({p.levels - 1}, "k_gs_colour, finest level")
+    # the dominant kernel: the smoother, over all its launches (colour
+    # passes and box sweeps, every instantiation and level; per
+    # instantiation beside it)
+    if box:
+        allp = passes(col | box, "smoother: k_gs_colour and k_gs_box (every instantiation and level)")
+        allp["traffic"] = measured_traffic("k_gs_colour (finest level) and k_gs_box (levels below)", args.config)
+    else:
+        allp = passes(col, "k_gs_colour (every instantiation and level)")
+        allp["traffic"] = measured_traffic({0: "k_gs_colour<double, float>", 1: "k_gs_colour<float, float>",
+                                            2: "k_gs_colour<float, _Float16> (two finest levels) and <float, float>"}
+                                           [mixed], args.config)
+    fin = passes({("gs_colour", p.levels - 1)}, "k_gs_colour, finest level")
     fin_kernel = [k for k, r in per_kernel.items() if p.levels - 1 in r["levels"]]
     fin["traffic"] = measured_traffic(fin_kernel[0], args.config, "finest_level_hbm_bytes_per_launch") \
         if fin_kernel else None
-    log(f"all colour passes: {allp['achieved']:.0f} GB/s; finest level {fin['avg_launch_ms'] * 1e3:.1f} us each, "
+    log(f"all smoother launches: {allp['achieved']:.0f} GB/s; finest level {fin['avg_launch_ms'] * 1e3:.1f} us each, "
         f"{fin['achieved']:.0f} GB/s; matvec: {ms_mv:.1f} ms, {(bytes_mv / max(ms_mv, 1e-9)) / 1e6:.0f} GB/s")
     visc = (f"viscosity exp(-{p.viscE[0]:.4g} T)" + (f", {p.num_mat} layers (visc0 {p.visc0[:p.num_mat]})"
                                                       if p.num_mat > 1 else "")) if p.tdepv else "isoviscous"

@@ -172,6 +172,22 @@ void do_gs(chip_ctx *c, int lev, V *d, const V *F, V *Ad, int sweeps, int guess,
         gs_wg(S(c), L, BI, d, F, Ad, rhs, res, sweeps, guess, pg.a, pg.b);
         return;
     }
+    if constexpr (sizeof(V) == 4) {
+        if (c->box_gs && L.Kbox && sweeps > 0) {
+            // sweeps alternate between d and the level's scratch; a sweep
+            // from zero reads nothing, so it can write d itself
+            const float *src = guess == 1 ? d : nullptr;
+            float *dst = guess == 1 ? L.tmpf : d;
+            for (int cyc = 0; cyc < sweeps; cyc++) {
+                ProfGuard pg(c, 5, gs_bytes<V>(L) * L.g.nno, lev);
+                gs_box(S(c), L, BI, src, dst, F, pg.a, pg.b);
+                src = dst;
+                dst = dst == d ? L.tmpf : d;
+            }
+            do_matvec(c, lev, (const V *)src, Ad, rhs, res, src == d ? nullptr : d);
+            return;
+        }
+    }
     if (guess == 0) fill(S(c), d, V(0), 3L * L.g.nno);
     for (int cyc = 0; cyc < sweeps; cyc++)
         for (int col = 0; col < 8; col++) {
@@ -634,6 +650,7 @@ int rebuild(chip_ctx *c) {
         for (int lev = c->levmin; lev <= c->levmax; lev++) to_f32(q, c->L[lev].BIf, c->L[lev].BI, 3L * c->L[lev].g.nno);
     for (int lev = c->levmin; lev <= c->levmax; lev++)
         if (c->L[lev].Kh) scale_half(q, c->L[lev]);
+    for (int lev = c->levmin; lev <= c->levmax; lev++) box_tiles(q, c->L[lev]);
     CHIP_CHECK(hipGetLastError());
     return direct_factor(c);
 }
@@ -1051,6 +1068,31 @@ int chip_apply(chip_ctx *c, const char *op, int lev, const double *in, const dou
         CHIP_CHECK(down(out, a, nv));
         CHIP_CHECK(down(out2, d, nv));
         if (sync_dots(c) || xcd_status(c)) return -1;
+    } else if (o == "box0" || o == "box1") {   // FP32 box smoother call: arg sweeps, then Ad = K d
+        if (!L.Kbox || !L.velf) {
+            set_error("box: the level has no box smoother (FP32 cycles of a mixed form, levels between the "
+                      "exact or affine level and the finest)");
+            return -1;
+        }
+        const int guess = (o == "box1");
+        float *af = L.delf, *bf = L.resf, *df = L.AUf;
+        if (guess) {
+            CHIP_CHECK(up(a, in, nv));
+            to_f32(q, af, a, nv);
+        }
+        CHIP_CHECK(up(b, in2, nv));
+        to_f32(q, bf, b, nv);
+        const bool keep = c->box_gs;
+        c->box_gs = true;
+        do_gs(c, lev, af, (const float *)bf, df, arg, guess);
+        c->box_gs = keep;
+        fill(q, a, 0.0, nv);
+        add_f32(q, a, af, nv);
+        fill(q, d, 0.0, nv);
+        add_f32(q, d, df, nv);
+        CHIP_CHECK(down(out, a, nv));
+        CHIP_CHECK(down(out2, d, nv));
+        CHIP_CHECK(hipStreamSynchronize(q));
     } else if (o == "affine0" || o == "affine1") {
         if (!L.Ma) {
             set_error("affine: the level has no affine smoother form");
@@ -1228,7 +1270,7 @@ int chip_profile_enable(chip_ctx *c, const char *kernel, int on) {
         c->prof.kernel = 0;
         return 0;
     }
-    c->prof.kernel = (k == "gs_colour") ? 1 : (k == "matvec") ? 2 : (k == "gs_wg") ? 3 : (k == "gs_xcd") ? 4 : 0;
+    c->prof.kernel = (k == "gs_colour") ? 1 : (k == "matvec") ? 2 : (k == "gs_wg") ? 3 : (k == "gs_xcd") ? 4 : (k == "gs_box") ? 5 : 0;
     c->prof.launches = 0;
     c->prof.total_ms = c->prof.total_bytes = 0.0;
     if (!c->prof.kernel) {
@@ -1387,10 +1429,11 @@ int chip_set_option(chip_ctx *c, const char *name, int value) {
             }
         }
     }
-    if (n == "small_levels" || n == "gs_wg" || n == "coarse_net" || n == "xcd_gs") {
+    if (n == "small_levels" || n == "gs_wg" || n == "coarse_net" || n == "xcd_gs" || n == "box_gs") {
         (n == "small_levels" ? c->small_levels
          : n == "gs_wg"      ? c->gs_wg
          : n == "xcd_gs"     ? c->xcd_gs
+         : n == "box_gs"     ? c->box_gs
                              : c->cnet_on) = value != 0;
         if (c->mg_graph) {   // the captured kernel sequence changes
             CHIP_CHECK(hipStreamSynchronize(c->stream));

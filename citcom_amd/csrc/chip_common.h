@@ -45,6 +45,17 @@ struct Geo {
     int cz[8], cx[8]; // nodes of colour c along z and x
 };
 
+// A level cut into boxes of at most BOX nodes a side for the box smoother
+// of the FP32 cycles: s[a][t] .. s[a][t+1]-1 are the node indices of the
+// t-th segment along axis a (0 = z, 1 = x, 2 = y), n[a] segments.  A box of
+// at most 8 nodes a side holds at most 4x4x4 = 64 nodes of each colour.
+constexpr int BOX = 8;
+constexpr int BOX_MAXSEG = 40;
+struct BoxPart {
+    int n[3];
+    unsigned short s[3][BOX_MAXSEG + 1];
+};
+
 // arrival counter and XCD mask of a level's one-XCD smoother launches
 // (k_gs_xcd); zeroed once at set-up, never reset by the launches
 struct XcdBar {
@@ -88,6 +99,13 @@ struct Level {
     float *BIf = nullptr;
     // one-XCD smoother state (levels gs_xcd_ok accepts)
     XcdBar *xbar = nullptr;
+    // box smoother of the FP32 cycles: the partition, and the cycle's
+    // stiffness (Kh where the level has it, else K) re-tiled box by box,
+    // [nbox * 8 colours][243][64 nodes of the colour in the box]
+    BoxPart bp{};
+    int nbox = 0;
+    void *Kbox = nullptr;
+    bool Kbox_half = false;
 };
 
 struct Prof {
@@ -141,6 +159,9 @@ struct chip_ctx {
     bool small_levels = true;  // single-workgroup smoother on small levels
     bool gs_wg = true;         // ... including levels whose stiffness is read through L2
     bool xcd_gs = false;       // whole smoother calls of mid-size levels on one XCD (k_gs_xcd)
+    // FP32 cycles: the levels between the exact (or affine) level and the
+    // finest smoothed by boxes, one launch per sweep (option "box_gs")
+    bool box_gs = true;
     // exact solve on the FP32 cycles' coarsest level (mixed forms, option
     // "coarse_direct"): the level (-1: none), its stiffness as a dense FP64
     // matrix factored and inverted in place (rocSOLVER, column-major), the

@@ -54,6 +54,16 @@ int xcd_groups(const Level &L);
 template <typename V>
 void gs_xcd(hipStream_t s, const Level &L, const V *BI, V *d, const V *F, V *Ad, const V *rhs, V *res, int sweeps,
             int guess, double *status, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+// box smoother of the FP32 cycles: one sweep as one launch, a workgroup per
+// box of the level's partition; each box runs the 8 colours in order on its
+// own nodes (in LDS) and reads the other boxes' nodes from din, as they were
+// before the sweep, so the result does not depend on the order the boxes
+// run in.  din == nullptr: the sweep starts from zero.  Writes every node of
+// dout (dout != din).
+bool box_part(int nno_z, int nno_x, int nno_y, BoxPart &bp);
+void box_tiles(hipStream_t s, const Level &L);   // L.Kbox from L.Kh or L.K
+void gs_box(hipStream_t s, const Level &L, const float *BI, const float *din, float *dout, const float *F,
+            hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // set-up check: do the active blocks of an 8 nwg-block launch share one XCD
 // (four launches)?  ids_dev holds 16 unsigned
 int xcd_colocated(hipStream_t s, int nwg, unsigned *ids_dev, bool *same);

@@ -1139,6 +1139,219 @@ int xcd_colocated(hipStream_t s, int nwg, unsigned *ids_dev, bool *same) {
 }
 
 // ---------------------------------------------------------------------------
+// Box smoother of the FP32 cycles.  A level's nodes are cut into boxes of at
+// most BOX nodes a side; one workgroup takes one box, copies it with its
+// one-node halo from din into LDS, runs the 8 colour passes of the
+// node-block update on the box's own nodes in LDS (the update and row sums
+// of k_gs_colour), and writes the box to dout.  Inside a box each colour
+// sees the colours before it as updated; across a box face the neighbour's
+// values are those of din.  One launch replaces the 8 colour launches of a
+// sweep, whose cost on the small levels is launch latency, not bytes.
+
+// segments of at most BOX nodes, as equal as they come
+static bool axis_split(int n, int &nseg, unsigned short *s) {
+    nseg = (n + BOX - 1) / BOX;
+    if (nseg < 1 || nseg > BOX_MAXSEG) return false;
+    int pos = 0;
+    for (int t = 0; t < nseg; t++) {
+        s[t] = (unsigned short)pos;
+        pos += n / nseg + (t < n % nseg ? 1 : 0);
+    }
+    s[nseg] = (unsigned short)n;
+    return true;
+}
+
+bool box_part(int nz, int nx, int ny, BoxPart &bp) {
+    return axis_split(nz, bp.n[0], bp.s[0]) && axis_split(nx, bp.n[1], bp.s[1]) && axis_split(ny, bp.n[2], bp.s[2]);
+}
+
+// the nodes of colour c in box (bz, bx, by): first index and count per axis
+struct BoxColour {
+    int k0, i0, j0, nz, nx, ny;
+};
+__device__ __forceinline__ BoxColour box_colour(const BoxPart &bp, int bz, int bx, int by, int c) {
+    BoxColour r;
+    const int z0 = bp.s[0][bz], z1 = bp.s[0][bz + 1];
+    const int x0 = bp.s[1][bx], x1 = bp.s[1][bx + 1];
+    const int y0 = bp.s[2][by], y1 = bp.s[2][by + 1];
+    r.k0 = z0 + ((z0 ^ c) & 1);
+    r.i0 = x0 + ((x0 ^ (c >> 1)) & 1);
+    r.j0 = y0 + ((y0 ^ (c >> 2)) & 1);
+    r.nz = max(0, (z1 - r.k0 + 1) >> 1);
+    r.nx = max(0, (x1 - r.i0 + 1) >> 1);
+    r.ny = max(0, (y1 - r.j0 + 1) >> 1);
+    return r;
+}
+
+// colour-ordered index of node (k, i, j) of colour c (inverse of colour_kij)
+__device__ __forceinline__ int colour_index(const Geo &g, int c, int k, int i, int j) {
+    return g.cs[c] + (k >> 1) + g.cz[c] * ((i >> 1) + g.cx[c] * (j >> 1));
+}
+
+// Kbox[(b 8 + c)][ci][lane] = K[ktile(q)][ci] of the lane-th node of colour
+// c in box b (lanes past the colour's count hold zeros)
+template <typename KT>
+__global__ void __launch_bounds__(64) k_box_tiles(Geo g, BoxPart bp, const KT *__restrict__ K, KT *__restrict__ Kb) {
+    const int tile = blockIdx.x, lane = threadIdx.x;
+    const int b = tile >> 3, c = tile & 7;
+    const int bz = b % bp.n[0], bx = (b / bp.n[0]) % bp.n[1], by = b / (bp.n[0] * bp.n[1]);
+    const BoxColour bc = box_colour(bp, bz, bx, by, c);
+    KT *dst = Kb + (size_t)tile * (KROW * 64) + lane;
+    if (lane >= bc.nz * bc.nx * bc.ny) {
+        for (int ci = 0; ci < KROW; ci++) dst[ci * 64] = KT(0);
+        return;
+    }
+    const int kk = lane % bc.nz, t = lane / bc.nz;
+    const int k = bc.k0 + 2 * kk, i = bc.i0 + 2 * (t % bc.nx), j = bc.j0 + 2 * (t / bc.nx);
+    const KT *src = K + ktile(colour_index(g, c, k, i, j));
+    for (int ci = 0; ci < KROW; ci++) dst[ci * 64] = src[ci * 64];
+}
+
+void box_tiles(hipStream_t s, const Level &L) {
+    if (!L.Kbox) return;
+    const dim3 grid(8 * L.nbox);
+    if (L.Kbox_half)
+        hipLaunchKernelGGL(k_box_tiles<_Float16>, grid, dim3(64), 0, s, L.g, L.bp, (const _Float16 *)L.Kh,
+                           (_Float16 *)L.Kbox);
+    else
+        hipLaunchKernelGGL(k_box_tiles<float>, grid, dim3(64), 0, s, L.g, L.bp, (const float *)L.K, (float *)L.Kbox);
+}
+
+constexpr int BOXH = BOX + 2;   // box side with its halo
+static_assert(BOX <= 8, "a box holds at most 64 nodes of one colour");
+
+// One sweep.  Wavefront w forms y-plane w of the three rows of the colour's
+// (at most 64) nodes; the coefficients of the next colour are requested
+// before the workgroup barriers of this one, so their latency overlaps the
+// update.  ZIN: din is zero (not read).
+template <typename KT, bool ZIN>
+__global__ void __launch_bounds__(192) k_gs_box(Geo g, BoxPart bp, const KT *__restrict__ Kb,
+                                                const float *__restrict__ BI, const float *__restrict__ F,
+                                                const float *__restrict__ din, float *__restrict__ dout) {
+    __shared__ float sd[BOXH * BOXH * BOXH * 3];
+    __shared__ float sF[BOX * BOX * BOX * 3], sB[BOX * BOX * BOX * 3];   // the box's own F and BI
+    __shared__ float Ps[3][3][64];   // [plane][row][node]
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int b = blockIdx.x;
+    const int bz = b % bp.n[0], bx = (b / bp.n[0]) % bp.n[1], by = b / (bp.n[0] * bp.n[1]);
+    const int z0 = bp.s[0][bz], x0 = bp.s[1][bx], y0 = bp.s[2][by];
+    const int hz = bp.s[0][bz + 1] - z0 + 2, hx = bp.s[1][bx + 1] - x0 + 2, hy = bp.s[2][by + 1] - y0 + 2;
+    const int nz = hz - 2, nx = hx - 2, ny = hy - 2;
+    for (int t = threadIdx.x; t < nz * nx * ny; t += 192) {
+        const int lz = t % nz, r = t / nz, lx = r % nx, ly = r / nx;
+        const int n = nid(g, z0 + lz, x0 + lx, y0 + ly);
+#pragma unroll
+        for (int e = 0; e < 3; e++) {
+            sF[3 * t + e] = F[3 * n + e];
+            sB[3 * t + e] = BI[3 * n + e];
+        }
+    }
+    // the box and its halo; entries outside the mesh are zero and never read
+    for (int t = threadIdx.x; t < hz * hx * hy; t += 192) {
+        const int lz = t % hz, r = t / hz, lx = r % hx, ly = r / hx;
+        const int k = z0 - 1 + lz, i = x0 - 1 + lx, j = y0 - 1 + ly;
+        float v0 = 0.0f, v1 = 0.0f, v2 = 0.0f;
+        if (!ZIN && (unsigned)k < (unsigned)g.noz && (unsigned)i < (unsigned)g.nox && (unsigned)j < (unsigned)g.noy) {
+            const int n = nid(g, k, i, j);
+            v0 = din[3 * n];
+            v1 = din[3 * n + 1];
+            v2 = din[3 * n + 2];
+        }
+        sd[3 * t] = v0;
+        sd[3 * t + 1] = v1;
+        sd[3 * t + 2] = v2;
+    }
+    const KT *kb = Kb + (size_t)b * 8 * (KROW * 64) + lane;
+    float kv[81];
+#pragma unroll
+    for (int ci = 0; ci < 81; ci++) kv[ci] = (float)kb[(81 * w + ci) * 64];
+    __syncthreads();
+#pragma unroll 1
+    for (int c = 0; c < 8; c++) {
+        const BoxColour bc = box_colour(bp, bz, bx, by, c);
+        const bool act = lane < bc.nz * bc.nx * bc.ny;
+        int k = 0, i = 0, j = 0;
+        if (act) {
+            const int kk = lane % bc.nz, t = lane / bc.nz;
+            k = bc.k0 + 2 * kk;
+            i = bc.i0 + 2 * (t % bc.nx);
+            j = bc.j0 + 2 * (t / bc.nx);
+        }
+        const int lself = (k - z0 + 1) + hz * ((i - x0 + 1) + hx * (j - y0 + 1));
+        const int lown = (k - z0) + nz * ((i - x0) + nx * (j - y0));
+        if (act) {
+            // plane w: the 9 neighbours (dz, dx) at dy = w - 1
+            float uv[27];
+            bool okb[9];
+            const int dy = w - 1;
+            const bool oy = (unsigned)(j + dy) < (unsigned)g.noy;
+#pragma unroll
+            for (int dx = -1; dx <= 1; dx++) {
+                const bool ox = oy && (unsigned)(i + dx) < (unsigned)g.nox;
+#pragma unroll
+                for (int dz = -1; dz <= 1; dz++) {
+                    const int bb = (dz + 1) + 3 * (dx + 1);
+                    okb[bb] = ox && (unsigned)(k + dz) < (unsigned)g.noz;
+                    const int m = 3 * (lself + dz + hz * (dx + hx * dy));
+                    uv[3 * bb] = sd[m];
+                    uv[3 * bb + 1] = sd[m + 1];
+                    uv[3 * bb + 2] = sd[m + 2];
+                }
+            }
+            float out[3];
+            plane_sums<float>(kv, uv, okb, out);
+            Ps[w][0][lane] = out[0];
+            Ps[w][1][lane] = out[1];
+            Ps[w][2][lane] = out[2];
+        }
+        {   // the next colour's coefficients (the last colour's again after
+            // the last: unconditional, so no wait is placed on them here)
+            const KT *kn = kb + (size_t)(c < 7 ? c + 1 : 7) * (KROW * 64);
+#pragma unroll
+            for (int ci = 0; ci < 81; ci++) kv[ci] = (float)kn[(81 * w + ci) * 64];
+        }
+        __syncthreads();
+        if (act) {
+            const float Fr = sF[3 * lown + w], Br = sB[3 * lown + w];
+            const float o = (Ps[0][w][lane] + Ps[1][w][lane]) + Ps[2][w][lane];
+            // FP16 rows are stored scaled by BI: t = BI F - (BI K) d
+            const float t = sizeof(KT) == 2 ? Fr * Br - o : (Fr - o) * Br;
+            sd[3 * lself + w] += t;
+        }
+        __syncthreads();
+    }
+    // the box's own nodes
+    for (int t = threadIdx.x; t < nz * nx * ny; t += 192) {
+        const int lz = t % nz, r = t / nz, lx = r % nx, ly = r / nx;
+        const int n = nid(g, z0 + lz, x0 + lx, y0 + ly);
+        const int m = 3 * ((lz + 1) + hz * ((lx + 1) + hx * (ly + 1)));
+        dout[3 * n] = sd[m];
+        dout[3 * n + 1] = sd[m + 1];
+        dout[3 * n + 2] = sd[m + 2];
+    }
+}
+
+void gs_box(hipStream_t s, const Level &L, const float *BI, const float *din, float *dout, const float *F,
+            hipEvent_t e0, hipEvent_t e1) {
+    const dim3 grid(L.nbox), block(192);
+    auto launch = [&](auto kern, const auto *Kb) {
+        if (e0)
+            hipExtLaunchKernelGGL(kern, grid, block, 0, s, e0, e1, 0, L.g, L.bp, Kb, BI, F, din, dout);
+        else
+            hipLaunchKernelGGL(kern, grid, block, 0, s, L.g, L.bp, Kb, BI, F, din, dout);
+    };
+    if (L.Kbox_half) {
+        const _Float16 *Kb = (const _Float16 *)L.Kbox;
+        if (din) launch(k_gs_box<_Float16, false>, Kb);
+        else launch(k_gs_box<_Float16, true>, Kb);
+    } else {
+        const float *Kb = (const float *)L.Kbox;
+        if (din) launch(k_gs_box<float, false>, Kb);
+        else launch(k_gs_box<float, true>, Kb);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Coarsest level as a matrix.  The gauss_seidel call on the coarsest level of
 // ---------------------------------------------------------------------------
 // Exact solve on the FP32 cycles' coarsest level (mixed forms): the level's

@@ -461,6 +461,18 @@ int build_mesh(chip_ctx *c) {
                 c->dlev = lev;
                 break;
             }
+    // the FP32 cycles' box smoother: the levels between the exact (or the
+    // affine) level and the finest, larger than one workgroup's level
+    if (p.mixed)
+        for (int lev = c->levmax - 1; lev > c->levmin + 1 && lev > c->dlev; lev--) {
+            Level &L = c->L[lev];
+            if (L.g.nno <= 1024 || !box_part(L.g.noz, L.g.nox, L.g.noy, L.bp)) continue;
+            L.nbox = L.bp.n[0] * L.bp.n[1] * L.bp.n[2];
+            L.Kbox_half = L.Kh != nullptr;
+            const size_t n = (size_t)L.nbox * 8 * KROW * 64;
+            if (L.Kbox_half ? dev_alloc(c, (uint16_t **)&L.Kbox, n) : dev_alloc(c, (float **)&L.Kbox, n)) return -1;
+            if (!L.tmpf && dev_alloc(c, &L.tmpf, 3 * (size_t)L.g.nno)) return -1;
+        }
     const Geo &g = c->L[c->levmax].g;
     const size_t nno = g.nno, nel = g.nel, neq = 3 * nno;
     if (dev_alloc(c, &c->U, neq) || dev_alloc(c, &c->P, nel) || dev_alloc(c, &c->F, neq) ||

@@ -284,15 +284,16 @@ class Citcom:
     def apply(self, op: str, lev: int, x=None, x2=None, sweeps: int = 0):
         """Single operator on host arrays (see chip_apply in include/citcom_hip.h)."""
         n3, ne = 3 * self.nno(lev), self.nel(lev)
-        out_n = {"matvec": n3, "gs0": n3, "gs1": n3, "coarse": n3, "direct": n3, "affine0": n3, "affine1": n3,
+        out_n = {"matvec": n3, "gs0": n3, "gs1": n3, "box0": n3, "box1": n3, "coarse": n3, "direct": n3,
+                 "affine0": n3, "affine1": n3,
                  "div": ne, "grad": n3,
                  "strip": n3, "cnet": n3,
                  "project": 3 * self.nno(lev - 1) if lev > 0 else 0, "interp": n3}[op]
         # what chip_apply reads from x (in) and x2 (in2): it copies these many
         # doubles from the host pointers, so short or missing arrays are refused
-        n_in = {"matvec": n3, "gs1": n3, "affine1": n3, "div": n3, "grad": ne, "strip": n3,
+        n_in = {"matvec": n3, "gs1": n3, "box1": n3, "affine1": n3, "div": n3, "grad": ne, "strip": n3,
                 "project": n3, "interp": 3 * self.nno(lev - 1) if lev > 0 else 0}.get(op, 0)
-        n_in2 = {"gs0": n3, "gs1": n3, "coarse": n3, "direct": n3, "affine0": n3, "affine1": n3,
+        n_in2 = {"gs0": n3, "gs1": n3, "box0": n3, "box1": n3, "coarse": n3, "direct": n3, "affine0": n3, "affine1": n3,
                  "cnet": n3}.get(op, 0)
         for name, arr, need in (("x", x, n_in), ("x2", x2, n_in2)):
             if need and (arr is None or np.size(arr) != need):
@@ -304,7 +305,7 @@ class Citcom:
         b = np.ascontiguousarray(x2, dtype=np.float64) if x2 is not None else np.zeros(1)
         _check(lib().chip_apply(self.h, op.encode(), lev, a.ctypes.data, b.ctypes.data, out.ctypes.data,
                                 out2.ctypes.data, sweeps), f"apply {op}")
-        return (out, out2) if op in ("gs0", "gs1") else out
+        return (out, out2) if op in ("gs0", "gs1", "box0", "box1") else out
 
     # ---- measurement
     def profile(self, kernel: str | None):

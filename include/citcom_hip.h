@@ -117,6 +117,9 @@ int chip_construct_stiffness_B_matrix(chip_ctx *c);
  *     "gs0"/"gs1" gauss_seidel guess 0/1 with arg sweeps, 8-colour
  *         (General_matrix_functions.c:1160-1361); in = d0 (gs1), in2 = F,
  *         out = d0 after, out2 = A d0
+ *     "box0"/"box1" (levels with the box smoother of the FP32 cycles, mixed
+ *         forms) arg sweeps of the box smoother in FP32 from zero / from in,
+ *         then A d0: in, in2, out, out2 as for "gs0"/"gs1"
  *     "coarse" (coarsest level) the multigrid's coarsest-level smoother call
  *         (v_steps_low sweeps from zero) applied as its matrix: out = Mc in2
  *     "affine0"/"affine1" (the level above) its multigrid smoother call
@@ -180,8 +183,9 @@ int chip_general_stokes_solver(chip_ctx *c, chip_stokes_stats *st);
 
 /* ---- measurement ---------------------------------------------------------- */
 /* kernel: "gs_colour", "matvec", "gs_wg" (the one-workgroup smoother call
- * of small levels) or "gs_xcd" (the one-XCD smoother call of mid-size
- * levels).  While enabled, every launch of that kernel
+ * of small levels), "gs_xcd" (the one-XCD smoother call of mid-size
+ * levels) or "gs_box" (one sweep of the box smoother of the FP32
+ * cycles).  While enabled, every launch of that kernel
  * is bracketed by HIP events on the work stream; read returns the number of
  * launches, their summed duration and the algorithmic bytes they moved. */
 int chip_profile_enable(chip_ctx *c, const char *kernel, int on);
@@ -215,6 +219,14 @@ long chip_device_bytes(chip_ctx *c);
  * per colour pass there; results are identical.  Setting 1 first checks
  * that the device dispatches workgroups to its XCDs round-robin, and is
  * refused where it does not.
+ * "box_gs": 1 (default) smooths the levels of the mixed forms' FP32 cycles
+ * between the exact (or the affine) level and the finest, larger than 1024
+ * nodes, by boxes of at most 8 nodes a side: one launch per sweep, a
+ * workgroup per box running the 8 colour passes on its own nodes in LDS,
+ * the other boxes' nodes as they were before the sweep; 0 launches one
+ * kernel per colour pass there.  The iterates differ (a box's face sees its
+ * neighbour's values of the previous sweep); the solve still iterates to
+ * the same criterion.
  * "profile_level": >= 0 restricts chip_profile_enable to that level's
  * launches, -1 (default) profiles every level.
  * "coarse_net": 1 replaces the coarsest level's smoother call in multigrid

@@ -1,0 +1,137 @@
+"""The box smoother of the FP32 cycles (mixed forms) against a NumPy
+statement of it.
+
+The box smoother cuts a level into boxes of at most 8 nodes a side; a sweep
+runs the 8 colour passes of the node-block Gauss-Seidel update
+(General_matrix_functions.c:1160-1361, in the 8-colour order of the
+restatement) inside each box, where the colours before see their update,
+and reads the nodes of the other boxes as they were before the sweep.  The
+model below states that on the oracle's stiffness (bitwise the device's) in
+FP64; the device runs it in FP32, so the comparison is to a relative 2e-5.
+The same model with one box for the whole level is the plain 8-colour sweep,
+which the device result must NOT match to that tolerance: the test would
+otherwise not see whether the boxes are there.
+"""
+import numpy as np
+import pytest
+
+from conftest import problem
+
+pytestmark = pytest.mark.gpu
+
+BOX = 8
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+
+
+def axis_split(n):
+    """segment starts of at most BOX nodes, as equal as they come (the
+    library's partition)"""
+    nseg = (n + BOX - 1) // BOX
+    s, pos = [], 0
+    for t in range(nseg):
+        s.append(pos)
+        pos += n // nseg + (1 if t < n % nseg else 0)
+    return np.array(s + [n])
+
+
+def box_ids(n):
+    s = axis_split(n)
+    ids = np.zeros(n, dtype=np.int64)
+    for t in range(len(s) - 1):
+        ids[s[t]:s[t + 1]] = t
+    return ids
+
+
+def sweep_model(K, BI, F, d, dims, boxes=True, half=False):
+    """one sweep; K [nno][243] (row r, slot (dz+1)+3(dx+1)+9(dy+1), column),
+    node n = k + noz (i + nox j).  half: the rows scaled by BI and rounded
+    to FP16, the update BI F - (BI K) d (the mixed = 2 form's levels)"""
+    noz, nox, noy = dims
+    nno = noz * nox * noy
+    K = K.reshape(nno, 3, 27, 3)
+    if half:
+        K = (K * BI.astype(np.float32).reshape(nno, 3, 1, 1)).astype(np.float16)
+    K = K.astype(np.float64)
+    d = d.reshape(nno, 3).copy()
+    old = d.copy()
+    F = F.reshape(nno, 3)
+    BI = BI.reshape(nno, 3)
+    if boxes:
+        bz, bx, by = box_ids(noz), box_ids(nox), box_ids(noy)
+    else:
+        bz, bx, by = np.zeros(noz, int), np.zeros(nox, int), np.zeros(noy, int)
+    kk, ii, jj = np.meshgrid(np.arange(noz), np.arange(nox), np.arange(noy), indexing="ij")
+    for c in range(8):
+        sel = ((kk & 1) == (c & 1)) & ((ii & 1) == ((c >> 1) & 1)) & ((jj & 1) == ((c >> 2) & 1))
+        k, i, j = kk[sel], ii[sel], jj[sel]
+        n = k + noz * (i + nox * j)
+        Ad = np.zeros((n.size, 3))
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    s = (dz + 1) + 3 * (dx + 1) + 9 * (dy + 1)
+                    k2, i2, j2 = k + dz, i + dx, j + dy
+                    ok = (k2 >= 0) & (k2 < noz) & (i2 >= 0) & (i2 < nox) & (j2 >= 0) & (j2 < noy)
+                    k2c, i2c, j2c = np.clip(k2, 0, noz - 1), np.clip(i2, 0, nox - 1), np.clip(j2, 0, noy - 1)
+                    m = k2c + noz * (i2c + nox * j2c)
+                    same = (bz[k2c] == bz[k]) & (bx[i2c] == bx[i]) & (by[j2c] == by[j])
+                    u = np.where(same[:, None], d[m], old[m])
+                    Ad += np.where(ok[:, None], np.einsum("nrc,nc->nr", K[n, :, s, :], u), 0.0)
+        d[n] += F[n] * BI[n] - Ad if half else (F[n] - Ad) * BI[n]
+    return d.reshape(-1)
+
+
+@pytest.fixture(scope="module", params=[1, 2])
+def tdep64_mixed(request, pyoracle):
+    from citcom_amd import deck
+    from citcom_amd.hip import Citcom
+    p = problem("tdep64")
+    T = deck.perturbed_temperature(p)
+    c = Citcom(p, mixed=request.param)
+    c.initialize(T)
+    c.get_system_viscosity()
+    c.construct_stiffness_B_matrix()
+    o = pyoracle.Oracle(p, smoother=1)
+    o.set_viscosity(o.visc_from_T(T))
+    yield request.param, p, c, o
+    c.close()
+
+
+@pytest.mark.parametrize("lev", [2, 3])
+@pytest.mark.parametrize("guess,sweeps", [(0, 2), (0, 3), (1, 3), (1, 2)])
+def test_box_sweeps_match_model(tdep64_mixed, lev, guess, sweeps):
+    """mixed = 1: FP32 stiffness on both levels; mixed = 2: level 3 (the
+    second finest of 65^3 nodes) has the BI-scaled FP16 rows"""
+    mixed, p, c, o = tdep64_mixed
+    half = mixed == 2 and lev == p.levels - 2
+    if mixed == 2 and not half:
+        pytest.skip("the FP32 level is covered by mixed = 1")
+    nox, noy, noz = p.dims(lev)
+    dims = (noz, nox, noy)
+    nno = c.nno(lev)
+    rng = np.random.default_rng(100 * lev + 10 * guess + sweeps)
+    K = o.get("K", lev)
+    BI = o.get("BI", lev)
+    F = rng.standard_normal(3 * nno)
+    d0 = rng.standard_normal(3 * nno) * 1e-2 if guess else np.zeros(3 * nno)
+    say(f"box smoother mixed {mixed} level {lev} {dims} guess {guess} sweeps {sweeps}")
+    d, Ad = c.apply(f"box{guess}", lev, d0 if guess else None, F, sweeps=sweeps)
+    f32 = lambda v: np.asarray(v, dtype=np.float32).astype(np.float64)
+    ref = f32(d0)
+    plain = f32(d0)
+    for _ in range(sweeps):
+        ref = sweep_model(K, f32(BI), f32(F), ref, dims, boxes=True, half=half)
+        plain = sweep_model(K, f32(BI), f32(F), plain, dims, boxes=False, half=half)
+    scale = np.max(np.abs(ref))
+    err = np.max(np.abs(d - ref)) / scale
+    err_plain = np.max(np.abs(d - plain)) / scale
+    say(f"  against the box model {err:.2e}, against the plain 8-colour sweep {err_plain:.2e}")
+    assert err < 2e-5
+    assert err_plain > 100 * err
+    # Ad = K d (the matvec of the level after the sweeps; the FP16 rows'
+    # matvec is K d to their rounding)
+    Kd = o.matvec(lev, d)
+    assert np.max(np.abs(Ad - Kd)) / np.max(np.abs(Kd)) < (2e-3 if half else 2e-5)
