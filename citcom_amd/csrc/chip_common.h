@@ -101,7 +101,8 @@ struct Level {
     XcdBar *xbar = nullptr;
     // box smoother of the FP32 cycles: the partition, and the cycle's
     // stiffness (Kh where the level has it, else K) re-tiled box by box,
-    // [nbox * 8 colours][243][64 nodes of the colour in the box]
+    // per (box, colour) and y-plane each node's 81 coefficients in 16-byte
+    // groups (k_box_tiles)
     BoxPart bp{};
     int nbox = 0;
     void *Kbox = nullptr;

@@ -61,6 +61,7 @@ void gs_xcd(hipStream_t s, const Level &L, const V *BI, V *d, const V *F, V *Ad,
 // run in.  din == nullptr: the sweep starts from zero.  Writes every node of
 // dout (dout != din).
 bool box_part(int nno_z, int nno_x, int nno_y, BoxPart &bp);
+size_t box_tile_elems(const Level &L);           // size of L.Kbox (L.nbox, L.Kbox_half set)
 void box_tiles(hipStream_t s, const Level &L);   // L.Kbox from L.Kh or L.K
 void gs_box(hipStream_t s, const Level &L, const float *BI, const float *din, float *dout, const float *F,
             hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);

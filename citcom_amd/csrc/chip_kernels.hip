@@ -1188,23 +1188,44 @@ __device__ __forceinline__ int colour_index(const Geo &g, int c, int k, int i, i
     return g.cs[c] + (k >> 1) + g.cz[c] * ((i >> 1) + g.cx[c] * (j >> 1));
 }
 
-// Kbox[(b 8 + c)][ci][lane] = K[ktile(q)][ci] of the lane-th node of colour
-// c in box b (lanes past the colour's count hold zeros)
+// Box tiles: for the (box, colour) tile b 8 + c and y-plane P, each node's
+// 81 coefficients of that plane (K[ktile(q)][81 P + cc]) contiguous in
+// 16-byte groups of G = 4 (FP32) or 8 (FP16): [tile][P][cc / G][lane][cc % G],
+// so a wavefront reads its plane in 21 or 11 coalesced 16-byte loads per
+// lane rather than 81 four- or two-byte ones (a wavefront can have at most
+// 63 vector loads in flight).  Lanes past the colour's count and the
+// padding past cc = 80 hold zeros.
+template <typename KT>
+struct BoxTile {
+    static constexpr int G = 16 / (int)sizeof(KT);
+    static constexpr int NG = (81 + G - 1) / G;
+    static constexpr size_t PLANE = (size_t)NG * 64 * G;
+    static constexpr size_t TILE = 3 * PLANE;
+};
+
+size_t box_tile_elems(const Level &L) {
+    return (size_t)L.nbox * 8 * (L.Kbox_half ? BoxTile<_Float16>::TILE : BoxTile<float>::TILE);
+}
+
 template <typename KT>
 __global__ void __launch_bounds__(64) k_box_tiles(Geo g, BoxPart bp, const KT *__restrict__ K, KT *__restrict__ Kb) {
+    using BT = BoxTile<KT>;
     const int tile = blockIdx.x, lane = threadIdx.x;
     const int b = tile >> 3, c = tile & 7;
     const int bz = b % bp.n[0], bx = (b / bp.n[0]) % bp.n[1], by = b / (bp.n[0] * bp.n[1]);
     const BoxColour bc = box_colour(bp, bz, bx, by, c);
-    KT *dst = Kb + (size_t)tile * (KROW * 64) + lane;
-    if (lane >= bc.nz * bc.nx * bc.ny) {
-        for (int ci = 0; ci < KROW; ci++) dst[ci * 64] = KT(0);
-        return;
+    KT *dst = Kb + (size_t)tile * BT::TILE + lane * BT::G;
+    const bool act = lane < bc.nz * bc.nx * bc.ny;
+    const KT *src = K;
+    if (act) {
+        const int kk = lane % bc.nz, t = lane / bc.nz;
+        const int k = bc.k0 + 2 * kk, i = bc.i0 + 2 * (t % bc.nx), j = bc.j0 + 2 * (t / bc.nx);
+        src = K + ktile(colour_index(g, c, k, i, j));
     }
-    const int kk = lane % bc.nz, t = lane / bc.nz;
-    const int k = bc.k0 + 2 * kk, i = bc.i0 + 2 * (t % bc.nx), j = bc.j0 + 2 * (t / bc.nx);
-    const KT *src = K + ktile(colour_index(g, c, k, i, j));
-    for (int ci = 0; ci < KROW; ci++) dst[ci * 64] = src[ci * 64];
+    for (int P = 0; P < 3; P++)
+        for (int cc = 0; cc < BT::NG * BT::G; cc++)
+            dst[P * BT::PLANE + (size_t)(cc / BT::G) * 64 * BT::G + cc % BT::G] =
+                act && cc < 81 ? src[(81 * P + cc) * 64] : KT(0);
 }
 
 void box_tiles(hipStream_t s, const Level &L) {
@@ -1220,14 +1241,89 @@ void box_tiles(hipStream_t s, const Level &L) {
 constexpr int BOXH = BOX + 2;   // box side with its halo
 static_assert(BOX <= 8, "a box holds at most 64 nodes of one colour");
 
-// One sweep.  Wavefront w forms y-plane w of the three rows of the colour's
-// (at most 64) nodes; the coefficients of the next colour are requested
-// before the workgroup barriers of this one, so their latency overlaps the
-// update.  ZIN: din is zero (not read).
+// one plane's coefficients of one lane, as loaded (16-byte groups)
+template <typename KT>
+struct BoxCoefs {
+    uint4 q[BoxTile<KT>::NG];
+    __device__ __forceinline__ void load(const uint4 *__restrict__ p) {
+#pragma unroll
+        for (int t = 0; t < BoxTile<KT>::NG; t++) q[t] = p[t * 64];
+    }
+    __device__ __forceinline__ void widen(float kv[81]) const {
+        const KT *e = reinterpret_cast<const KT *>(q);
+#pragma unroll
+        for (int ci = 0; ci < 81; ci++) kv[ci] = (float)e[ci];
+    }
+};
+
+// One colour pass of a box: wavefront w forms y-plane w of the three rows
+// of the colour's (at most 64) nodes with the coefficients kc, then requests
+// the coefficients of the colour two ahead (kn) into kc before the workgroup
+// barriers, so their latency overlaps this colour's update and the next
+// colour's pass.
+template <typename KT>
+__device__ __forceinline__ void box_pass(const Geo &g, const BoxPart &bp, int bz, int bx, int by, int c, int z0,
+                                         int x0, int y0, int hz, int hx, int nz, int nx, float *sd,
+                                         const float *sF, const float *sB, float (*Ps)[3][64], BoxCoefs<KT> &kc,
+                                         const uint4 *__restrict__ kn) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const BoxColour bc = box_colour(bp, bz, bx, by, c);
+    const bool act = lane < bc.nz * bc.nx * bc.ny;
+    int k = 0, i = 0, j = 0;
+    if (act) {
+        const int kk = lane % bc.nz, t = lane / bc.nz;
+        k = bc.k0 + 2 * kk;
+        i = bc.i0 + 2 * (t % bc.nx);
+        j = bc.j0 + 2 * (t / bc.nx);
+    }
+    const int lself = (k - z0 + 1) + hz * ((i - x0 + 1) + hx * (j - y0 + 1));
+    const int lown = (k - z0) + nz * ((i - x0) + nx * (j - y0));
+    if (act) {
+        // plane w: the 9 neighbours (dz, dx) at dy = w - 1
+        float uv[27];
+        bool okb[9];
+        const int dy = w - 1;
+        const bool oy = (unsigned)(j + dy) < (unsigned)g.noy;
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) {
+            const bool ox = oy && (unsigned)(i + dx) < (unsigned)g.nox;
+#pragma unroll
+            for (int dz = -1; dz <= 1; dz++) {
+                const int bb = (dz + 1) + 3 * (dx + 1);
+                okb[bb] = ox && (unsigned)(k + dz) < (unsigned)g.noz;
+                const int m = 3 * (lself + dz + hz * (dx + hx * dy));
+                uv[3 * bb] = sd[m];
+                uv[3 * bb + 1] = sd[m + 1];
+                uv[3 * bb + 2] = sd[m + 2];
+            }
+        }
+        float kv[81], out[3];
+        kc.widen(kv);
+        plane_sums<float>(kv, uv, okb, out);
+        Ps[w][0][lane] = out[0];
+        Ps[w][1][lane] = out[1];
+        Ps[w][2][lane] = out[2];
+    }
+    // unconditional (the last colour's again near the end), so that no wait
+    // is placed on these loads before their use two passes later
+    kc.load(kn);
+    __syncthreads();
+    if (act) {
+        const float Fr = sF[3 * lown + w], Br = sB[3 * lown + w];
+        const float o = (Ps[0][w][lane] + Ps[1][w][lane]) + Ps[2][w][lane];
+        // FP16 rows are stored scaled by BI: t = BI F - (BI K) d
+        const float t = sizeof(KT) == 2 ? Fr * Br - o : (Fr - o) * Br;
+        sd[3 * lself + w] += t;
+    }
+    __syncthreads();
+}
+
+// One sweep, a workgroup per box.  ZIN: din is zero (not read).
 template <typename KT, bool ZIN>
 __global__ void __launch_bounds__(192) k_gs_box(Geo g, BoxPart bp, const KT *__restrict__ Kb,
                                                 const float *__restrict__ BI, const float *__restrict__ F,
                                                 const float *__restrict__ din, float *__restrict__ dout) {
+    using BT = BoxTile<KT>;
     __shared__ float sd[BOXH * BOXH * BOXH * 3];
     __shared__ float sF[BOX * BOX * BOX * 3], sB[BOX * BOX * BOX * 3];   // the box's own F and BI
     __shared__ float Ps[3][3][64];   // [plane][row][node]
@@ -1237,6 +1333,14 @@ __global__ void __launch_bounds__(192) k_gs_box(Geo g, BoxPart bp, const KT *__r
     const int z0 = bp.s[0][bz], x0 = bp.s[1][bx], y0 = bp.s[2][by];
     const int hz = bp.s[0][bz + 1] - z0 + 2, hx = bp.s[1][bx + 1] - x0 + 2, hy = bp.s[2][by + 1] - y0 + 2;
     const int nz = hz - 2, nx = hx - 2, ny = hy - 2;
+    // this wavefront's plane of colour c's tile, lane's first group
+    auto plane = [&](int c) {
+        return reinterpret_cast<const uint4 *>(Kb + ((size_t)b * 8 + c) * BT::TILE + w * BT::PLANE) + lane;
+    };
+    // the first two colours' coefficients, in flight while the box loads
+    BoxCoefs<KT> ka, kb;
+    ka.load(plane(0));
+    kb.load(plane(1));
     for (int t = threadIdx.x; t < nz * nx * ny; t += 192) {
         const int lz = t % nz, r = t / nz, lx = r % nx, ly = r / nx;
         const int n = nid(g, z0 + lz, x0 + lx, y0 + ly);
@@ -1261,64 +1365,11 @@ __global__ void __launch_bounds__(192) k_gs_box(Geo g, BoxPart bp, const KT *__r
         sd[3 * t + 1] = v1;
         sd[3 * t + 2] = v2;
     }
-    const KT *kb = Kb + (size_t)b * 8 * (KROW * 64) + lane;
-    float kv[81];
-#pragma unroll
-    for (int ci = 0; ci < 81; ci++) kv[ci] = (float)kb[(81 * w + ci) * 64];
     __syncthreads();
 #pragma unroll 1
-    for (int c = 0; c < 8; c++) {
-        const BoxColour bc = box_colour(bp, bz, bx, by, c);
-        const bool act = lane < bc.nz * bc.nx * bc.ny;
-        int k = 0, i = 0, j = 0;
-        if (act) {
-            const int kk = lane % bc.nz, t = lane / bc.nz;
-            k = bc.k0 + 2 * kk;
-            i = bc.i0 + 2 * (t % bc.nx);
-            j = bc.j0 + 2 * (t / bc.nx);
-        }
-        const int lself = (k - z0 + 1) + hz * ((i - x0 + 1) + hx * (j - y0 + 1));
-        const int lown = (k - z0) + nz * ((i - x0) + nx * (j - y0));
-        if (act) {
-            // plane w: the 9 neighbours (dz, dx) at dy = w - 1
-            float uv[27];
-            bool okb[9];
-            const int dy = w - 1;
-            const bool oy = (unsigned)(j + dy) < (unsigned)g.noy;
-#pragma unroll
-            for (int dx = -1; dx <= 1; dx++) {
-                const bool ox = oy && (unsigned)(i + dx) < (unsigned)g.nox;
-#pragma unroll
-                for (int dz = -1; dz <= 1; dz++) {
-                    const int bb = (dz + 1) + 3 * (dx + 1);
-                    okb[bb] = ox && (unsigned)(k + dz) < (unsigned)g.noz;
-                    const int m = 3 * (lself + dz + hz * (dx + hx * dy));
-                    uv[3 * bb] = sd[m];
-                    uv[3 * bb + 1] = sd[m + 1];
-                    uv[3 * bb + 2] = sd[m + 2];
-                }
-            }
-            float out[3];
-            plane_sums<float>(kv, uv, okb, out);
-            Ps[w][0][lane] = out[0];
-            Ps[w][1][lane] = out[1];
-            Ps[w][2][lane] = out[2];
-        }
-        {   // the next colour's coefficients (the last colour's again after
-            // the last: unconditional, so no wait is placed on them here)
-            const KT *kn = kb + (size_t)(c < 7 ? c + 1 : 7) * (KROW * 64);
-#pragma unroll
-            for (int ci = 0; ci < 81; ci++) kv[ci] = (float)kn[(81 * w + ci) * 64];
-        }
-        __syncthreads();
-        if (act) {
-            const float Fr = sF[3 * lown + w], Br = sB[3 * lown + w];
-            const float o = (Ps[0][w][lane] + Ps[1][w][lane]) + Ps[2][w][lane];
-            // FP16 rows are stored scaled by BI: t = BI F - (BI K) d
-            const float t = sizeof(KT) == 2 ? Fr * Br - o : (Fr - o) * Br;
-            sd[3 * lself + w] += t;
-        }
-        __syncthreads();
+    for (int c = 0; c < 8; c += 2) {
+        box_pass<KT>(g, bp, bz, bx, by, c, z0, x0, y0, hz, hx, nz, nx, sd, sF, sB, Ps, ka, plane(min(c + 2, 7)));
+        box_pass<KT>(g, bp, bz, bx, by, c + 1, z0, x0, y0, hz, hx, nz, nx, sd, sF, sB, Ps, kb, plane(min(c + 3, 7)));
     }
     // the box's own nodes
     for (int t = threadIdx.x; t < nz * nx * ny; t += 192) {

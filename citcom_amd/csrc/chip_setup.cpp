@@ -469,7 +469,7 @@ int build_mesh(chip_ctx *c) {
             if (L.g.nno <= 1024 || !box_part(L.g.noz, L.g.nox, L.g.noy, L.bp)) continue;
             L.nbox = L.bp.n[0] * L.bp.n[1] * L.bp.n[2];
             L.Kbox_half = L.Kh != nullptr;
-            const size_t n = (size_t)L.nbox * 8 * KROW * 64;
+            const size_t n = box_tile_elems(L);
             if (L.Kbox_half ? dev_alloc(c, (uint16_t **)&L.Kbox, n) : dev_alloc(c, (float **)&L.Kbox, n)) return -1;
             if (!L.tmpf && dev_alloc(c, &L.tmpf, 3 * (size_t)L.g.nno)) return -1;
         }
