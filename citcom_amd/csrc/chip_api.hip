@@ -576,7 +576,10 @@ int direct_factor(chip_ctx *c) {
     const int n = 3 * L.g.nno;
     if (!c->rb) {
         rocblas_handle h;
-        if (rocblas_create_handle(&h) != rocblas_status_success) {
+        // no atomics in the library's algorithms: they would make the
+        // factorization's last bits depend on timing
+        if (rocblas_create_handle(&h) != rocblas_status_success ||
+            rocblas_set_atomics_mode(h, rocblas_atomics_not_allowed) != rocblas_status_success) {
             set_error("coarse direct solve: rocblas_create_handle failed");
             return -1;
         }

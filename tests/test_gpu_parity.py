@@ -317,8 +317,13 @@ def test_one_xcd_smoother_same_bits(name, mixed):
     passes separated by barriers through that XCD's L2) gives the per-colour
     launches' iterate bit for bit, in the FP64, FP32 and FP16-stiffness
     forms: the step-0 solve (Uzawa capped at 20 iterations) and two
-    timesteps agree exactly with the kernel on and off.  The FP32 cycles'
-    box smoother, which would otherwise take tdep64's level, is off."""
+    timesteps agree exactly with the kernel on and off.  In the FP32 forms
+    the box smoother, which would otherwise take tdep64's level, and the
+    exact coarse level, which would otherwise take tdep128's, are off: the
+    one-XCD kernel smooths those levels, and the runs do not depend on
+    whether rocSOLVER's factorization of the exact level gives the same
+    last bits twice (once, at tdep128 with the FP16 stiffness, two runs
+    differed in U's last bits with the same multigrid calls)."""
     from citcom_amd import deck
     from citcom_amd.hip import Citcom
     runs = []
@@ -328,6 +333,7 @@ def test_one_xcd_smoother_same_bits(name, mixed):
         c = Citcom(p, mixed=mixed)
         if mixed:
             c.set_option("box_gs", 0)
+            c.set_option("coarse_direct", 0)
         c.set_option("xcd_gs", on)
         c.initialize(deck.seeded_temperature(p, seed=5))
         st = c.general_stokes_solver()
