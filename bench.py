@@ -210,8 +210,10 @@ FORMS = {
     # name: (mixed, warm_start, description)
     "plain": (0, 0, "the reference method in FP64 arithmetic (FP64 vectors, the reference's FP32 stiffness), each "
                     "Uzawa solve started from the previous solution as the reference does"),
-    "mixed": (1, 0, "FP32 multigrid cycles inside FP64 iterative refinement (FP64 residual, Uzawa CG, coarsest "
-                    "level), previous-solution start"),
+    "mixed": (1, 0, "FP32 multigrid cycles inside FP64 iterative refinement (FP64 residual and Uzawa CG); the "
+                    "cycles solve the 17x17x9-node level exactly (its inverse factored by rocSOLVER) and smooth the "
+                    "levels between it and the finest by 8-node boxes, one launch per sweep; previous-solution "
+                    "start"),
     "mixed16": (2, 0, "as mixed, the two finest levels' stiffness rows scaled by BI and stored in FP16 for the cycles "
                       "(FP32 arithmetic; the FP64 residual keeps the reference's FP32 stiffness), previous-solution "
                       "start"),
@@ -227,8 +229,9 @@ FORMS = {
 def run_form(p, T, form, steps, warmup, graphs, profile=False, same_state=False):
     """Timestep loop of one form on its own trajectory from the seeded state:
     the step-0 solve and `warmup` steps untimed, then `steps` timed steps.
-    profile: one more step with kernel-stamped events on every colour pass,
-    one on every matvec, one on the finest level's colour passes.
+    profile: one more step per level and smoother kernel (colour passes,
+    box sweeps) with kernel-stamped events on its launches, then one on
+    every matvec.
     same_state: one more step solved twice from the same state (same
     temperature, same previous solutions): with this form's initial guess and
     with the previous solution as it is (the reference's start)."""
