@@ -211,9 +211,9 @@ FORMS = {
     "plain": (0, 0, "the reference method in FP64 arithmetic (FP64 vectors, the reference's FP32 stiffness), each "
                     "Uzawa solve started from the previous solution as the reference does"),
     "mixed": (1, 0, "FP32 multigrid cycles inside FP64 iterative refinement (FP64 residual and Uzawa CG); the "
-                    "cycles solve the 17x17x9-node level exactly (its inverse factored by rocSOLVER) and smooth the "
-                    "levels between it and the finest by 8-node boxes, one launch per sweep; previous-solution "
-                    "start"),
+                    "cycles solve a small level exactly where there is one (17x17x9 nodes at 128x128x64 elements; "
+                    "its inverse factored by rocSOLVER) and smooth the levels between it and the finest by 8-node "
+                    "boxes, one launch per sweep; previous-solution start"),
     "mixed16": (2, 0, "as mixed, the two finest levels' stiffness rows scaled by BI and stored in FP16 for the cycles "
                       "(FP32 arithmetic; the FP64 residual keeps the reference's FP32 stiffness), previous-solution "
                       "start"),
